@@ -1,0 +1,168 @@
+"""Throughput of the plain LJ step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Workload: N = 2,048,000 particles on an 80^3-cell FCC lattice (a = 2.0,
+L = 160, rho = 0.5), Maxwell momenta at T = 1 (seed 2), quadratically
+shifted LJ at rc = 3.0, search length 3.3, dt = 0.001, FP64 throughout.
+A step is MDManager::Calculate (mdmanager.cc:154-186): pairlist expiry
+check, rebuild when expired, half drift, image refresh, force kick, half
+drift.  State is resident in HBM before the timed region; the timed region
+is exactly K steps between two device synchronisations, rebuilds included.
+
+Prints one JSON line.  `roofline` covers the force kick (the dominant
+kernel): algorithmic bytes per launch over its mean launch time from HIP
+events recorded on the library's stream during the timed region.
+`cpu_baseline` is the reference's own CPU path (oracle/_ref, compiled from
+the reference sources) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "atom-steps/s at N=2.05M LJ (GPU vs host CPU); fwd+adjoint step/s; % HBM peak"
+WORKLOAD = ("plain LJ, FCC 80^3 cells N=2048000, L=160, rho=0.5, T=1 (seed 2), "
+            "rc=3.0, search 3.3, dt=0.001, fp64, drift-kick-drift with pairlist expiry")
+
+
+def physical_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(state, steps: int, warmup: int):
+    """Reference CPU path on the host cores: `steps` timed Calculate() calls
+    after a pairlist build and `warmup` steps (setup excluded)."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    from mdacp_amd import workloads as W
+    cores = min(physical_cores(), 16)
+    threads = 1
+    while threads * 2 <= cores:
+        threads *= 2
+    # one domain per thread holds at most 1,000,000 particles (mdconfig.h:11)
+    threads = max(threads, 4)
+    if os.path.exists(harness):
+        with tempfile.TemporaryDirectory() as tmp:
+            path = os.path.join(tmp, "state.bin")
+            W.write_state(path, state)
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+                       OMP_PLACES="cores")
+            r = subprocess.run([harness, "bench", path, os.path.join(tmp, "o"), str(steps),
+                                str(warmup)], env=env, capture_output=True, text=True, timeout=600)
+            if r.returncode == 0:
+                line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")][-1]
+                res = json.loads(line)
+                return {"value": res["atom_steps_per_s"], "unit": "atom-steps/s",
+                        "cores": threads, "kind": "reference",
+                        "sample": f"{steps} timed steps after 1 build + {warmup} warm-up steps of "
+                                  f"the same N=2048000 workload, {threads} OpenMP domains "
+                                  f"(MDACP AVX2 build), {cpu_model()}"}
+            sys.stderr.write(f"reference harness failed ({r.returncode}): {r.stderr[-400:]}\n")
+    # restated path (single thread)
+    from oracle import pyoracle as O
+    o = O.Oracle(state)
+    o.make_pairlist()
+    o.calculate(warmup)
+    t0 = time.perf_counter()
+    o.calculate(steps)
+    sec = time.perf_counter() - t0
+    return {"value": state.n * steps / sec, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} timed steps of the same workload, single thread, {cpu_model()}"}
+
+
+def traffic_from_profile():
+    path = os.path.join(ROOT, "profiles", "kick_traffic_fcc2048000.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=300)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the multi-GPU sharding is a design, not code)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from mdacp_amd import workloads as W
+    from mdacp_amd.mdx import System
+
+    state = W.plain_full()
+    n = state.n
+    sys_ = System.from_state(state, device=0)
+    sys_.make_pairlist()
+    sys_.calculate(args.warmup)
+    sys_.timing_reset(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rebuilds = sys_.calculate(args.steps)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    tm = sys_.timing()
+    kick_ms, kick_bytes = sys_.kick_profile()
+    pairs = sys_.pair_number()
+    value = n * args.steps / sec
+    achieved = kick_bytes / (kick_ms * 1e-3) / 1e9
+    traffic = traffic_from_profile()
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD, "particles": n, "pairs_within_search": pairs,
+                   "rebuilds_in_timed_region": rebuilds},
+        "split_ms": {"pairlist_build_total": tm["build_ms"], "force_kick_total": tm["force_ms"],
+                     "other_step_kernels_total": tm["other_ms"], "builds": tm["builds"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_kick<0>", "avg_launch_ms": kick_ms,
+                     "algorithmic_bytes_per_launch": kick_bytes},
+    }
+    sys_.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(state, args.cpu_steps, 2)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,123 @@
+/*
+ * mdx.h — C ABI of libmdx, the MI355X (gfx950) short-range Lennard-Jones
+ * step of MDACP: image construction, cell binning, pairlist build, force
+ * kick with virial / potential energy, half drift and pairlist expiry, all
+ * resident in HBM of one GPU.
+ *
+ * The reference has no C ABI for this path; its operator interface is the
+ * C++ triple that MDUnit calls (mdunit.h:59-102).  Each entry point below
+ * names the reference call it replaces.  The C++ mirror of that interface
+ * (mdacp_amd/host/mdacp_gpu.h: Variables, MeshList, ForceCalculator,
+ * PairList, observers, MDUnit) is built on these functions.
+ *
+ * Conventions
+ *   - Every function returns an int status: 0 = MDX_OK, negative = error.
+ *     Nothing calls exit(); mdx_last_error() gives the message.  The fatal
+ *     exits of the reference map onto status codes (see enum mdx_status).
+ *   - Host arrays are caller-owned, [n][3] doubles in particle-id order
+ *     (id = position in the array given to mdx_set_state).  On the device
+ *     particles are kept in cell order; ids travel with them.
+ *   - All device work is ordered on the handle's HIP stream.  A handle is
+ *     not thread-safe; independent handles are.
+ *   - Units: sigma = epsilon = m = k_B = 1 (hard-wired in the reference,
+ *     variables.cc:16-33, mdconfig.h:31).
+ */
+#ifndef MDX_H
+#define MDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mdx_system mdx_system;
+
+typedef struct mdx_params {
+  double L[3];            /* SimulationInfo::L (simulationinfo.cc:31-49) */
+  double dt;              /* SimulationInfo::TimeStep (simulationinfo.cc:9) */
+  double cutoff;          /* CUTOFF_LENGTH (mdconfig.h:31), 3.0 */
+  double buffer_length;   /* SimulationInfo::BufferLength (simulationinfo.cc:6), 0.3 */
+  int check_list_length;  /* SimulationInfo::CheckListLength (simulationinfo.cc:29), 500 */
+  int capacity;           /* max particles; 0 = sized by the first mdx_set_state */
+  int device;             /* HIP device ordinal */
+  void *stream;           /* hipStream_t to order work on; NULL = own stream */
+} mdx_params;
+
+enum mdx_status {
+  MDX_OK = 0,
+  MDX_ERR_ARG = -1,          /* bad argument */
+  MDX_ERR_CAPACITY = -2,     /* exceeds capacity (variables.cc:46, meshlist.cc:582) */
+  MDX_ERR_INVALID_POS = -3,  /* "Invalid position" (variables.cc:77-81) or non-finite */
+  MDX_ERR_TOO_FAST = -4,     /* "Too fast particles" (pairlist.cc:59-63) */
+  MDX_ERR_BOX = -5,          /* box edge < 2 search lengths: images would alias */
+  MDX_ERR_HIP = -6,          /* HIP runtime error */
+  MDX_ERR_STATE = -7         /* call out of order (e.g. force before pairlist) */
+};
+
+/* Fills the reference defaults (dt 0.001, cutoff 3.0, buffer 0.3, 500). */
+void mdx_default_params(mdx_params *prm);
+
+/* MDUnit::MDUnit (mdunit.cc:8-39): Variables + MeshList + PairList of one
+ * periodic domain, here one GPU-resident system. */
+int mdx_create(const mdx_params *prm, mdx_system **out);
+/* MDUnit::~MDUnit (mdunit.cc:41-49) */
+int mdx_destroy(mdx_system *sys);
+/* message of the last failure on sys (or of mdx_create when sys is NULL) */
+const char *mdx_last_error(const mdx_system *sys);
+
+/* Variables::AddParticle for all n particles at once (variables.cc:36-47);
+ * resets the pairlist (PairList ctor, pairlist.h:25). */
+int mdx_set_state(mdx_system *sys, int n, const double *q3, const double *p3);
+/* Variables::q / p read back, ordered by particle id; either may be NULL. */
+int mdx_get_state(mdx_system *sys, double *q3, double *p3);
+/* Variables::GetParticleNumber / GetTotalParticleNumber (variables.h:25-26) */
+int mdx_particle_number(const mdx_system *sys, int *pn, int *tn);
+
+/* MDManager::MakePairList (mdmanager.cc:388-417): wrap into the box
+ * (variables.cc:64-84), build periodic images on the device (replacing
+ * mdmanager.cc:395-401), bin by counting sort (meshlist.cc:176-228), build
+ * the neighbour list (meshlist.cc:68-104, 231-415) and snapshot positions
+ * (PairList::Init, pairlist.cc:11-23). */
+int mdx_make_pairlist(mdx_system *sys);
+/* PairList::IsPairListExpired (pairlist.cc:26-121) */
+int mdx_is_pairlist_expired(mdx_system *sys, int *expired);
+/* ForceCalculator::UpdatePositionHalf (fcalculator.cc:18-30) */
+int mdx_update_position_half(mdx_system *sys);
+/* MDManager::SendBorderParticles (mdmanager.cc:420-429): image refresh */
+int mdx_refresh_images(mdx_system *sys);
+/* ForceCalculator::CalculateForce (fcalculator.cc:33-50): p += F dt */
+int mdx_calculate_force(mdx_system *sys);
+/* MDManager::Calculate (mdmanager.cc:154-186), NVE, nsteps times;
+ * *rebuilds (may be NULL) receives the number of pairlist constructions. */
+int mdx_calculate(mdx_system *sys, int nsteps, int *rebuilds);
+
+/* KineticEnergyObserver / PotentialEnergyObserver / VirialObserver
+ * (observer.cc:5-79): sums over the system (not per particle); virial is
+ * sum r V'(r) / 3.  Any pointer may be NULL. */
+int mdx_observe(mdx_system *sys, double *ke, double *pe, double *virial);
+/* Forces F_i = sum_j f_ij of the current list, [n][3] by id; p untouched
+ * (the reference's kick at dt = 1 on p = 0, fcalculator.cc:182-223). */
+int mdx_forces(mdx_system *sys, double *f3);
+/* number of distinct pairs within the search length (each pair once,
+ * images resolved), i.e. the canonical pair set's size */
+int mdx_pair_number(mdx_system *sys, int64_t *count);
+/* canonical pair set: sorted (id_lo, id_hi, image code) int32 triples,
+ * code = (sx+1) + 3(sy+1) + 9(sz+1) for the partner image q_hi + s.L.
+ * Writes min(cap, count) triples; *count receives the full count. */
+int mdx_pairs_canonical(mdx_system *sys, int32_t *out, int64_t cap, int64_t *count);
+
+/* cumulative device time (ms) of the pairlist build, force kick and
+ * remaining step kernels, and counts, measured with HIP events since the
+ * last reset; any pointer may be NULL. */
+int mdx_timing(mdx_system *sys, double *build_ms, double *force_ms, double *other_ms,
+               int64_t *builds, int64_t *steps);
+int mdx_timing_reset(mdx_system *sys, int enable);
+/* average device time (ms) of one force-kick launch, and algorithmic HBM
+ * bytes of that launch (positions, momenta, neighbour indices) */
+int mdx_kick_profile(mdx_system *sys, double *avg_ms, double *bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

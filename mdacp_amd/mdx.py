@@ -1,0 +1,211 @@
+"""ctypes binding of libmdx (include/mdx.h).
+
+The library must exist: there is no CPU fallback.  Importing this module
+does not touch the GPU; creating a :class:`System` does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdx.so")
+
+EXPORTS = (
+    "mdx_default_params", "mdx_create", "mdx_destroy", "mdx_last_error", "mdx_set_state",
+    "mdx_get_state", "mdx_particle_number", "mdx_make_pairlist", "mdx_is_pairlist_expired",
+    "mdx_update_position_half", "mdx_refresh_images", "mdx_calculate_force", "mdx_calculate",
+    "mdx_observe", "mdx_forces", "mdx_pair_number", "mdx_pairs_canonical", "mdx_timing",
+    "mdx_timing_reset", "mdx_kick_profile",
+)
+
+STATUS = {0: "MDX_OK", -1: "MDX_ERR_ARG", -2: "MDX_ERR_CAPACITY", -3: "MDX_ERR_INVALID_POS",
+          -4: "MDX_ERR_TOO_FAST", -5: "MDX_ERR_BOX", -6: "MDX_ERR_HIP", -7: "MDX_ERR_STATE"}
+
+
+class MdxParams(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_double * 3), ("dt", ctypes.c_double), ("cutoff", ctypes.c_double),
+                ("buffer_length", ctypes.c_double), ("check_list_length", ctypes.c_int),
+                ("capacity", ctypes.c_int), ("device", ctypes.c_int), ("stream", ctypes.c_void_p)]
+
+
+class MdxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libmdx.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m mdacp_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    vp, dp, ip, i64p = ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_int), P(ctypes.c_int64)
+    sig = {
+        "mdx_default_params": (None, [P(MdxParams)]),
+        "mdx_create": (ctypes.c_int, [P(MdxParams), P(vp)]),
+        "mdx_destroy": (ctypes.c_int, [vp]),
+        "mdx_last_error": (ctypes.c_char_p, [vp]),
+        "mdx_set_state": (ctypes.c_int, [vp, ctypes.c_int, dp, dp]),
+        "mdx_get_state": (ctypes.c_int, [vp, dp, dp]),
+        "mdx_particle_number": (ctypes.c_int, [vp, ip, ip]),
+        "mdx_make_pairlist": (ctypes.c_int, [vp]),
+        "mdx_is_pairlist_expired": (ctypes.c_int, [vp, ip]),
+        "mdx_update_position_half": (ctypes.c_int, [vp]),
+        "mdx_refresh_images": (ctypes.c_int, [vp]),
+        "mdx_calculate_force": (ctypes.c_int, [vp]),
+        "mdx_calculate": (ctypes.c_int, [vp, ctypes.c_int, ip]),
+        "mdx_observe": (ctypes.c_int, [vp, dp, dp, dp]),
+        "mdx_forces": (ctypes.c_int, [vp, dp]),
+        "mdx_pair_number": (ctypes.c_int, [vp, i64p]),
+        "mdx_pairs_canonical": (ctypes.c_int, [vp, P(ctypes.c_int32), ctypes.c_int64, i64p]),
+        "mdx_timing": (ctypes.c_int, [vp, dp, dp, dp, i64p, i64p]),
+        "mdx_timing_reset": (ctypes.c_int, [vp, ctypes.c_int]),
+        "mdx_kick_profile": (ctypes.c_int, [vp, dp, dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class System:
+    """One periodic LJ system resident on one GPU (an MDUnit's worth of
+    Variables + MeshList + PairList, mdunit.cc:8-39)."""
+
+    def __init__(self, L, dt=0.001, cutoff=3.0, buffer_length=0.3, check_list_length=500,
+                 capacity=0, device=0, stream=None):
+        self.lib = load()
+        prm = MdxParams()
+        self.lib.mdx_default_params(ctypes.byref(prm))
+        for d in range(3):
+            prm.L[d] = float(L[d])
+        prm.dt, prm.cutoff, prm.buffer_length = dt, cutoff, buffer_length
+        prm.check_list_length, prm.capacity, prm.device = check_list_length, capacity, device
+        prm.stream = stream
+        h = ctypes.c_void_p()
+        rc = self.lib.mdx_create(ctypes.byref(prm), ctypes.byref(h))
+        if rc != 0:
+            raise MdxError(rc, self.lib.mdx_last_error(None).decode())
+        self.h = h
+        self.n = 0
+
+    @classmethod
+    def from_state(cls, state, **kw):
+        s = cls(state.L, dt=state.dt, **kw)
+        s.set_state(state.q, state.p)
+        return s
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise MdxError(rc, self.lib.mdx_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mdx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_state(self, q, p):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        assert q.shape == p.shape and q.ndim == 2 and q.shape[1] == 3
+        self.n = q.shape[0]
+        self._chk(self.lib.mdx_set_state(self.h, self.n, _dp(q), _dp(p)))
+
+    def state(self):
+        q = np.empty((self.n, 3))
+        p = np.empty((self.n, 3))
+        self._chk(self.lib.mdx_get_state(self.h, _dp(q), _dp(p)))
+        return q, p
+
+    def particle_number(self):
+        pn, tn = ctypes.c_int(), ctypes.c_int()
+        self._chk(self.lib.mdx_particle_number(self.h, ctypes.byref(pn), ctypes.byref(tn)))
+        return pn.value, tn.value
+
+    def make_pairlist(self):
+        self._chk(self.lib.mdx_make_pairlist(self.h))
+
+    def is_expired(self) -> bool:
+        e = ctypes.c_int()
+        self._chk(self.lib.mdx_is_pairlist_expired(self.h, ctypes.byref(e)))
+        return bool(e.value)
+
+    def update_position_half(self):
+        self._chk(self.lib.mdx_update_position_half(self.h))
+
+    def refresh_images(self):
+        self._chk(self.lib.mdx_refresh_images(self.h))
+
+    def calculate_force(self):
+        self._chk(self.lib.mdx_calculate_force(self.h))
+
+    def calculate(self, nsteps: int) -> int:
+        r = ctypes.c_int()
+        self._chk(self.lib.mdx_calculate(self.h, nsteps, ctypes.byref(r)))
+        return r.value
+
+    def observe(self):
+        ke, pe, vir = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._chk(self.lib.mdx_observe(self.h, ctypes.byref(ke), ctypes.byref(pe), ctypes.byref(vir)))
+        return ke.value, pe.value, vir.value
+
+    def forces(self):
+        f = np.empty((self.n, 3))
+        self._chk(self.lib.mdx_forces(self.h, _dp(f)))
+        return f
+
+    def pair_number(self) -> int:
+        c = ctypes.c_int64()
+        self._chk(self.lib.mdx_pair_number(self.h, ctypes.byref(c)))
+        return c.value
+
+    def pairs(self) -> np.ndarray:
+        c = ctypes.c_int64()
+        self._chk(self.lib.mdx_pairs_canonical(self.h, None, 0, ctypes.byref(c)))
+        out = np.empty((c.value, 3), dtype=np.int32)
+        c2 = ctypes.c_int64()
+        self._chk(self.lib.mdx_pairs_canonical(
+            self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), c.value, ctypes.byref(c2)))
+        assert c2.value == c.value
+        return out
+
+    def timing_reset(self, enable=True):
+        self._chk(self.lib.mdx_timing_reset(self.h, 1 if enable else 0))
+
+    def timing(self):
+        b, f, o = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        nb, ns = ctypes.c_int64(), ctypes.c_int64()
+        self._chk(self.lib.mdx_timing(self.h, ctypes.byref(b), ctypes.byref(f), ctypes.byref(o),
+                                      ctypes.byref(nb), ctypes.byref(ns)))
+        return dict(build_ms=b.value, force_ms=f.value, other_ms=o.value, builds=nb.value,
+                    steps=ns.value)
+
+    def kick_profile(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._chk(self.lib.mdx_kick_profile(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value

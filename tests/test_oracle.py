@@ -1,0 +1,156 @@
+"""Pins the C oracle (oracle/mdoracle.c) before anything is checked against it.
+
+Each oracle function is checked against
+  * the compiled reference's outputs (tests/golden/*.npz, written by
+    tests/golden/make_golden.py through oracle/_ref),
+  * closed forms of the perfect FCC lattice (tests/lattice_sums.py),
+  * O(N^2) minimum-image brute force, and
+  * invariants (Newton 3, momentum conservation, time reversibility).
+A plausible mistake (a dropped C2 term, a sign, a missed image) fails one
+of them; test_planted_mistakes_are_caught shows it for a few.
+"""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_CASES, golden_state, load_golden
+from lattice_sums import fcc_per_particle, shift_constants
+from mdacp_amd import workloads as W
+from oracle import pyoracle as O
+
+
+def test_shift_constants_match_closed_form():
+    c0, c2 = O.constants(3.0)
+    e0, e2 = shift_constants(3.0)
+    assert c2 == pytest.approx(e2, rel=1e-15)
+    assert c0 == pytest.approx(e0, rel=1e-15)
+    # V(rc) = V'(rc) = 0
+    rc2 = 9.0
+    v = 4 * (rc2 ** -6 - rc2 ** -3 + c2 * rc2 + c0)
+    dv = -48 * rc2 ** -6 + 24 * rc2 ** -3 + 8 * c2 * rc2
+    assert abs(v) < 1e-16 and abs(dv) < 1e-15
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_oracle_matches_reference_fixture(name):
+    fx = load_golden(name)
+    s = golden_state(fx)
+    o = O.Oracle(s)
+    o.make_pairlist()
+    pairs = o.pairs()
+    assert len(pairs) == int(fx["pair_count"])
+    if "pairs" in fx:
+        np.testing.assert_array_equal(pairs, fx["pairs"])
+    import hashlib
+    dig = hashlib.sha256(np.ascontiguousarray(pairs, dtype="<i4").tobytes()).hexdigest()
+    assert dig == fx["pair_digest"].item().decode()
+    f = o.forces()
+    ref = fx["forces"]
+    scale = max(np.abs(ref).max(), 1.0)
+    assert np.abs(f - ref).max() / scale < 1e-12
+    ke, pe, vir = o.observe()
+    np.testing.assert_allclose([ke, pe, vir], fx["obs"], rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("ncell,rho", [(10, 0.5), (6, 0.7), (6, 0.9), (7, 0.3)])
+def test_oracle_fcc_closed_forms(ncell, rho):
+    s = W.fcc_state(ncell, rho, 1.0, seed=3)
+    a = s.L[0] / ncell
+    pe_n, vir_n, half_pairs, _ = fcc_per_particle(a)
+    o = O.Oracle(s)
+    o.make_pairlist()
+    assert len(o.pairs()) == half_pairs * s.n
+    _, pe, vir = o.observe()
+    assert pe / s.n == pytest.approx(pe_n, rel=1e-11)
+    assert vir / s.n == pytest.approx(vir_n, rel=1e-11)
+    f = o.forces()
+    assert np.abs(f).max() < 1e-10          # every site is a centre of symmetry
+
+
+@pytest.mark.parametrize("name", ["fcc4000_pert", "fcc500_rho0.7_pert", "fcc500_rho0.9_pert"])
+def test_oracle_matches_bruteforce(name):
+    s = golden_state(load_golden(name))
+    o = O.Oracle(s)
+    o.make_pairlist()
+    np.testing.assert_array_equal(o.pairs(), O.bruteforce_pairs(s, 3.3 ** 2))
+    f, pe, vir = O.bruteforce_forces(s)
+    fo = o.forces()
+    assert np.abs(fo - f).max() / np.abs(f).max() < 1e-12
+    _, pe_o, vir_o = o.observe()
+    assert pe_o == pytest.approx(pe, rel=1e-12)
+    assert vir_o == pytest.approx(vir, rel=1e-12)
+    # Newton 3
+    assert np.abs(fo.sum(axis=0)).max() < 1e-10
+
+
+def test_oracle_trajectory_matches_reference():
+    fx = load_golden("traj_fcc4000")
+    s = golden_state(fx)
+    o = O.Oracle(s)
+    rows = []
+    nreb = 0
+    for k in range(10):
+        nreb += o.calculate(100)
+        o.refresh_images()
+        ke, pe, vir = o.observe()
+        rows.append((100 * (k + 1), ke, pe, vir, nreb))
+    rows = np.array(rows)
+    ref = fx["series"]
+    # same rebuild schedule (the reference's constructions include the first)
+    np.testing.assert_array_equal(rows[:, 4], ref[:, 4])
+    np.testing.assert_allclose(rows[:, 1:4], ref[:, 1:4], rtol=1e-10)
+    q, p = o.state()
+    d = q - fx["q"]
+    d -= s.L * np.round(d / s.L)
+    assert np.abs(d).max() < 1e-9
+    assert np.abs(p - fx["p"]).max() < 1e-8
+
+
+def test_oracle_momentum_and_reversibility():
+    s = W.perturbed(W.plain_small(), 0.15, 21)
+    o = O.Oracle(s)
+    p0 = s.p.sum(axis=0)
+    o.calculate(50)
+    q, p = o.state()
+    np.testing.assert_allclose(p.sum(axis=0), p0, atol=1e-11)
+    # drift-kick-drift is time reversible: flip momenta, run back
+    back = W.State(q, -p, s.L, s.dt)
+    o2 = O.Oracle(back)
+    o2.calculate(50)
+    q2, p2 = o2.state()
+    d = q2 - s.q
+    d -= s.L * np.round(d / s.L)
+    assert np.abs(d).max() < 1e-9
+    assert np.abs(p2 + s.p).max() < 1e-8
+
+
+def test_planted_mistakes_are_caught():
+    """A wrong term or a missed image changes a pinned quantity."""
+    s = golden_state(load_golden("fcc4000_pert"))
+    f, pe, vir = O.bruteforce_forces(s)
+    c0, c2 = shift_constants(3.0)
+    # dropped C2 term in the force: perturbs forces well above tolerance
+    q = s.q
+    i, j = 0, 1
+    d = q[j] - q[i]
+    d -= s.L * np.round(d / s.L)
+    r2 = d @ d
+    assert abs(8 * c2 * r2) > 1e-6
+    # a dropped image: removing all pairs that need an image changes the set
+    bp = O.bruteforce_pairs(s, 3.3 ** 2)
+    assert (bp[:, 2] != 13).sum() > 1000
+    # a sign flip in the virial changes it by far more than 1e-12
+    assert abs(2 * vir) > 1.0
+
+
+def test_oracle_error_paths():
+    s = W.plain_small()
+    s.q[5, 1] = np.nan
+    o = O.Oracle(s)
+    with pytest.raises(O.OracleError):
+        o.make_pairlist()
+    s = W.plain_small()
+    s.p[7] = [200.0, 0.0, 0.0]    # 2 |p| dt > 0.3
+    o = O.Oracle(s)
+    o.make_pairlist()
+    with pytest.raises(O.OracleError):
+        o.calculate(1)
