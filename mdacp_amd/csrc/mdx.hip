@@ -51,20 +51,54 @@ struct DevStatus {
   int cand_count;
   int pair_hit;
   int pad0;
-  unsigned long long vmax2_bits;   // max |p|^2 over reals (bit pattern)
-  unsigned long long dmax2_bits;   // max displacement^2 (bit pattern)
+  double vmax2;                    // max |p|^2 over the reals
+  double dmax2;                    // max displacement^2 since the last build
+  double kmax_d;                   // longest neighbour list (as a double)
   long long canon_count;
   double sums[4];                  // KE, PE, virial, spare
 };
 
-__device__ __forceinline__ void atomic_max_nonneg(unsigned long long *addr, double v) {
-  // non-negative doubles order like their bit patterns
-  atomicMax(addr, (unsigned long long)__double_as_longlong(v));
-}
-
 __device__ __forceinline__ double wave_max(double v) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
   return v;
+}
+
+// Block maximum of a non-negative value written to partial[blockIdx.x];
+// every thread of the block must call it.  A one-block kernel folds the
+// partials (k_max_partials), so no address sees more than one writer.
+__device__ __forceinline__ void block_max_to(double v, double *__restrict__ partial) {
+  __shared__ double lds[kBlock / 64];
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = lds[0];
+    for (int w = 1; w < kBlock / 64; w++) m = fmax(m, lds[w]);
+    partial[blockIdx.x] = m;
+  }
+}
+
+__global__ void k_max_partials(const double *__restrict__ partial, int nb, double *__restrict__ out) {
+  __shared__ double lds[kBlock];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) m = fmax(m, partial[b]);
+  lds[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = lds[0];
+}
+
+// 1/x from the hardware estimate and two Newton steps (each squares the
+// relative error); within an ulp or two of the IEEE quotient.
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __fma_rn(-x, r, 1.0);
+  r = __fma_rn(r, e, r);
+  e = __fma_rn(-x, r, 1.0);
+  return __fma_rn(r, e, r);
 }
 
 // ---------------------------------------------------------------- scans
@@ -309,8 +343,8 @@ __global__ void k_gather_images(const int *__restrict__ order, int n, int g,
 // (meshlist.cc:354-362).  Column-major ELL: nbr[k * n_pad + i].
 __global__ void k_build_list(const double4 *__restrict__ pos, const int *__restrict__ cell_start,
                              const int *__restrict__ cell_count, int n, int n_pad, int kcap, Geom g,
-                             int *__restrict__ nbr, int *__restrict__ nn, int *__restrict__ kmax,
-                             int *__restrict__ flags) {
+                             int *__restrict__ nbr, int *__restrict__ nn,
+                             double *__restrict__ partial, int *__restrict__ flags) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int cnt = 0;
   if (i < n) {
@@ -337,10 +371,7 @@ __global__ void k_build_list(const double4 *__restrict__ pos, const int *__restr
     nn[i] = cnt < kcap ? cnt : kcap;
     if (cnt > kcap) atomicOr(flags, ST_LIST_OVERFLOW);
   }
-  // one atomic per wavefront
-  int m = cnt;
-  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
-  if ((threadIdx.x & 63) == 0) atomicMax(kmax, m);
+  block_max_to((double)cnt, partial);
 }
 
 // ---------------------------------------------------------------- step
@@ -373,19 +404,48 @@ __global__ void k_refresh_images(double4 *__restrict__ pos, const int *__restric
   pos[n + k] = r;
 }
 
+// Drift by half a step fused with the image refresh, out of place:
+// out[i] = q_i + p_i dt/2 for the reals (fcalculator.cc:26-28) and, for
+// image k, the same drifted source plus its shift (mdunit.cc:181-191), so
+// the images equal a refresh after the drift bit for bit.
+__global__ void k_drift_refresh(const double4 *__restrict__ pos, const double4 *__restrict__ mom,
+                                const int *__restrict__ gsrc, const int *__restrict__ gcode, int n,
+                                int g, double dt2, double Lx, double Ly, double Lz,
+                                double4 *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n + g) return;
+  const int src = i < n ? i : gsrc[i - n];
+  double4 q = pos[src];
+  const double4 p = mom[src];
+  q.x = __dadd_rn(q.x, __dmul_rn(p.x, dt2));
+  q.y = __dadd_rn(q.y, __dmul_rn(p.y, dt2));
+  q.z = __dadd_rn(q.z, __dmul_rn(p.z, dt2));
+  if (i >= n) {
+    const int c = gcode[i - n];
+    q.x = shifted(q.x, c % 3 - 1, Lx);
+    q.y = shifted(q.y, (c / 3) % 3 - 1, Ly);
+    q.z = shifted(q.z, c / 9 - 1, Lz);
+  }
+  q.w = 0.0;
+  out[i] = q;
+}
+
 // Force kick: lane per real, loop over its full list (no reaction, so no
 // write conflicts).  Pair term as ForceCalculator::CalculateForceSorted
 // (fcalculator.cc:203-217): df = ((24 r^6 - 48) / (r^6 r^6 r^2) + 8 C2) dt,
-// zero beyond the cutoff.  mode 0: p += F dt and track max |p|^2;
-// mode 1: write F (dt = 1) to out, p untouched.
+// zero beyond the cutoff; the quotient uses recip().
+//   MODE 0  p += F dt in place, block max |p|^2 to partial
+//   MODE 1  out = F (dt = 1), p untouched
+//   MODE 2  MODE 0 plus the trailing half drift written out of place
+//           (out[i] = q_i + p_i dt/2) and the images copied to out unchanged
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_kick(const double4 *__restrict__ pos,
                                                  double4 *__restrict__ mom,
                                                  const int *__restrict__ nbr,
                                                  const int *__restrict__ nn, int n, int n_pad,
-                                                 double C2_8, double dt, double CL2,
+                                                 int g, double C2_8, double dt, double CL2,
                                                  double4 *__restrict__ out,
-                                                 unsigned long long *__restrict__ vmax2) {
+                                                 double *__restrict__ partial) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double v2 = 0.0;
   if (i < n) {
@@ -393,44 +453,51 @@ __global__ void __launch_bounds__(kBlock) k_kick(const double4 *__restrict__ pos
     double fx = 0.0, fy = 0.0, fz = 0.0;
     const int m = nn[i];
     const int *lst = nbr + i;
+#pragma unroll 2
     for (int k = 0; k < m; k++) {
       const int j = lst[(size_t)k * n_pad];
       const double4 qj = pos[j];
       const double dx = qj.x - qi.x, dy = qj.y - qi.y, dz = qj.z - qi.z;
       const double r2 = dx * dx + dy * dy + dz * dz;
       const double r6 = r2 * r2 * r2;
-      double df = ((24.0 * r6 - 48.0) / (r6 * r6 * r2) + C2_8) * dt;
+      double df = ((24.0 * r6 - 48.0) * recip(r6 * r6 * r2) + C2_8) * dt;
       if (r2 > CL2) df = 0.0;
       fx += df * dx;
       fy += df * dy;
       fz += df * dz;
     }
-    if (MODE == 0) {
+    if (MODE == 1) {
+      out[i] = make_double4(fx, fy, fz, 0.0);
+    } else {
       double4 p = mom[i];
       p.x += fx;
       p.y += fy;
       p.z += fz;
       mom[i] = p;
       v2 = p.x * p.x + p.y * p.y + p.z * p.z;
-    } else {
-      out[i] = make_double4(fx, fy, fz, 0.0);
+      if (MODE == 2) {
+        const double dt2 = dt * 0.5;
+        double4 q = qi;
+        q.x = __dadd_rn(q.x, __dmul_rn(p.x, dt2));
+        q.y = __dadd_rn(q.y, __dmul_rn(p.y, dt2));
+        q.z = __dadd_rn(q.z, __dmul_rn(p.z, dt2));
+        out[i] = q;
+      }
     }
+  } else if (MODE == 2 && i < n + g) {
+    out[i] = pos[i];
   }
-  if (MODE == 0) {
-    v2 = wave_max(v2);
-    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(vmax2, v2);
-  }
+  if (MODE != 1) block_max_to(v2, partial);
 }
 
-__global__ void k_vmax(const double4 *__restrict__ mom, int n, unsigned long long *vmax2) {
+__global__ void k_vmax(const double4 *__restrict__ mom, int n, double *__restrict__ partial) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double v2 = 0.0;
   if (i < n) {
     const double4 p = mom[i];
     v2 = p.x * p.x + p.y * p.y + p.z * p.z;
   }
-  v2 = wave_max(v2);
-  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(vmax2, v2);
+  block_max_to(v2, partial);
 }
 
 // ------------------------------------------------------------ observers
@@ -509,7 +576,7 @@ __global__ void k_sum_partials(const double *__restrict__ partial, int nblocks,
 // ------------------------------------------------------------ expiry
 // PairList::CheckByDisplacement (pairlist.cc:68-121) in three kernels.
 __global__ void k_disp(const double4 *__restrict__ pos, const double4 *__restrict__ ref, int tn,
-                       double *__restrict__ disp, unsigned long long *__restrict__ dmax2) {
+                       double *__restrict__ disp, double *__restrict__ partial) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
   if (i < tn) {
@@ -518,8 +585,7 @@ __global__ void k_disp(const double4 *__restrict__ pos, const double4 *__restric
     d2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
     disp[i] = d2;
   }
-  d2 = wave_max(d2);
-  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(dmax2, d2);
+  block_max_to(d2, partial);
 }
 
 __global__ void k_candidates(const double *__restrict__ disp, int tn, double c_d2, int cap,
@@ -630,7 +696,7 @@ struct mdx_system {
   size_t tot_elems = 0;     // length of pos / ref_pos / cellid / ... arrays
   long long list_elems = 0;
   // device buffers
-  double4 *pos = nullptr, *pos_tmp = nullptr, *ref_pos = nullptr, *mom = nullptr,
+  double4 *pos = nullptr, *pos_alt = nullptr, *pos_tmp = nullptr, *ref_pos = nullptr, *mom = nullptr,
           *mom_tmp = nullptr, *gpos_tmp = nullptr, *vec_out = nullptr;
   int *tag = nullptr, *tag_tmp = nullptr, *gsrc = nullptr, *gcode = nullptr, *gsrc_tmp = nullptr,
       *gcode_tmp = nullptr;
@@ -639,7 +705,7 @@ struct mdx_system {
   int *cell_count = nullptr, *cell_start = nullptr, *cnt_in = nullptr, *cnt_out = nullptr,
       *scan_in = nullptr, *scan_out = nullptr, *scan_aux = nullptr;
   int *nbr = nullptr, *nn = nullptr;
-  double *disp = nullptr, *partial = nullptr;
+  double *disp = nullptr, *partial = nullptr, *pmax = nullptr;
   int *cand = nullptr;
   double *io_buf = nullptr;   // [cap][3] staging for host transfers
   int *i_buf = nullptr;
@@ -701,12 +767,6 @@ int sync_status(mdx_system *s) {
   return MDX_OK;
 }
 
-double bits_to_double(unsigned long long b) {
-  double d;
-  std::memcpy(&d, &b, sizeof d);
-  return d;
-}
-
 // exclusive scan helper (tiles staged in LDS); grand total to *grand if set
 int scan(mdx_system *s, const int *in, int *out, int count, int *grand) {
   const int ntiles = (count + kScanTile - 1) / kScanTile;
@@ -753,7 +813,8 @@ int ensure_image_capacity(mdx_system *s, int gcap) {
   }
   s->pos = npos;
   int rc;
-  if ((rc = dev_alloc(s, &s->ref_pos, tot)) ||
+  if ((rc = dev_alloc(s, &s->ref_pos, tot)) || (rc = dev_alloc(s, &s->pos_alt, tot)) ||
+      (rc = dev_alloc(s, &s->pmax, (size_t)nblk((long long)tot))) ||
       (rc = dev_alloc(s, &s->gpos_tmp, gcap)) || (rc = dev_alloc(s, &s->gsrc, gcap)) ||
       (rc = dev_alloc(s, &s->gcode, gcap)) || (rc = dev_alloc(s, &s->gsrc_tmp, gcap)) ||
       (rc = dev_alloc(s, &s->gcode_tmp, gcap)) || (rc = dev_alloc(s, &s->cellid, tot)) ||
@@ -851,15 +912,15 @@ int build_pairlist(mdx_system *s) {
   if ((rc = launch_check(s, "sort"))) return rc;
   // 5. neighbour list, one pass at the current width, again if it overflows
   for (int attempt = 0; attempt < 3; attempt++) {
-    HIPCHK(s, hipMemsetAsync(&s->dstat->kmax, 0, sizeof(int), s->stream));
     HIPCHK(s, hipMemsetAsync(&s->dstat->flags, 0, sizeof(int), s->stream));
     k_build_list<<<nblk(n), kBlock, 0, s->stream>>>(s->pos, s->cell_start, s->cell_count, n,
                                                      s->n_pad, s->kcap, s->geom, s->nbr, s->nn,
-                                                     &s->dstat->kmax, &s->dstat->flags);
+                                                     s->pmax, &s->dstat->flags);
+    k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(n), &s->dstat->kmax_d);
     if ((rc = launch_check(s, "list"))) return rc;
     if ((rc = sync_status(s))) return rc;
     if (!(s->hstat->flags & ST_LIST_OVERFLOW)) break;
-    if ((rc = ensure_list_capacity(s, s->hstat->kmax + 8))) return rc;
+    if ((rc = ensure_list_capacity(s, (int)s->hstat->kmax_d + 8))) return rc;
   }
   if (s->hstat->flags & ST_LIST_OVERFLOW) return fail(s, MDX_ERR_CAPACITY, "list capacity");
   // 6. PairList::Init: snapshot of reals and images
@@ -876,15 +937,37 @@ int build_pairlist(mdx_system *s) {
 }
 
 int kick(mdx_system *s) {
-  HIPCHK(s, hipMemsetAsync(&s->dstat->vmax2_bits, 0, sizeof(unsigned long long), s->stream));
   ev_record(s, 2);
-  k_kick<0><<<nblk(s->n), kBlock, 0, s->stream>>>(s->pos, s->mom, s->nbr, s->nn, s->n, s->n_pad,
+  k_kick<0><<<nblk(s->n), kBlock, 0, s->stream>>>(s->pos, s->mom, s->nbr, s->nn, s->n, s->n_pad, 0,
                                                    8.0 * s->C2, s->prm.dt, s->geom.CL2, nullptr,
-                                                   &s->dstat->vmax2_bits);
+                                                   s->pmax);
   ev_record(s, 3);
+  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(s->n), &s->dstat->vmax2);
   s->vmax_valid = true;
   s->n_kick++;
   return launch_check(s, "kick");
+}
+
+// One NVE step after the expiry check, in two passes over double-buffered
+// positions: drift + image refresh, then kick + drift (images carried over
+// at their mid-step positions, as the reference leaves them until the next
+// refresh).  Same arithmetic as drift(); refresh(); kick(); drift().
+int fused_step(mdx_system *s) {
+  const int n = s->n, g = s->g;
+  k_drift_refresh<<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->gsrc, s->gcode, n, g,
+                                                          s->prm.dt * 0.5, s->geom.L[0],
+                                                          s->geom.L[1], s->geom.L[2], s->pos_alt);
+  std::swap(s->pos, s->pos_alt);
+  ev_record(s, 2);
+  k_kick<2><<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->nbr, s->nn, n, s->n_pad, g,
+                                                    8.0 * s->C2, s->prm.dt, s->geom.CL2, s->pos_alt,
+                                                    s->pmax);
+  ev_record(s, 3);
+  std::swap(s->pos, s->pos_alt);
+  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(n + g), &s->dstat->vmax2);
+  s->vmax_valid = true;
+  s->n_kick++;
+  return launch_check(s, "step");
 }
 
 int drift(mdx_system *s) {
@@ -910,12 +993,12 @@ int expired(mdx_system *s, int *out) {
   if (s->buffer_length > 0.0) {
     // UpdatePairListValidity (pairlist.cc:48-65): max |p| of the reals
     if (!s->vmax_valid) {
-      HIPCHK(s, hipMemsetAsync(&s->dstat->vmax2_bits, 0, sizeof(unsigned long long), s->stream));
-      k_vmax<<<nblk(s->n), kBlock, 0, s->stream>>>(s->mom, s->n, &s->dstat->vmax2_bits);
+      k_vmax<<<nblk(s->n), kBlock, 0, s->stream>>>(s->mom, s->n, s->pmax);
+      k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(s->n), &s->dstat->vmax2);
       s->vmax_valid = true;
     }
     if ((rc = sync_status(s))) return rc;
-    const double vmax = std::sqrt(bits_to_double(s->hstat->vmax2_bits));
+    const double vmax = std::sqrt(s->hstat->vmax2);
     if (vmax * 2.0 * s->prm.dt > s->prm.buffer_length)
       return fail(s, MDX_ERR_TOO_FAST, "Too fast particles exists. max_velocity = %f", vmax);
     s->buffer_length = s->buffer_length - vmax * 2.0 * s->prm.dt;
@@ -924,12 +1007,12 @@ int expired(mdx_system *s, int *out) {
   if (!(s->buffer_length < 0.0)) return MDX_OK;
   // CheckByDisplacement (pairlist.cc:68-121) over reals and images
   const int tn = s->n + s->g;
-  HIPCHK(s, hipMemsetAsync(&s->dstat->dmax2_bits, 0, sizeof(unsigned long long), s->stream));
   HIPCHK(s, hipMemsetAsync(&s->dstat->cand_count, 0, 2 * sizeof(int), s->stream));
-  k_disp<<<nblk(tn), kBlock, 0, s->stream>>>(s->pos, s->ref_pos, tn, s->disp, &s->dstat->dmax2_bits);
+  k_disp<<<nblk(tn), kBlock, 0, s->stream>>>(s->pos, s->ref_pos, tn, s->disp, s->pmax);
+  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(tn), &s->dstat->dmax2);
   if ((rc = sync_status(s))) return rc;
   const int Nl = tn > s->prm.check_list_length ? s->prm.check_list_length : tn;
-  const double c_distance = s->prm.buffer_length - std::sqrt(bits_to_double(s->hstat->dmax2_bits));
+  const double c_distance = s->prm.buffer_length - std::sqrt(s->hstat->dmax2);
   const double c_d2 = c_distance * c_distance;
   k_candidates<<<nblk(tn), kBlock, 0, s->stream>>>(s->disp, tn, c_d2, Nl, s->cand, &s->dstat->cand_count);
   if ((rc = sync_status(s))) return rc;
@@ -1042,7 +1125,7 @@ int mdx_create(const mdx_params *prm, mdx_system **out) {
 int mdx_destroy(mdx_system *s) {
   if (!s) return MDX_OK;
   if (s->stream) hipStreamSynchronize(s->stream);
-  void *bufs[] = {s->pos, s->pos_tmp, s->ref_pos, s->mom, s->mom_tmp, s->gpos_tmp, s->vec_out,
+  void *bufs[] = {s->pos, s->pos_alt, s->pmax, s->pos_tmp, s->ref_pos, s->mom, s->mom_tmp, s->gpos_tmp, s->vec_out,
                   s->tag, s->tag_tmp, s->gsrc, s->gcode, s->gsrc_tmp, s->gcode_tmp, s->cellid,
                   s->slot, s->order, s->inv, s->nimg, s->goff, s->cell_count, s->cell_start,
                   s->cnt_in, s->cnt_out, s->scan_in, s->scan_out, s->scan_aux, s->nbr, s->nn,
@@ -1150,7 +1233,7 @@ int mdx_calculate(mdx_system *s, int nsteps, int *rebuilds) {
       nreb++;
     }
     ev_record(s, 4);
-    if ((rc = drift(s)) || (rc = refresh(s)) || (rc = kick(s)) || (rc = drift(s))) return rc;
+    if ((rc = fused_step(s))) return rc;
     ev_record(s, 5);
     s->n_steps++;
     if (s->timing) {
@@ -1174,7 +1257,7 @@ int mdx_observe(mdx_system *s, double *ke, double *pe, double *vir) {
 int mdx_forces(mdx_system *s, double *f3) {
   if (!s || !f3) return fail(s, MDX_ERR_ARG, "null argument");
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
-  k_kick<1><<<nblk(s->n), kBlock, 0, s->stream>>>(s->pos, s->mom, s->nbr, s->nn, s->n, s->n_pad,
+  k_kick<1><<<nblk(s->n), kBlock, 0, s->stream>>>(s->pos, s->mom, s->nbr, s->nn, s->n, s->n_pad, 0,
                                                    8.0 * s->C2, 1.0, s->geom.CL2, s->vec_out,
                                                    nullptr);
   k_scatter_by_tag<<<nblk(s->n), kBlock, 0, s->stream>>>(s->vec_out, s->tag, s->n, s->io_buf);
@@ -1243,20 +1326,19 @@ int mdx_timing_reset(mdx_system *s, int enable) {
 
 int mdx_kick_profile(mdx_system *s, double *avg_ms, double *bytes) {
   if (!s) return MDX_ERR_ARG;
-  // algorithmic bytes per launch: read q_i and p_i (24 B each), write p_i
-  // (24 B), the count nn_i (4 B) and every neighbour index (4 B); partner
-  // positions are re-reads of q, counted once in q_i.
-  long long entries = 0;
   if (avg_ms) *avg_ms = s->n_steps > 0 && s->timing ? s->t_kick_only / (double)s->n_steps : 0.0;
   if (bytes) {
+    // Algorithmic bytes of one fused kick + half-drift launch, per real:
+    // read q and p, write p and q (24 B each), the partner count (4 B), and
+    // 4 B per half-list partner (each pair once, as the reference's half
+    // list); partner positions are re-reads of q and images re-reads of
+    // their sources, counted once in q.
+    int64_t pairs = 0;
     if (s->list_valid) {
-      // sum of nn via a device scan is overkill here: use the canonical count
-      int64_t pairs = 0;
       int rc = mdx_pair_number(s, &pairs);
       if (rc) return rc;
-      entries = 2 * pairs;
     }
-    *bytes = (double)s->n * (24.0 + 24.0 + 24.0 + 4.0) + 4.0 * (double)entries;
+    *bytes = (double)s->n * (4.0 * 24.0 + 4.0) + 4.0 * (double)pairs;
   }
   return MDX_OK;
 }
