@@ -37,6 +37,28 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+HOST_SRC = os.path.join(HERE, "host", "mdacp_gpu.cpp")
+HOST_LIB = os.path.join(HERE, "libmdacp_gpu.so")
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "shim_driver.cc")
+DRIVER = os.path.join(ROOT, "tests", "cpp", "_build", "shim_driver")
+
+
+def build_host(verbose: bool = False) -> None:
+    """The C++ mirror of the reference interface over libmdx, and the test
+    driver that runs it (g++; no device code)."""
+    cmds = [
+        ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", HOST_LIB, HOST_SRC,
+         f"-L{HERE}", "-lmdx", "-Wl,-rpath,$ORIGIN"],
+        ["g++", "-O2", "-std=c++17", "-Wall", "-o", DRIVER, DRIVER_SRC, f"-L{HERE}",
+         "-lmdacp_gpu", "-lmdx", "-Wl,-rpath,$ORIGIN/../../../mdacp_amd"],
+    ]
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+
+
 def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     cmd = [hipcc(), *HIP_FLAGS, SRC, "-o", tmp]
@@ -47,6 +69,7 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed for mdx.hip")
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
 
 

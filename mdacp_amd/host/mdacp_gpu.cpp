@@ -1,0 +1,295 @@
+// mdacp_gpu.cpp — see mdacp_gpu.h.
+#include "mdacp_gpu.h"
+
+#include <cmath>
+#include <cstring>
+#include <random>
+
+namespace mdx {
+namespace {
+
+void check(mdx_system *s, int rc) {
+  if (rc != MDX_OK) throw Error(rc, mdx_last_error(s));
+}
+
+}  // namespace
+
+SimulationInfo::SimulationInfo(double lx, double ly, double lz, double time_step) {
+  L[X] = lx;
+  L[Y] = ly;
+  L[Z] = lz;
+  L[3] = 0.0;
+  TimeStep = time_step;
+  BufferLength = 0.3;                         // simulationinfo.cc:6
+  SearchLength = CUTOFF_LENGTH + BufferLength;  // simulationinfo.cc:7
+  CheckListLength = 500;                      // simulationinfo.cc:29
+}
+
+MDRect::MDRect(const double s2[D], const double e2[D]) {
+  for (int i = 0; i < D; i++) {
+    s[i] = s2[i];
+    e[i] = e2[i];
+  }
+}
+
+// ------------------------------------------------------------ Variables
+Variables::Variables(SimulationInfo *si) : sinfo(si) {
+  // variables.cc:21-27
+  const double s2 = 1.0 / (CUTOFF_LENGTH * CUTOFF_LENGTH);
+  const double s6 = s2 * s2 * s2, s8 = s6 * s2, s12 = s6 * s6, s14 = s12 * s2;
+  C2 = 6.0 * s14 - 3.0 * s8;
+  C0 = -s12 + s6 - C2 / s2;
+}
+
+Variables::~Variables() {
+  if (sys) mdx_destroy(sys);
+}
+
+void Variables::Grow(int need) {
+  if (need <= capacity) return;
+  int cap = capacity ? capacity : 1024;
+  while (cap < need) cap *= 2;
+  qbuf.resize(static_cast<size_t>(cap) * D, 0.0);
+  pbuf.resize(static_cast<size_t>(cap) * D, 0.0);
+  type.resize(cap, 0);
+  q = reinterpret_cast<double(*)[D]>(qbuf.data());
+  p = reinterpret_cast<double(*)[D]>(pbuf.data());
+  capacity = cap;
+}
+
+int Variables::GetTotalParticleNumber() const {
+  if (!sys) return particle_number;
+  int pn = 0, tn = 0;
+  mdx_particle_number(sys, &pn, &tn);
+  return tn;
+}
+
+void Variables::AddParticle(double x[D], double v[D], int t) {
+  Grow(particle_number + 1);
+  for (int d = 0; d < 3; d++) {
+    q[particle_number][d] = x[d];
+    p[particle_number][d] = v[d];
+  }
+  q[particle_number][3] = p[particle_number][3] = 0.0;
+  type[particle_number] = t;
+  particle_number++;
+  host_dirty = true;
+}
+
+void Variables::SetInitialVelocity(const double V0, const int seed) {
+  SyncToHost();
+  std::mt19937 mt(seed);
+  std::uniform_real_distribution<double> ud(0.0, 1.0);
+  for (int i = 0; i < particle_number; i++) {
+    const double z = ud(mt) * 2.0 - 1.0;
+    const double phi = ud(mt) * 2.0 * M_PI;
+    p[i][X] = V0 * std::sqrt(1 - z * z) * std::cos(phi);
+    p[i][Y] = V0 * std::sqrt(1 - z * z) * std::sin(phi);
+    p[i][Z] = V0 * z;
+  }
+  double av[3] = {0, 0, 0};
+  for (int i = 0; i < particle_number; i++)
+    for (int d = 0; d < 3; d++) av[d] += p[i][d];
+  for (int d = 0; d < 3; d++) av[d] /= static_cast<double>(particle_number);
+  for (int i = 0; i < particle_number; i++)
+    for (int d = 0; d < 3; d++) p[i][d] -= av[d];
+  host_dirty = true;
+}
+
+void Variables::SyncToDevice() {
+  if (particle_number == 0) return;
+  if (!sys) {
+    mdx_params prm;
+    mdx_default_params(&prm);
+    for (int d = 0; d < 3; d++) prm.L[d] = sinfo->L[d];
+    prm.dt = sinfo->TimeStep;
+    prm.buffer_length = sinfo->BufferLength;
+    prm.cutoff = sinfo->SearchLength - sinfo->BufferLength;
+    prm.check_list_length = sinfo->CheckListLength;
+    prm.device = sinfo->Device;
+    int rc = mdx_create(&prm, &sys);
+    if (rc != MDX_OK) throw Error(rc, mdx_last_error(nullptr));
+  }
+  std::vector<double> q3(3 * static_cast<size_t>(particle_number)), p3(q3.size());
+  for (int i = 0; i < particle_number; i++)
+    for (int d = 0; d < 3; d++) {
+      q3[3 * i + d] = q[i][d];
+      p3[3 * i + d] = p[i][d];
+    }
+  check(sys, mdx_set_state(sys, particle_number, q3.data(), p3.data()));
+  // the device numbers particles by their position in this upload
+  host_ids.assign(type.begin(), type.begin() + particle_number);
+  host_dirty = false;
+}
+
+void Variables::SyncToHost() {
+  if (!sys || host_dirty) return;
+  std::vector<double> q3(3 * static_cast<size_t>(particle_number)), p3(q3.size());
+  check(sys, mdx_get_state(sys, q3.data(), p3.data()));
+  for (int i = 0; i < particle_number; i++) {
+    for (int d = 0; d < 3; d++) {
+      q[i][d] = q3[3 * i + d];
+      p[i][d] = p3[3 * i + d];
+    }
+    type[i] = host_ids[i];
+  }
+}
+
+mdx_system *Variables::Device() {
+  if (host_dirty || !sys) SyncToDevice();
+  if (!sys) throw Error(MDX_ERR_STATE, "no particles");
+  return sys;
+}
+
+// ------------------------------------------------------------ MeshList
+MeshList::MeshList(SimulationInfo *sinfo, MDRect &r) {
+  (void)sinfo;
+  (void)r;
+}
+
+void MeshList::MakeList(Variables *vars, SimulationInfo *sinfo, MDRect &myrect) {
+  (void)sinfo;
+  (void)myrect;
+  mdx_system *s = vars->Device();
+  check(s, mdx_make_pairlist(s));
+  number_of_constructions++;
+}
+
+void MeshList::RefreshImages(Variables *vars) {
+  mdx_system *s = vars->Device();
+  check(s, mdx_refresh_images(s));
+}
+
+long MeshList::GetPairNumber(Variables *vars) {
+  mdx_system *s = vars->Device();
+  int64_t c = 0;
+  check(s, mdx_pair_number(s, &c));
+  return static_cast<long>(c);
+}
+
+std::vector<int> MeshList::GetPairsCanonical(Variables *vars) {
+  mdx_system *s = vars->Device();
+  int64_t c = 0;
+  check(s, mdx_pairs_canonical(s, nullptr, 0, &c));
+  std::vector<int32_t> t(3 * static_cast<size_t>(c));
+  check(s, mdx_pairs_canonical(s, t.data(), c, &c));
+  std::vector<int> out(t.size());
+  for (size_t k = 0; k < t.size(); k += 3) {   // device ids -> the caller's type[] ids
+    out[k] = vars->host_ids[t[k]];
+    out[k + 1] = vars->host_ids[t[k + 1]];
+    out[k + 2] = t[k + 2];
+  }
+  return out;
+}
+
+// ------------------------------------------------------------ PairList
+bool PairList::IsPairListExpired(Variables *vars, MeshList *mesh, SimulationInfo *sinfo) {
+  (void)mesh;
+  (void)sinfo;
+  if (!initialized) return true;
+  int expired = 0;
+  mdx_system *s = vars->Device();
+  check(s, mdx_is_pairlist_expired(s, &expired));
+  return expired != 0;
+}
+
+// ------------------------------------------------------------ ForceCalculator
+namespace ForceCalculator {
+void CalculateForce(Variables *vars, MeshList *mesh, SimulationInfo *sinfo) {
+  (void)mesh;
+  (void)sinfo;
+  mdx_system *s = vars->Device();
+  check(s, mdx_calculate_force(s));
+}
+
+void UpdatePositionHalf(Variables *vars, SimulationInfo *sinfo) {
+  (void)sinfo;
+  mdx_system *s = vars->Device();
+  check(s, mdx_update_position_half(s));
+}
+
+std::vector<double> Forces(Variables *vars, MeshList *mesh) {
+  (void)mesh;
+  mdx_system *s = vars->Device();
+  const int n = vars->GetParticleNumber();
+  std::vector<double> f(3 * static_cast<size_t>(n));
+  check(s, mdx_forces(s, f.data()));
+  return f;
+}
+}  // namespace ForceCalculator
+
+// ------------------------------------------------------------ observers
+double KineticEnergyObserver::Observe(Variables *vars, MeshList *) {
+  mdx_system *s = vars->Device();
+  double ke = 0;
+  check(s, mdx_observe(s, &ke, nullptr, nullptr));
+  return ke;
+}
+
+double PotentialEnergyObserver::Observe(Variables *vars, MeshList *) {
+  mdx_system *s = vars->Device();
+  double pe = 0;
+  check(s, mdx_observe(s, nullptr, &pe, nullptr));
+  return pe;
+}
+
+double VirialObserver::Observe(Variables *vars, MeshList *) {
+  mdx_system *s = vars->Device();
+  double vir = 0;
+  check(s, mdx_observe(s, nullptr, nullptr, &vir));
+  return vir;
+}
+
+// ------------------------------------------------------------ MDUnit
+MDUnit::MDUnit(SimulationInfo *si) : sinfo(si) {
+  vars = new Variables(si);
+  plist = new PairList();
+  double s[D] = {0, 0, 0, 0}, e[D] = {si->L[X], si->L[Y], si->L[Z], 0};
+  myrect = MDRect(s, e);
+  mesh = new MeshList(si, myrect);
+}
+
+MDUnit::~MDUnit() {
+  delete mesh;
+  delete plist;
+  delete vars;
+}
+
+void MDUnit::AddParticle(double x[D], double v[D], int type) {
+  if (myrect.IsInside(x)) vars->AddParticle(x, v, type);   // mdunit.cc:61-65
+}
+
+void MDUnit::MakePairList() {   // MDUnit::MakePairList (mdunit.cc:211-215)
+  plist->Init(vars, sinfo);
+  mesh->MakeList(vars, sinfo, myrect);
+}
+
+void MDUnit::Calculate() {      // MDManager::Calculate, NVE (mdmanager.cc:154-186)
+  if (IsPairListExpired()) MakePairList();
+  ForceCalculator::UpdatePositionHalf(vars, sinfo);
+  mesh->RefreshImages(vars);
+  ForceCalculator::CalculateForce(vars, mesh, sinfo);
+  ForceCalculator::UpdatePositionHalf(vars, sinfo);
+  vars->SimulationTime += sinfo->TimeStep;
+}
+
+double MDUnit::Temperature() {
+  KineticEnergyObserver obs;
+  return obs.Observe(vars, mesh) / vars->GetParticleNumber() / 1.5;
+}
+
+double MDUnit::PotentialEnergy() {
+  PotentialEnergyObserver obs(sinfo);
+  return obs.Observe(vars, mesh) / vars->GetParticleNumber();
+}
+
+double MDUnit::Pressure() {
+  VirialObserver obs;
+  const double pn = vars->GetParticleNumber();
+  const double phi = obs.Observe(vars, mesh) / pn;
+  const double T = Temperature();
+  const double V = sinfo->L[X] * sinfo->L[Y] * sinfo->L[Z];
+  return (T - phi) * pn / V;
+}
+
+}  // namespace mdx

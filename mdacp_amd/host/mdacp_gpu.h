@@ -1,0 +1,186 @@
+// mdacp_gpu.h — the reference's hot-path C++ interface (the calls MDUnit
+// makes, mdunit.h:59-102) implemented over libmdx (include/mdx.h).
+//
+// Names, arguments and call order follow the reference:
+//   Variables          variables.h:12-49
+//   SimulationInfo     simulationinfo.h:11-34 (the fields the path reads)
+//   MDRect             mdrect.h:9-45
+//   MeshList           meshlist.h:13-102
+//   PairList           pairlist.h:10-31
+//   ForceCalculator    fcalculator.h:10-44
+//   *Observer          observer.h:8-38
+// so that MDManager::Calculate (mdmanager.cc:154-186) reads unchanged:
+//   if (plist->IsPairListExpired(vars, mesh, sinfo)) { plist->Init(...); mesh->MakeList(...); }
+//   ForceCalculator::UpdatePositionHalf(vars, sinfo);
+//   <image refresh>;  ForceCalculator::CalculateForce(vars, mesh, sinfo);
+//   ForceCalculator::UpdatePositionHalf(vars, sinfo);
+//
+// What differs, because the state lives in HBM:
+//   * q / p / type are host mirrors.  Variables::SyncToHost() refreshes them
+//     from the device; after writing them on the host call
+//     Variables::SyncToDevice().  AddParticle and SetInitialVelocity do this
+//     bookkeeping themselves.
+//   * Periodic images are made and refreshed on the device: MeshList::MakeList
+//     builds them (the reference's MDManager::MakePairList halo exchange) and
+//     MeshList::RefreshImages replaces MDManager::SendBorderParticles.
+//   * Fatal conditions throw mdx::Error (status code + message) where the
+//     reference calls exit(1) or assert.
+#ifndef MDACP_GPU_H
+#define MDACP_GPU_H
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mdx.h"
+
+namespace mdx {
+
+constexpr int D = 4;  // mdconfig.h:9, lane 3 unused
+constexpr int X = 0, Y = 1, Z = 2;
+constexpr double CUTOFF_LENGTH = 3.0;  // mdconfig.h:31
+
+struct Error : std::runtime_error {
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+  int code;
+};
+
+// the SimulationInfo fields on the path (simulationinfo.cc:5-51)
+struct SimulationInfo {
+  SimulationInfo(double lx, double ly, double lz, double time_step = 0.001);
+  double L[D];
+  double TimeStep;
+  double SearchLength;   // CUTOFF_LENGTH + BufferLength
+  double BufferLength;   // 0.3
+  int CheckListLength;   // 500
+  int Device = 0;
+};
+
+class MDRect {  // mdrect.h:9-45, one domain [0, L)
+ public:
+  MDRect() = default;
+  MDRect(const double s2[D], const double e2[D]);
+  double s[D] = {0, 0, 0, 0}, e[D] = {0, 0, 0, 0};
+  double GetWidth(int dir) const { return e[dir] - s[dir]; }
+  bool IsInside(const double q[D]) const {
+    return s[X] <= q[X] && q[X] < e[X] && s[Y] <= q[Y] && q[Y] < e[Y] && s[Z] <= q[Z] && q[Z] < e[Z];
+  }
+};
+
+class Variables {
+ public:
+  explicit Variables(SimulationInfo *sinfo);
+  ~Variables();
+  Variables(const Variables &) = delete;
+  Variables &operator=(const Variables &) = delete;
+
+  // host mirrors (variables.h:19-28); see SyncToHost / SyncToDevice
+  std::vector<int> type;
+  double (*q)[D] = nullptr;
+  double (*p)[D] = nullptr;
+  double Zeta = 0.0;
+  double SimulationTime = 0.0;
+
+  double GetC0() const { return C0; }
+  double GetC2() const { return C2; }
+  int GetParticleNumber() const { return particle_number; }
+  int GetTotalParticleNumber() const;
+  void AddParticle(double x[D], double v[D], int t = 0);   // variables.cc:36-47
+  void SetInitialVelocity(double V0, int seed);             // variables.cc:87-113
+  void SyncToHost();
+  void SyncToDevice();
+  mdx_system *Device();   // uploads pending host changes first
+
+ private:
+  void Grow(int need);
+  SimulationInfo *sinfo;
+  mdx_system *sys = nullptr;
+  std::vector<double> qbuf, pbuf;
+  std::vector<int> host_ids;   // type[] of each particle at the last upload
+  int particle_number = 0;
+  int capacity = 0;
+  bool host_dirty = false;
+  double C0, C2;
+  friend class MeshList;
+};
+
+class MeshList {
+ public:
+  MeshList(SimulationInfo *sinfo, MDRect &r);
+  // MeshList::MakeList (meshlist.cc:68-104), including the periodic images
+  // the reference builds in MDManager::MakePairList (mdmanager.cc:388-417)
+  void MakeList(Variables *vars, SimulationInfo *sinfo, MDRect &myrect);
+  // MDManager::SendBorderParticles (mdmanager.cc:420-429)
+  void RefreshImages(Variables *vars);
+  long GetPairNumber(Variables *vars);           // canonical pair count
+  // canonical pair set, (id_lo, id_hi, image code) triples
+  std::vector<int> GetPairsCanonical(Variables *vars);
+  int GetNumberOfConstructions() const { return number_of_constructions; }
+  void ClearNumberOfConstructions() { number_of_constructions = 0; }
+
+ private:
+  int number_of_constructions = 0;
+};
+
+class PairList {  // pairlist.h:10-31
+ public:
+  void Init(Variables *vars, SimulationInfo *sinfo) { (void)vars; (void)sinfo; initialized = true; }
+  bool IsPairListExpired(Variables *vars, MeshList *mesh, SimulationInfo *sinfo);
+
+ private:
+  bool initialized = false;
+};
+
+namespace ForceCalculator {  // fcalculator.h:10-44
+void CalculateForce(Variables *vars, MeshList *mesh, SimulationInfo *sinfo);
+void UpdatePositionHalf(Variables *vars, SimulationInfo *sinfo);
+// forces at dt = 1 with p untouched, [pn][3] by particle id (test hook)
+std::vector<double> Forces(Variables *vars, MeshList *mesh);
+}  // namespace ForceCalculator
+
+class DoubleObserver {  // observer.h:13-16
+ public:
+  virtual ~DoubleObserver() = default;
+  virtual double Observe(Variables *vars, MeshList *mesh) = 0;
+};
+class KineticEnergyObserver : public DoubleObserver {
+ public:
+  double Observe(Variables *vars, MeshList *mesh) override;
+};
+class PotentialEnergyObserver : public DoubleObserver {
+ public:
+  explicit PotentialEnergyObserver(SimulationInfo *si = nullptr) { (void)si; }
+  double Observe(Variables *vars, MeshList *mesh) override;
+};
+class VirialObserver : public DoubleObserver {
+ public:
+  double Observe(Variables *vars, MeshList *mesh) override;
+};
+
+// One periodic domain driven in the order of MDManager::Calculate
+// (mdmanager.cc:154-186) and MDManager::MakePairList (:388-417).
+class MDUnit {
+ public:
+  MDUnit(SimulationInfo *si);
+  ~MDUnit();
+  Variables *GetVariables() { return vars; }
+  MeshList *GetMeshList() { return mesh; }
+  void AddParticle(double x[D], double v[D], int type = 1);
+  void MakePairList();
+  bool IsPairListExpired() { return plist->IsPairListExpired(vars, mesh, sinfo); }
+  void Calculate();  // one NVE step
+  double ObserveDouble(DoubleObserver *obs) { return obs->Observe(vars, mesh); }
+  double Temperature();        // KE/N/1.5 (mdmanager.cc:531-533)
+  double PotentialEnergy();    // per particle (mdmanager.cc:551-555)
+  double Pressure();           // (T - vir/N) N/V (mdmanager.cc:563-570)
+
+ private:
+  SimulationInfo *sinfo;
+  Variables *vars;
+  MeshList *mesh;
+  PairList *plist;
+  MDRect myrect;
+};
+
+}  // namespace mdx
+#endif
