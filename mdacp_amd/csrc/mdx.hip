@@ -5,18 +5,19 @@
 //   mom   double4[n]      momenta of the reals (same order)
 //   tag   int[n]          particle id of each real
 //   gsrc  int[g], gcode   image -> its real (cell-ordered index) and shift
-//   cell_start/cell_count over the (ms+2)^3 padded cell grid: interior cells
-//         hold reals only, the outer layer holds images only
-//   nbr   int[K][n_pad]   column-major neighbour list (full list, partners
-//         may be images), nn[n] partner counts
+//   scan_in/cnt_in, scan_out/cnt_out over the (ms+2)^3 padded cell grid:
+//         interior cells hold reals only, the outer layer images only
+//   tiles TileDesc[]      <= kTile consecutive reals of one cell row
+//   list16 u16[tile][K][kTile]  full neighbour list of every real (partners
+//         real or image), as offsets into its tile's LDS staging; nn[n]
 //
 // The reference keeps q/p as AoS double[N][4] on the host, materialises
 // images with an MPI halo exchange and builds a half list on the CPU
-// (mdmanager.cc:388-489, meshlist.cc:68-415); here images are formed on
+// (mdmanager.cc:388-489, meshlist.cc:68-415).  Here images are formed on
 // the device from the same rule (mdrect.cc:24-47, mdunit.cc:153-192), the
 // cell sort is a counting sort with LDS-staged scans, and the force kernel
-// is reactionless over a full list (each pair evaluated from both sides, no
-// atomics, results deterministic).
+// is reactionless over a full list: each pair is evaluated from both
+// sides, nothing is written by two lanes, and results are deterministic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,13 +48,14 @@ enum StatusBits : int {
 struct DevStatus {
   int flags;
   int ghost_total;
-  int kmax;
+  int ntiles;
   int cand_count;
   int pair_hit;
   int pad0;
   double vmax2;                    // max |p|^2 over the reals
   double dmax2;                    // max displacement^2 since the last build
   double kmax_d;                   // longest neighbour list (as a double)
+  double stage_max;                // largest tile staging (particles)
   long long canon_count;
   double sums[4];                  // KE, PE, virial, spare
 };
@@ -73,7 +75,7 @@ __device__ __forceinline__ void block_max_to(double v, double *__restrict__ part
   __syncthreads();
   if (threadIdx.x == 0) {
     double m = lds[0];
-    for (int w = 1; w < kBlock / 64; w++) m = fmax(m, lds[w]);
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, lds[w]);
     partial[blockIdx.x] = m;
   }
 }
@@ -337,37 +339,180 @@ __global__ void k_gather_images(const int *__restrict__ order, int n, int g,
   gcode[k] = gcode_in[pre];
 }
 
-// Full neighbour list: every real i lists every particle j != i (real or
-// image) of the 27 surrounding cells with r^2 <= SL^2, r^2 formed as the
-// reference's AVX2 search, fma(dx,dx, fma(dy,dy, dz*dz)) with dq = q_j - q_i
-// (meshlist.cc:354-362).  Column-major ELL: nbr[k * n_pad + i].
-__global__ void k_build_list(const double4 *__restrict__ pos, const int *__restrict__ cell_start,
-                             const int *__restrict__ cell_count, int n, int n_pad, int kcap, Geom g,
-                             int *__restrict__ nbr, int *__restrict__ nn,
-                             double *__restrict__ partial, int *__restrict__ flags) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// ---------------------------------------------------------------- tiles
+// A tile is up to kTile consecutive reals of one interior cell row (y, z),
+// spanning cells xf..xl.  Its partners all lie in 9 cell-row ranges
+// (y+dy, z+dz, xf-1..xl+1), each of which is one contiguous run of reals
+// (interior cells, slots scan_in) and one of images (outer cells, slots
+// n + scan_out): 18 segments, staged once into LDS as x[], y[], z[].
+// Neighbour lists hold 16-bit offsets into that staging.
+constexpr int kTile = 128;
+constexpr int kSeg = 18;
+
+struct TileDesc {
+  int i0, ni, y, z, xf, xl, stage, pad;
+};
+
+struct Cells {
+  const int *scan_in, *cnt_in, *scan_out, *cnt_out;
+};
+
+__device__ __forceinline__ bool is_outer(const Geom &g, int x, int y, int z) {
+  return x == 0 || y == 0 || z == 0 || x == g.m[0] - 1 || y == g.m[1] - 1 || z == g.m[2] - 1;
+}
+
+// [start, end) of segment s (0..17) of a tile, as global slots
+__device__ __forceinline__ void tile_segment(const Geom &g, const Cells &c, int n, int y, int z,
+                                             int xf, int xl, int s, int *start, int *len) {
+  const int r = s >> 1, dy = r % 3 - 1, dz = r / 3 - 1;
+  const int a = cell_linear(g, xf - 1, y + dy, z + dz);
+  const int b = cell_linear(g, xl + 1, y + dy, z + dz);
+  if ((s & 1) == 0) {
+    *start = c.scan_in[a];
+    *len = c.scan_in[b] + c.cnt_in[b] - *start;
+  } else {
+    *start = n + c.scan_out[a];
+    *len = n + c.scan_out[b] + c.cnt_out[b] - *start;
+  }
+}
+
+// XCD-aware tile order: the 8 XCDs each take one contiguous block of tiles
+// (workgroup b runs on XCD b % 8), so neighbouring rows share an L2.
+__device__ __forceinline__ int tile_of_block(int b, int ntiles) {
+  const int chunk = (ntiles + 7) >> 3;
+  return (b & 7) * chunk + (b >> 3);
+}
+__host__ __device__ inline int tile_grid(int ntiles) { return ((ntiles + 7) >> 3) << 3; }
+
+struct TileSmem {
+  int seg_start[kSeg];
+  int seg_base[kSeg + 1];
+};
+
+// segments and staging; every thread of the block calls it
+__device__ void tile_stage(const TileDesc &t, const Geom &g, const Cells &c, int n,
+                           const double4 *__restrict__ pos, TileSmem &sm, double *sx, double *sy,
+                           double *sz, int stride) {
+  if (threadIdx.x < kSeg) {
+    int st, ln;
+    tile_segment(g, c, n, t.y, t.z, t.xf, t.xl, threadIdx.x, &st, &ln);
+    sm.seg_start[threadIdx.x] = st;
+    sm.seg_base[threadIdx.x + 1] = ln;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sm.seg_base[0] = 0;
+    for (int k = 0; k < kSeg; k++) sm.seg_base[k + 1] += sm.seg_base[k];
+  }
+  __syncthreads();
+  for (int k = 0; k < kSeg; k++) {
+    const int b = sm.seg_base[k], len = sm.seg_base[k + 1] - b, st = sm.seg_start[k];
+    for (int u = threadIdx.x; u < len; u += blockDim.x) {
+      const double4 q = pos[st + u];
+      sx[b + u] = q.x;
+      sy[b + u] = q.y;
+      sz[b + u] = q.z;
+    }
+  }
+  (void)stride;
+  __syncthreads();
+}
+
+// rows -> number of tiles
+__global__ void k_row_tiles(Geom g, Cells c, int *__restrict__ row_tiles) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int rows = g.ms[1] * g.ms[2];
+  if (r >= rows) return;
+  const int y = r % g.ms[1] + 1, z = r / g.ms[1] + 1;
+  const int a = cell_linear(g, 1, y, z), b = cell_linear(g, g.ms[0], y, z);
+  const int cnt = c.scan_in[b] + c.cnt_in[b] - c.scan_in[a];
+  row_tiles[r] = (cnt + kTile - 1) / kTile;
+}
+
+__global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict__ pos,
+                              const int *__restrict__ row_tiles, const int *__restrict__ row_off,
+                              TileDesc *__restrict__ tiles, double *__restrict__ partial) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int rows = g.ms[1] * g.ms[2];
+  double smax = 0.0;
+  if (r < rows) {
+    const int y = r % g.ms[1] + 1, z = r / g.ms[1] + 1;
+    const int a = cell_linear(g, 1, y, z), b = cell_linear(g, g.ms[0], y, z);
+    const int first = c.scan_in[a], cnt = c.scan_in[b] + c.cnt_in[b] - first;
+    for (int k = 0; k < row_tiles[r]; k++) {
+      TileDesc t;
+      t.i0 = first + k * kTile;
+      t.ni = min(kTile, cnt - k * kTile);
+      t.y = y;
+      t.z = z;
+      t.xf = interior_coord(pos[t.i0].x, g.inv_w[0], g.ms[0]);
+      t.xl = interior_coord(pos[t.i0 + t.ni - 1].x, g.inv_w[0], g.ms[0]);
+      int stage = 0;
+      for (int sgi = 0; sgi < kSeg; sgi++) {
+        int st, ln;
+        tile_segment(g, c, n, y, z, t.xf, t.xl, sgi, &st, &ln);
+        stage += ln;
+      }
+      t.stage = stage;
+      t.pad = 0;
+      tiles[row_off[r] + k] = t;
+      smax = fmax(smax, (double)stage);
+    }
+  }
+  block_max_to(smax, partial);
+}
+
+// Full neighbour list per tile: lane = real i, every partner j != i (real
+// or image) of its 27 cells with r^2 <= SL^2, r^2 formed as the reference's
+// AVX2 search, fma(dx,dx, fma(dy,dy, dz*dz)) with dq = q_j - q_i
+// (meshlist.cc:354-362).  Entry k of lane l sits at
+// list[tile * kcap * kTile + k * kTile + l] and is the partner's LDS offset.
+__global__ void __launch_bounds__(kTile) k_build_tiles(
+    const double4 *__restrict__ pos, const TileDesc *__restrict__ tiles, int ntiles, Geom g,
+    Cells c, int n, int kcap, unsigned short *__restrict__ list, int *__restrict__ nn,
+    double *__restrict__ partial, int *__restrict__ flags) {
+  extern __shared__ double stage_mem[];
+  __shared__ TileSmem sm;
+  const int tile = tile_of_block(blockIdx.x, ntiles);
+  if (tile >= ntiles) {
+    if (threadIdx.x == 0) partial[blockIdx.x] = 0.0;
+    return;
+  }
+  const TileDesc t = tiles[tile];
+  double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+  tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
+  const int lane = threadIdx.x;
   int cnt = 0;
-  if (i < n) {
-    const double4 qi = pos[i];
-    const int cx = interior_coord(qi.x, g.inv_w[0], g.ms[0]);
-    const int cy = interior_coord(qi.y, g.inv_w[1], g.ms[1]);
-    const int cz = interior_coord(qi.z, g.inv_w[2], g.ms[2]);
-    for (int dz = -1; dz <= 1; dz++)
-      for (int dy = -1; dy <= 1; dy++)
-        for (int dx = -1; dx <= 1; dx++) {
-          const int c = cell_linear(g, cx + dx, cy + dy, cz + dz);
-          const int b = cell_start[c], e = b + cell_count[c];
-          for (int j = b; j < e; j++) {
-            if (j == i) continue;
-            const double4 qj = pos[j];
-            const double ddx = qj.x - qi.x, ddy = qj.y - qi.y, ddz = qj.z - qi.z;
-            const double r2 = __fma_rn(ddx, ddx, __fma_rn(ddy, ddy, __dmul_rn(ddz, ddz)));
-            if (r2 <= g.SL2) {
-              if (cnt < kcap) nbr[(size_t)cnt * n_pad + i] = j;
-              cnt++;
-            }
+  if (lane < t.ni) {
+    const int i = t.i0 + lane;
+    const int self = sm.seg_base[8] + (i - sm.seg_start[8]);   // range (dy, dz) = (0, 0)
+    const double qx = sx[self], qy = sy[self], qz = sz[self];
+    const int cx = interior_coord(qx, g.inv_w[0], g.ms[0]);
+    unsigned short *out = list + (size_t)tile * kcap * kTile + lane;
+    for (int r = 0; r < 9; r++) {
+      const int y = t.y + r % 3 - 1, z = t.z + r / 3 - 1;
+      for (int dx = -1; dx <= 1; dx++) {
+        const int x = cx + dx;
+        const int cl = cell_linear(g, x, y, z);
+        int lo, len;
+        if (is_outer(g, x, y, z)) {
+          lo = sm.seg_base[2 * r + 1] + (n + c.scan_out[cl] - sm.seg_start[2 * r + 1]);
+          len = c.cnt_out[cl];
+        } else {
+          lo = sm.seg_base[2 * r] + (c.scan_in[cl] - sm.seg_start[2 * r]);
+          len = c.cnt_in[cl];
+        }
+        for (int o = lo; o < lo + len; o++) {
+          if (o == self) continue;
+          const double ddx = sx[o] - qx, ddy = sy[o] - qy, ddz = sz[o] - qz;
+          const double r2 = __fma_rn(ddx, ddx, __fma_rn(ddy, ddy, __dmul_rn(ddz, ddz)));
+          if (r2 <= g.SL2) {
+            if (cnt < kcap) out[(size_t)cnt * kTile] = (unsigned short)o;
+            cnt++;
           }
         }
+      }
+    }
     nn[i] = cnt < kcap ? cnt : kcap;
     if (cnt > kcap) atomicOr(flags, ST_LIST_OVERFLOW);
   }
@@ -407,8 +552,10 @@ __global__ void k_refresh_images(double4 *__restrict__ pos, const int *__restric
 // Drift by half a step fused with the image refresh, out of place:
 // out[i] = q_i + p_i dt/2 for the reals (fcalculator.cc:26-28) and, for
 // image k, the same drifted source plus its shift (mdunit.cc:181-191), so
-// the images equal a refresh after the drift bit for bit.
-__global__ void k_drift_refresh(const double4 *__restrict__ pos, const double4 *__restrict__ mom,
+// the images equal a refresh after the drift bit for bit.  Images are
+// also written back into pos, which keeps them at these mid-step positions
+// until the next refresh, as the reference's ghosts are.
+__global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__restrict__ mom,
                                 const int *__restrict__ gsrc, const int *__restrict__ gcode, int n,
                                 int g, double dt2, double Lx, double Ly, double Lz,
                                 double4 *__restrict__ out) {
@@ -428,40 +575,53 @@ __global__ void k_drift_refresh(const double4 *__restrict__ pos, const double4 *
   }
   q.w = 0.0;
   out[i] = q;
+  if (i >= n) pos[i] = q;
 }
 
-// Force kick: lane per real, loop over its full list (no reaction, so no
-// write conflicts).  Pair term as ForceCalculator::CalculateForceSorted
-// (fcalculator.cc:203-217): df = ((24 r^6 - 48) / (r^6 r^6 r^2) + 8 C2) dt,
-// zero beyond the cutoff; the quotient uses recip().
-//   MODE 0  p += F dt in place, block max |p|^2 to partial
-//   MODE 1  out = F (dt = 1), p untouched
-//   MODE 2  MODE 0 plus the trailing half drift written out of place
-//           (out[i] = q_i + p_i dt/2) and the images copied to out unchanged
+// Pair term of ForceCalculator::CalculateForceSorted (fcalculator.cc:203-217):
+// df = ((24 r^6 - 48) / (r^6 r^6 r^2) + 8 C2) dt, zero beyond the cutoff;
+// the quotient uses recip().
+__device__ __forceinline__ double pair_df(double r2, double C2_8, double dt, double CL2) {
+  const double r6 = r2 * r2 * r2;
+  const double df = ((24.0 * r6 - 48.0) * recip(r6 * r6 * r2) + C2_8) * dt;
+  return r2 > CL2 ? 0.0 : df;
+}
+
+// Force kick over the tile lists: lane per real, loop over its full list
+// with partners read from LDS (no reaction, so no write conflicts).
+//   MODE 0  p += F dt in place; block max |p|^2 to partial
+//   MODE 1  out[i] = F (dt = 1); p untouched
+//   MODE 2  MODE 0, plus the trailing half drift out of place:
+//           out[i] = q_i + p_i dt/2 (fcalculator.cc:26-28)
 template <int MODE>
-__global__ void __launch_bounds__(kBlock) k_kick(const double4 *__restrict__ pos,
-                                                 double4 *__restrict__ mom,
-                                                 const int *__restrict__ nbr,
-                                                 const int *__restrict__ nn, int n, int n_pad,
-                                                 int g, double C2_8, double dt, double CL2,
-                                                 double4 *__restrict__ out,
-                                                 double *__restrict__ partial) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kTile) k_kick_tiles(
+    const double4 *__restrict__ pos, double4 *__restrict__ mom, const TileDesc *__restrict__ tiles,
+    int ntiles, Geom g, Cells c, int n, int kcap, const unsigned short *__restrict__ list,
+    const int *__restrict__ nn, double C2_8, double dt, double4 *__restrict__ out,
+    double *__restrict__ partial) {
+  extern __shared__ double stage_mem[];
+  __shared__ TileSmem sm;
+  const int tile = tile_of_block(blockIdx.x, ntiles);
+  if (tile >= ntiles) {
+    if (MODE != 1 && threadIdx.x == 0) partial[blockIdx.x] = 0.0;
+    return;
+  }
+  const TileDesc t = tiles[tile];
+  double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+  tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
+  const int lane = threadIdx.x;
   double v2 = 0.0;
-  if (i < n) {
+  if (lane < t.ni) {
+    const int i = t.i0 + lane;
     const double4 qi = pos[i];
-    double fx = 0.0, fy = 0.0, fz = 0.0;
     const int m = nn[i];
-    const int *lst = nbr + i;
+    const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll 2
     for (int k = 0; k < m; k++) {
-      const int j = lst[(size_t)k * n_pad];
-      const double4 qj = pos[j];
-      const double dx = qj.x - qi.x, dy = qj.y - qi.y, dz = qj.z - qi.z;
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      const double r6 = r2 * r2 * r2;
-      double df = ((24.0 * r6 - 48.0) * recip(r6 * r6 * r2) + C2_8) * dt;
-      if (r2 > CL2) df = 0.0;
+      const int o = lst[(size_t)k * kTile];
+      const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
+      const double df = pair_df(dx * dx + dy * dy + dz * dz, C2_8, dt, g.CL2);
       fx += df * dx;
       fy += df * dy;
       fz += df * dz;
@@ -484,8 +644,6 @@ __global__ void __launch_bounds__(kBlock) k_kick(const double4 *__restrict__ pos
         out[i] = q;
       }
     }
-  } else if (MODE == 2 && i < n + g) {
-    out[i] = pos[i];
   }
   if (MODE != 1) block_max_to(v2, partial);
 }
@@ -501,30 +659,39 @@ __global__ void k_vmax(const double4 *__restrict__ mom, int n, double *__restric
 }
 
 // ------------------------------------------------------------ observers
-// Per block: KE = sum 0.5 p^2 (observer.cc:5-15), and over the full list
+// Per tile: KE = sum 0.5 p^2 (observer.cc:5-15), and over the full list
 // PE = sum 0.5 * 4 (r^-12 - r^-6 + C2 r^2 + C0) (observer.cc:18-47) and
 // virial = sum 0.5 * r V'(r) (observer.cc:50-79); every pair sits in two
 // lists, hence the halves.  Fixed-order tree within the block.
-__global__ void __launch_bounds__(kBlock) k_observe(const double4 *__restrict__ pos,
-                                                    const double4 *__restrict__ mom,
-                                                    const int *__restrict__ nbr,
-                                                    const int *__restrict__ nn, int n, int n_pad,
-                                                    double C0, double C2, double CL2,
-                                                    double *__restrict__ partial) {
-  __shared__ double lds[3][kBlock];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kTile) k_observe_tiles(
+    const double4 *__restrict__ pos, const double4 *__restrict__ mom,
+    const TileDesc *__restrict__ tiles, int ntiles, Geom g, Cells c, int n, int kcap,
+    const unsigned short *__restrict__ list, const int *__restrict__ nn, double C0, double C2,
+    double *__restrict__ partial) {
+  extern __shared__ double stage_mem[];
+  __shared__ TileSmem sm;
+  __shared__ double red[3][kTile];
+  const int tile = tile_of_block(blockIdx.x, ntiles);
+  if (tile >= ntiles) {
+    if (threadIdx.x < 3) partial[(size_t)blockIdx.x * 3 + threadIdx.x] = 0.0;
+    return;
+  }
+  const TileDesc t = tiles[tile];
+  double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+  tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
+  const int lane = threadIdx.x;
   double ke = 0.0, pe = 0.0, vir = 0.0;
-  if (i < n) {
+  if (lane < t.ni) {
+    const int i = t.i0 + lane;
     const double4 p = mom[i];
     ke = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z;
     const double4 qi = pos[i];
-    const int m = nn[i];
-    for (int k = 0; k < m; k++) {
-      const int j = nbr[(size_t)k * n_pad + i];
-      const double4 qj = pos[j];
-      const double dx = qi.x - qj.x, dy = qi.y - qj.y, dz = qi.z - qj.z;
+    const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
+    for (int k = 0; k < nn[i]; k++) {
+      const int o = lst[(size_t)k * kTile];
+      const double dx = qi.x - sx[o], dy = qi.y - sy[o], dz = qi.z - sz[o];
       const double r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 > CL2) continue;
+      if (r2 > g.CL2) continue;
       const double r6 = r2 * r2 * r2;
       pe += 4.0 * (1.0 / (r2 * r2 * r2 * r2 * r2 * r2) - 1.0 / r6 + C2 * r2 + C0);
       vir += ((24.0 * r6 - 48.0) / (r6 * r6 * r2) + C2 * 8.0) * r2;
@@ -532,20 +699,19 @@ __global__ void __launch_bounds__(kBlock) k_observe(const double4 *__restrict__ 
     pe *= 0.5;
     vir *= 0.5;
   }
-  const int t = threadIdx.x;
-  lds[0][t] = ke;
-  lds[1][t] = pe;
-  lds[2][t] = vir;
+  red[0][lane] = ke;
+  red[1][lane] = pe;
+  red[2][lane] = vir;
   __syncthreads();
-  for (int off = kBlock / 2; off > 0; off >>= 1) {
-    if (t < off) {
-      lds[0][t] += lds[0][t + off];
-      lds[1][t] += lds[1][t + off];
-      lds[2][t] += lds[2][t + off];
+  for (int off = kTile / 2; off > 0; off >>= 1) {
+    if (lane < off) {
+      red[0][lane] += red[0][lane + off];
+      red[1][lane] += red[1][lane + off];
+      red[2][lane] += red[2][lane + off];
     }
     __syncthreads();
   }
-  if (t < 3) partial[(size_t)blockIdx.x * 3 + t] = lds[t][0];
+  if (lane < 3) partial[(size_t)blockIdx.x * 3 + lane] = red[lane][0];
 }
 
 __global__ void k_sum_partials(const double *__restrict__ partial, int nblocks,
@@ -628,21 +794,44 @@ __device__ __forceinline__ void partner_id(int j, int n, const int *tag, const i
   }
 }
 
-// each pair sits in both lists; emit it from the side with the smaller id
+// each pair sits in both lists; emit it from the side with the smaller id.
+// LDS offsets map back to global slots through the tile's segments.
 template <bool WRITE>
-__global__ void k_canonical(const int *__restrict__ nbr, const int *__restrict__ nn,
-                            const int *__restrict__ tag, const int *__restrict__ gsrc,
-                            const int *__restrict__ gcode, int n, int n_pad,
-                            const int *__restrict__ offset, int *__restrict__ count,
-                            int *__restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ void __launch_bounds__(kTile) k_canonical_tiles(
+    const TileDesc *__restrict__ tiles, int ntiles, Geom g, Cells c, int n, int kcap,
+    const unsigned short *__restrict__ list, const int *__restrict__ nn, const int *__restrict__ tag,
+    const int *__restrict__ gsrc, const int *__restrict__ gcode, const int *__restrict__ offset,
+    int *__restrict__ count, int *__restrict__ out) {
+  __shared__ TileSmem sm;
+  const int tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  const TileDesc t = tiles[tile];
+  if (threadIdx.x < kSeg) {
+    int st, ln;
+    tile_segment(g, c, n, t.y, t.z, t.xf, t.xl, threadIdx.x, &st, &ln);
+    sm.seg_start[threadIdx.x] = st;
+    sm.seg_base[threadIdx.x + 1] = ln;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sm.seg_base[0] = 0;
+    for (int k = 0; k < kSeg; k++) sm.seg_base[k + 1] += sm.seg_base[k];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x;
+  if (lane >= t.ni) return;
+  const int i = t.i0 + lane;
   const int ti = tag[i];
-  int c = 0;
+  const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
+  int cnum = 0;
   int w = WRITE ? offset[i] : 0;
   for (int k = 0; k < nn[i]; k++) {
+    const int o = lst[(size_t)k * kTile];
+    int sgi = 0;
+    while (o >= sm.seg_base[sgi + 1]) sgi++;
+    const int j = sm.seg_start[sgi] + (o - sm.seg_base[sgi]);
     int tj, code;
-    partner_id(nbr[(size_t)k * n_pad + i], n, tag, gsrc, gcode, &tj, &code);
+    partner_id(j, n, tag, gsrc, gcode, &tj, &code);
     if (ti < tj) {
       if (WRITE) {
         out[3 * (size_t)w + 0] = ti;
@@ -650,10 +839,10 @@ __global__ void k_canonical(const int *__restrict__ nbr, const int *__restrict__
         out[3 * (size_t)w + 2] = code;
         w++;
       }
-      c++;
+      cnum++;
     }
   }
-  if (!WRITE) count[i] = c;
+  if (!WRITE) count[i] = cnum;
 }
 
 __global__ void k_scatter_by_tag(const double4 *__restrict__ src, const int *__restrict__ tag, int n,
@@ -692,9 +881,8 @@ struct mdx_system {
   double SL = 0, C0 = 0, C2 = 0;
   int ncell = 0;
   // sizes
-  int n = 0, g = 0, cap = 0, gcap = 0, n_pad = 0, kcap = 0;
+  int n = 0, g = 0, cap = 0, gcap = 0, kcap = 0;
   size_t tot_elems = 0;     // length of pos / ref_pos / cellid / ... arrays
-  long long list_elems = 0;
   // device buffers
   double4 *pos = nullptr, *pos_alt = nullptr, *pos_tmp = nullptr, *ref_pos = nullptr, *mom = nullptr,
           *mom_tmp = nullptr, *gpos_tmp = nullptr, *vec_out = nullptr;
@@ -704,7 +892,15 @@ struct mdx_system {
       *goff = nullptr;
   int *cell_count = nullptr, *cell_start = nullptr, *cnt_in = nullptr, *cnt_out = nullptr,
       *scan_in = nullptr, *scan_out = nullptr, *scan_aux = nullptr;
-  int *nbr = nullptr, *nn = nullptr;
+  int *nn = nullptr;
+  // tiles and their 16-bit lists
+  TileDesc *tiles = nullptr;
+  int ntiles = 0, tiles_cap = 0, stage_max = 0;
+  int *row_tiles = nullptr, *row_off = nullptr;
+  unsigned short *list16 = nullptr;
+  size_t list16_elems = 0;
+  size_t lds_limit = 65536;
+  int partial_blocks = 0;
   double *disp = nullptr, *partial = nullptr, *pmax = nullptr;
   int *cand = nullptr;
   double *io_buf = nullptr;   // [cap][3] staging for host transfers
@@ -747,7 +943,7 @@ int fail(mdx_system *s, int code, const char *fmt, ...) {
 template <class T>
 int dev_alloc(mdx_system *s, T **p, size_t count) {
   if (*p) {
-    hipFree(*p);
+    (void)hipFree(*p);
     *p = nullptr;
   }
   if (count == 0) count = 1;
@@ -778,6 +974,8 @@ int scan(mdx_system *s, const int *in, int *out, int count, int *grand) {
   return launch_check(s, "scan");
 }
 
+int ensure_partials(mdx_system *s, int blocks);
+
 int ensure_capacity(mdx_system *s, int n) {
   if (n <= s->cap && s->cap > 0) return MDX_OK;
   int cap = std::max(n, s->prm.capacity);
@@ -791,9 +989,7 @@ int ensure_capacity(mdx_system *s, int n) {
     return rc;
   s->cap = cap;
   const int tiles = (std::max(cap, s->ncell) + kScanTile - 1) / kScanTile + 1;
-  if ((rc = dev_alloc(s, &s->scan_aux, tiles)) ||
-      (rc = dev_alloc(s, &s->partial, 3 * (size_t)nblk(cap) + 3)))
-    return rc;
+  if ((rc = dev_alloc(s, &s->scan_aux, tiles))) return rc;
   return MDX_OK;
 }
 
@@ -809,12 +1005,12 @@ int ensure_image_capacity(mdx_system *s, int gcap) {
     const size_t keep = std::min(tot, s->tot_elems);
     HIPCHK(s, hipMemcpyAsync(npos, s->pos, keep * sizeof(double4), hipMemcpyDeviceToDevice, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    hipFree(s->pos);
+    (void)hipFree(s->pos);
   }
   s->pos = npos;
   int rc;
-  if ((rc = dev_alloc(s, &s->ref_pos, tot)) || (rc = dev_alloc(s, &s->pos_alt, tot)) ||
-      (rc = dev_alloc(s, &s->pmax, (size_t)nblk((long long)tot))) ||
+  if ((rc = ensure_partials(s, nblk((long long)tot))) ||
+      (rc = dev_alloc(s, &s->ref_pos, tot)) || (rc = dev_alloc(s, &s->pos_alt, tot)) ||
       (rc = dev_alloc(s, &s->gpos_tmp, gcap)) || (rc = dev_alloc(s, &s->gsrc, gcap)) ||
       (rc = dev_alloc(s, &s->gcode, gcap)) || (rc = dev_alloc(s, &s->gsrc_tmp, gcap)) ||
       (rc = dev_alloc(s, &s->gcode_tmp, gcap)) || (rc = dev_alloc(s, &s->cellid, tot)) ||
@@ -828,22 +1024,37 @@ int ensure_image_capacity(mdx_system *s, int gcap) {
 }
 
 int ensure_list_capacity(mdx_system *s, int kcap) {
-  const long long need = (long long)kcap * s->n_pad;
-  if (need <= s->list_elems && kcap <= s->kcap) return MDX_OK;
-  int rc = dev_alloc(s, &s->nbr, (size_t)need);
+  const size_t need = (size_t)std::max(s->ntiles, 1) * kcap * kTile;
+  if (need <= s->list16_elems && kcap <= s->kcap) return MDX_OK;
+  int rc = dev_alloc(s, &s->list16, need);
   if (rc) return rc;
-  s->list_elems = need;
+  s->list16_elems = need;
   s->kcap = kcap;
   return MDX_OK;
 }
 
+size_t stage_bytes(const mdx_system *s) { return (size_t)s->stage_max * 3 * sizeof(double); }
+
+Cells cells_of(const mdx_system *s) { return Cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out}; }
+
+// per-block partial buffers for every grid the library launches
+int ensure_partials(mdx_system *s, int blocks) {
+  if (blocks <= s->partial_blocks) return MDX_OK;
+  int rc;
+  if ((rc = dev_alloc(s, &s->pmax, (size_t)blocks)) ||
+      (rc = dev_alloc(s, &s->partial, 3 * (size_t)blocks)))
+    return rc;
+  s->partial_blocks = blocks;
+  return MDX_OK;
+}
+
 void ev_record(mdx_system *s, int k) {
-  if (s->timing) hipEventRecord(s->ev[k], s->stream);
+  if (s->timing) (void)hipEventRecord(s->ev[k], s->stream);
 }
 
 float ev_ms(mdx_system *s, int a, int b) {
   float ms = 0.f;
-  if (s->timing) hipEventElapsedTime(&ms, s->ev[a], s->ev[b]);
+  if (s->timing) (void)hipEventElapsedTime(&ms, s->ev[a], s->ev[b]);
   return ms;
 }
 
@@ -910,20 +1121,41 @@ int build_pairlist(mdx_system *s) {
                                                         s->gcode_tmp, s->inv, s->pos, s->gsrc,
                                                         s->gcode);
   if ((rc = launch_check(s, "sort"))) return rc;
-  // 5. neighbour list, one pass at the current width, again if it overflows
+  // 5. tiles over the interior cell rows
+  const Cells cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out};
+  const int rows = s->geom.ms[1] * s->geom.ms[2];
+  k_row_tiles<<<nblk(rows), kBlock, 0, s->stream>>>(s->geom, cells, s->row_tiles);
+  if ((rc = scan(s, s->row_tiles, s->row_off, rows, &s->dstat->ntiles))) return rc;
+  if ((rc = sync_status(s))) return rc;
+  s->ntiles = s->hstat->ntiles;
+  if (s->ntiles > s->tiles_cap) {
+    if ((rc = dev_alloc(s, &s->tiles, (size_t)s->ntiles + s->ntiles / 8 + 64))) return rc;
+    s->tiles_cap = s->ntiles + s->ntiles / 8 + 64;
+  }
+  if ((rc = ensure_partials(s, std::max(tile_grid(s->ntiles), nblk(rows))))) return rc;
+  k_write_tiles<<<nblk(rows), kBlock, 0, s->stream>>>(s->geom, cells, n, s->pos, s->row_tiles,
+                                                       s->row_off, s->tiles, s->pmax);
+  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(rows), &s->dstat->stage_max);
+  if ((rc = sync_status(s))) return rc;
+  s->stage_max = (int)s->hstat->stage_max;
+  if ((size_t)s->stage_max * 3 * sizeof(double) + 4096 > s->lds_limit)
+    return fail(s, MDX_ERR_CAPACITY, "tile staging of %d particles exceeds LDS (%zu B)",
+                s->stage_max, s->lds_limit);
+  if ((rc = ensure_list_capacity(s, s->kcap))) return rc;
+  // 6. neighbour lists, one pass at the current width, again if it overflows
   for (int attempt = 0; attempt < 3; attempt++) {
     HIPCHK(s, hipMemsetAsync(&s->dstat->flags, 0, sizeof(int), s->stream));
-    k_build_list<<<nblk(n), kBlock, 0, s->stream>>>(s->pos, s->cell_start, s->cell_count, n,
-                                                     s->n_pad, s->kcap, s->geom, s->nbr, s->nn,
-                                                     s->pmax, &s->dstat->flags);
-    k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(n), &s->dstat->kmax_d);
+    k_build_tiles<<<tile_grid(s->ntiles), kTile, stage_bytes(s), s->stream>>>(
+        s->pos, s->tiles, s->ntiles, s->geom, cells, n, s->kcap, s->list16, s->nn, s->pmax,
+        &s->dstat->flags);
+    k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, tile_grid(s->ntiles), &s->dstat->kmax_d);
     if ((rc = launch_check(s, "list"))) return rc;
     if ((rc = sync_status(s))) return rc;
     if (!(s->hstat->flags & ST_LIST_OVERFLOW)) break;
     if ((rc = ensure_list_capacity(s, (int)s->hstat->kmax_d + 8))) return rc;
   }
   if (s->hstat->flags & ST_LIST_OVERFLOW) return fail(s, MDX_ERR_CAPACITY, "list capacity");
-  // 6. PairList::Init: snapshot of reals and images
+  // 7. PairList::Init: snapshot of reals and images
   HIPCHK(s, hipMemcpyAsync(s->ref_pos, s->pos, sizeof(double4) * tn, hipMemcpyDeviceToDevice, s->stream));
   ev_record(s, 1);
   s->buffer_length = s->prm.buffer_length;
@@ -937,34 +1169,35 @@ int build_pairlist(mdx_system *s) {
 }
 
 int kick(mdx_system *s) {
+  const int grid = tile_grid(s->ntiles);
   ev_record(s, 2);
-  k_kick<0><<<nblk(s->n), kBlock, 0, s->stream>>>(s->pos, s->mom, s->nbr, s->nn, s->n, s->n_pad, 0,
-                                                   8.0 * s->C2, s->prm.dt, s->geom.CL2, nullptr,
-                                                   s->pmax);
+  k_kick_tiles<0><<<grid, kTile, stage_bytes(s), s->stream>>>(
+      s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), s->n, s->kcap, s->list16, s->nn,
+      8.0 * s->C2, s->prm.dt, nullptr, s->pmax);
   ev_record(s, 3);
-  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(s->n), &s->dstat->vmax2);
+  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, grid, &s->dstat->vmax2);
   s->vmax_valid = true;
   s->n_kick++;
   return launch_check(s, "kick");
 }
 
-// One NVE step after the expiry check, in two passes over double-buffered
-// positions: drift + image refresh, then kick + drift (images carried over
-// at their mid-step positions, as the reference leaves them until the next
-// refresh).  Same arithmetic as drift(); refresh(); kick(); drift().
+// One NVE step after the expiry check, in two passes: drift + image
+// refresh into pos_alt (images also written back to pos), then kick +
+// trailing drift reading pos_alt and writing the reals of pos.  pos ends
+// as the reference's state after Calculate(): reals at the end of the step,
+// images at their mid-step refresh.  Same arithmetic as
+// drift(); refresh(); kick(); drift().
 int fused_step(mdx_system *s) {
-  const int n = s->n, g = s->g;
+  const int n = s->n, g = s->g, grid = tile_grid(s->ntiles);
   k_drift_refresh<<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->gsrc, s->gcode, n, g,
                                                           s->prm.dt * 0.5, s->geom.L[0],
                                                           s->geom.L[1], s->geom.L[2], s->pos_alt);
-  std::swap(s->pos, s->pos_alt);
   ev_record(s, 2);
-  k_kick<2><<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->nbr, s->nn, n, s->n_pad, g,
-                                                    8.0 * s->C2, s->prm.dt, s->geom.CL2, s->pos_alt,
-                                                    s->pmax);
+  k_kick_tiles<2><<<grid, kTile, stage_bytes(s), s->stream>>>(
+      s->pos_alt, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn,
+      8.0 * s->C2, s->prm.dt, s->pos, s->pmax);
   ev_record(s, 3);
-  std::swap(s->pos, s->pos_alt);
-  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(n + g), &s->dstat->vmax2);
+  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, grid, &s->dstat->vmax2);
   s->vmax_valid = true;
   s->n_kick++;
   return launch_check(s, "step");
@@ -1032,9 +1265,11 @@ int expired(mdx_system *s, int *out) {
 
 int reduce_observe(mdx_system *s, double *ke, double *pe, double *vir) {
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "observe before a pairlist exists");
-  const int nb = nblk(s->n);
-  k_observe<<<nb, kBlock, 0, s->stream>>>(s->pos, s->mom, s->nbr, s->nn, s->n, s->n_pad, s->C0,
-                                          s->C2, s->geom.CL2, s->partial);
+  const int nb = tile_grid(s->ntiles);
+  k_observe_tiles<<<nb, kTile, stage_bytes(s), s->stream>>>(s->pos, s->mom, s->tiles, s->ntiles,
+                                                            s->geom, cells_of(s), s->n, s->kcap,
+                                                            s->list16, s->nn, s->C0, s->C2,
+                                                            s->partial);
   k_sum_partials<<<1, kBlock, 0, s->stream>>>(s->partial, nb, s->dstat->sums);
   int rc = launch_check(s, "observe");
   if (rc) return rc;
@@ -1109,8 +1344,14 @@ int mdx_create(const mdx_params *prm, mdx_system **out) {
   if (!rc && ((rc = dev_alloc(s, &s->cell_count, s->ncell)) || (rc = dev_alloc(s, &s->cell_start, s->ncell)) ||
               (rc = dev_alloc(s, &s->cnt_in, s->ncell)) || (rc = dev_alloc(s, &s->cnt_out, s->ncell)) ||
               (rc = dev_alloc(s, &s->scan_in, s->ncell)) || (rc = dev_alloc(s, &s->scan_out, s->ncell)) ||
-              (rc = dev_alloc(s, &s->dstat, 1))))
+              (rc = dev_alloc(s, &s->dstat, 1)) ||
+              (rc = dev_alloc(s, &s->row_tiles, s->geom.ms[1] * s->geom.ms[2])) ||
+              (rc = dev_alloc(s, &s->row_off, s->geom.ms[1] * s->geom.ms[2]))))
     ;
+  if (!rc) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, prm->device) == hipSuccess) s->lds_limit = prop.sharedMemPerBlock;
+  }
   if (!rc && hipHostMalloc((void **)&s->hstat, sizeof(DevStatus)) != hipSuccess) rc = MDX_ERR_HIP;
   if (!rc && prm->capacity > 0) rc = ensure_capacity(s, prm->capacity);
   if (rc) {
@@ -1124,18 +1365,19 @@ int mdx_create(const mdx_params *prm, mdx_system **out) {
 
 int mdx_destroy(mdx_system *s) {
   if (!s) return MDX_OK;
-  if (s->stream) hipStreamSynchronize(s->stream);
-  void *bufs[] = {s->pos, s->pos_alt, s->pmax, s->pos_tmp, s->ref_pos, s->mom, s->mom_tmp, s->gpos_tmp, s->vec_out,
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  void *bufs[] = {s->pos, s->pos_alt, s->pmax, s->tiles, s->row_tiles, s->row_off, s->list16,
+                  s->pos_tmp, s->ref_pos, s->mom, s->mom_tmp, s->gpos_tmp, s->vec_out,
                   s->tag, s->tag_tmp, s->gsrc, s->gcode, s->gsrc_tmp, s->gcode_tmp, s->cellid,
                   s->slot, s->order, s->inv, s->nimg, s->goff, s->cell_count, s->cell_start,
-                  s->cnt_in, s->cnt_out, s->scan_in, s->scan_out, s->scan_aux, s->nbr, s->nn,
+                  s->cnt_in, s->cnt_out, s->scan_in, s->scan_out, s->scan_aux, s->nn,
                   s->disp, s->partial, s->cand, s->io_buf, s->i_buf, s->dstat};
   for (void *b : bufs)
-    if (b) hipFree(b);
-  if (s->hstat) hipHostFree(s->hstat);
+    if (b) (void)hipFree(b);
+  if (s->hstat) (void)hipHostFree(s->hstat);
   for (auto &e : s->ev)
-    if (e) hipEventDestroy(e);
-  if (s->own_stream && s->stream) hipStreamDestroy(s->stream);
+    if (e) (void)hipEventDestroy(e);
+  if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return MDX_OK;
 }
@@ -1149,7 +1391,6 @@ int mdx_set_state(mdx_system *s, int n, const double *q3, const double *p3) {
   int rc = ensure_capacity(s, n);
   if (rc) return rc;
   s->n = n;
-  s->n_pad = (n + 63) & ~63;
   // initial image capacity: the SL shell around the box at the mean
   // density, doubled, plus slack; grown on demand
   const double V = s->geom.L[0] * s->geom.L[1] * s->geom.L[2];
@@ -1158,7 +1399,7 @@ int mdx_set_state(mdx_system *s, int n, const double *q3, const double *p3) {
   if ((rc = ensure_image_capacity(s, gguess))) return rc;
   const double rho = n / V;
   const int kguess = (int)(4.18879 * s->SL * s->SL * s->SL * rho * 1.35) + 16;
-  if ((rc = ensure_list_capacity(s, std::max(kguess, s->kcap)))) return rc;
+  s->kcap = std::max(kguess, s->kcap);
   // positions staged in io_buf, momenta in vec_out (double4[cap] >= 3n doubles)
   double *pbuf = reinterpret_cast<double *>(s->vec_out);
   HIPCHK(s, hipMemcpyAsync(s->io_buf, q3, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s->stream));
@@ -1257,9 +1498,9 @@ int mdx_observe(mdx_system *s, double *ke, double *pe, double *vir) {
 int mdx_forces(mdx_system *s, double *f3) {
   if (!s || !f3) return fail(s, MDX_ERR_ARG, "null argument");
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
-  k_kick<1><<<nblk(s->n), kBlock, 0, s->stream>>>(s->pos, s->mom, s->nbr, s->nn, s->n, s->n_pad, 0,
-                                                   8.0 * s->C2, 1.0, s->geom.CL2, s->vec_out,
-                                                   nullptr);
+  k_kick_tiles<1><<<tile_grid(s->ntiles), kTile, stage_bytes(s), s->stream>>>(
+      s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), s->n, s->kcap, s->list16, s->nn,
+      8.0 * s->C2, 1.0, s->vec_out, nullptr);
   k_scatter_by_tag<<<nblk(s->n), kBlock, 0, s->stream>>>(s->vec_out, s->tag, s->n, s->io_buf);
   HIPCHK(s, hipMemcpyAsync(f3, s->io_buf, sizeof(double) * 3 * s->n, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(s, hipStreamSynchronize(s->stream));
@@ -1270,8 +1511,9 @@ int mdx_pairs_canonical(mdx_system *s, int32_t *out, int64_t cap, int64_t *count
   if (!s) return MDX_ERR_ARG;
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
   const int n = s->n;
-  k_canonical<false><<<nblk(n), kBlock, 0, s->stream>>>(s->nbr, s->nn, s->tag, s->gsrc, s->gcode, n,
-                                                         s->n_pad, nullptr, s->i_buf, nullptr);
+  k_canonical_tiles<false><<<s->ntiles, kTile, 0, s->stream>>>(
+      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->tag, s->gsrc,
+      s->gcode, nullptr, s->i_buf, nullptr);
   int rc = scan(s, s->i_buf, s->goff, n, &s->dstat->ghost_total);
   if (rc) return rc;
   if ((rc = sync_status(s))) return rc;
@@ -1280,13 +1522,14 @@ int mdx_pairs_canonical(mdx_system *s, int32_t *out, int64_t cap, int64_t *count
   if (!out) return MDX_OK;
   int32_t *dout = nullptr;
   HIPCHK(s, hipMalloc((void **)&dout, sizeof(int32_t) * 3 * (size_t)std::max(total, 1LL)));
-  k_canonical<true><<<nblk(n), kBlock, 0, s->stream>>>(s->nbr, s->nn, s->tag, s->gsrc, s->gcode, n,
-                                                        s->n_pad, s->goff, nullptr, dout);
+  k_canonical_tiles<true><<<s->ntiles, kTile, 0, s->stream>>>(
+      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->tag, s->gsrc,
+      s->gcode, s->goff, nullptr, dout);
   std::vector<int32_t> h(3 * (size_t)total);
   hipError_t e = hipMemcpyAsync(h.data(), dout, sizeof(int32_t) * 3 * (size_t)total,
                                 hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  hipFree(dout);
+  (void)hipFree(dout);
   if (e != hipSuccess) return fail(s, MDX_ERR_HIP, "canonical copy: %s", hipGetErrorString(e));
   std::vector<long long> idx(total);
   for (long long k = 0; k < total; k++) idx[k] = k;
