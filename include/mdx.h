@@ -107,6 +107,34 @@ int mdx_pair_number(mdx_system *sys, int64_t *count);
  * Writes min(cap, count) triples; *count receives the full count. */
 int mdx_pairs_canonical(mdx_system *sys, int32_t *out, int64_t cap, int64_t *count);
 
+/* Differentiable form (no counterpart in the reference, whose epsilon and
+ * sigma are hard-wired to 1, variables.cc:16-33): V(r) = eps v(r / sigma)
+ * with v the reference's shifted LJ (variables.cc:21-27) cut at 3 sigma.
+ * From the loaded state, nsteps drift-kick-drift steps; over the last nobs
+ * of them (observed after each step)
+ *   loss = (mean P - p_target)^2 + (mean PE/N - pe_target)^2,
+ *   P = (2 KE / 3 - W) / V  (MDManager::Pressure, mdmanager.cc:563-570),
+ *   W = sum r V'(r) / 3     (VirialObserver, observer.cc:50-79).
+ * The pairlist is rebuilt every checkpoint_every steps (and must stay valid
+ * over each interval: MDX_ERR_STATE otherwise).  With grad != 0 the
+ * adjoint (Hessian-vector products of the forces, dF/deps, dF/dsigma)
+ * runs backwards over the checkpoints, recomputing each interval, and
+ * fills dloss_deps, dloss_dsigma and, if dloss_dq0 is not NULL, dL/dq0 as
+ * [n][3] by id.  Leaves the system at the final state without a list. */
+typedef struct mdx_lj_result {
+  double loss;
+  double pressure_mean;
+  double pe_mean;         /* per particle */
+  double dloss_deps;
+  double dloss_dsigma;
+  double forward_ms;      /* wall time of the forward pass */
+  double backward_ms;     /* wall time of the reverse sweep (0 without grad) */
+} mdx_lj_result;
+
+int mdx_lj_trajectory(mdx_system *sys, double eps, double sigma, int nsteps, int nobs,
+                      int checkpoint_every, double p_target, double pe_target, int grad,
+                      mdx_lj_result *out, double *dloss_dq0);
+
 /* cumulative device time (ms) of the pairlist build, force kick and
  * remaining step kernels, and counts, measured with HIP events since the
  * last reset; any pointer may be NULL. */

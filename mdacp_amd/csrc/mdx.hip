@@ -21,7 +21,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -60,6 +62,11 @@ struct DevStatus {
   double sums[4];                  // KE, PE, virial, spare
 };
 
+// |p|^2 rounded as written (pairlist.cc:52), identical in every kernel
+__device__ __forceinline__ double speed2(const double4 &p) {
+  return __dadd_rn(__dadd_rn(__dmul_rn(p.x, p.x), __dmul_rn(p.y, p.y)), __dmul_rn(p.z, p.z));
+}
+
 __device__ __forceinline__ double wave_max(double v) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
   return v;
@@ -91,6 +98,51 @@ __global__ void k_max_partials(const double *__restrict__ partial, int nb, doubl
     __syncthreads();
   }
   if (threadIdx.x == 0) *out = lds[0];
+}
+
+// Device side of the pairlist velocity budget (pairlist.cc:48-65), so that
+// steps can be queued without a host round trip each.  The guard kernel
+// ending step b folds |p|max^2 of that step's kick, records it in ring[b],
+// and applies the budget update the reference makes at the start of the
+// next step; if the budget could be spent (below a margin that covers
+// rounding of the device sqrt) or a particle is too fast, it raises halt
+// and every later kernel of the batch returns at once.  The host replays
+// the budget exactly from ring[] (steps_done entries).
+constexpr int kRing = 64;
+struct StepCtl {
+  double budget, dt, buffer;
+  int halt, steps_done, too_fast, pad;
+  double ring[kRing];
+};
+
+__global__ void k_step_guard(const double *__restrict__ partial, int nb, StepCtl *__restrict__ ctl) {
+  __shared__ double lds[kBlock];
+  if (ctl->halt) return;
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) m = fmax(m, partial[b]);
+  lds[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double v2 = lds[0];
+    const int k = ctl->steps_done;
+    ctl->ring[k] = v2;
+    ctl->steps_done = k + 1;
+    double b = ctl->budget;
+    if (b > 0.0) {
+      const double vmax = sqrt(v2);
+      if (vmax * 2.0 * ctl->dt > ctl->buffer) {
+        ctl->too_fast = 1;
+        ctl->halt = 1;
+      }
+      b = b - vmax * 2.0 * ctl->dt;
+      ctl->budget = b;
+    }
+    if (b < 1e-9 || k + 1 >= kRing) ctl->halt = 1;
+  }
 }
 
 // 1/x from the hardware estimate and two Newton steps (each squares the
@@ -348,6 +400,7 @@ __global__ void k_gather_images(const int *__restrict__ order, int n, int g,
 // Neighbour lists hold 16-bit offsets into that staging.
 constexpr int kTile = 128;
 constexpr int kSeg = 18;
+constexpr int kILP = 8;   // independent partner chains per lane in the kick
 
 struct TileDesc {
   int i0, ni, y, z, xf, xl, stage, pad;
@@ -558,7 +611,8 @@ __global__ void k_refresh_images(double4 *__restrict__ pos, const int *__restric
 __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__restrict__ mom,
                                 const int *__restrict__ gsrc, const int *__restrict__ gcode, int n,
                                 int g, double dt2, double Lx, double Ly, double Lz,
-                                double4 *__restrict__ out) {
+                                double4 *__restrict__ out, const int *__restrict__ halt) {
+  if (halt && *halt) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n + g) return;
   const int src = i < n ? i : gsrc[i - n];
@@ -587,20 +641,61 @@ __device__ __forceinline__ double pair_df(double r2, double C2_8, double dt, dou
   return r2 > CL2 ? 0.0 : df;
 }
 
+// Parameters of the differentiable form V(r) = eps v(r / sigma), v the
+// reference's shifted LJ cut at r / sigma = 3 (so the cutoff is 3 sigma).
+// With s = r^2 and u = s / sigma^2:
+//   phi(s)  = V'(r) / r = (eps / sigma^2) phi0(u),  phi0 = 24 u^-4 - 48 u^-7 + 8 C2
+//   phi'(s)             = (eps / sigma^4) phi0'(u), phi0' = -96 u^-5 + 336 u^-8
+//   V       = eps v0(u),  v0 = 4 (u^-6 - u^-3 + C2 u + C0)
+//   g = r V'(r) = s phi (virial term)
+// and the parameter derivatives
+//   dphi/deps = phi / eps,   dphi/dsigma = -(2 / sigma) (phi + s phi')
+//   dV/deps   = V / eps,     dV/dsigma   = -g / sigma
+//   dg/deps   = g / eps,     dg/dsigma   = -(2 / sigma) s (phi + s phi').
+// At eps = sigma = 1 the force term equals pair_df exactly.
+struct LJ {
+  double eps, sigma, inv_sig2, es2, es4, cl2, C0, C2;
+};
+
+__device__ __forceinline__ double pair_df_lj(double s, const LJ &p, double dt) {
+  const double u = s * p.inv_sig2;
+  const double u6 = u * u * u;
+  const double df = p.es2 * ((24.0 * u6 - 48.0) * recip(u6 * u6 * u) + 8.0 * p.C2) * dt;
+  return s > p.cl2 ? 0.0 : df;
+}
+
+// phi, phi' and V of one pair (zero beyond the cutoff)
+__device__ __forceinline__ void lj_terms(double s, const LJ &p, double *phi, double *dphi,
+                                         double *V) {
+  if (s > p.cl2) {
+    *phi = *dphi = *V = 0.0;
+    return;
+  }
+  const double u = s * p.inv_sig2;
+  const double iu = recip(u);
+  const double iu2 = iu * iu, iu3 = iu2 * iu, iu4 = iu2 * iu2;
+  const double iu6 = iu3 * iu3, iu7 = iu6 * iu, iu8 = iu4 * iu4;
+  *phi = p.es2 * (24.0 * iu4 - 48.0 * iu7 + 8.0 * p.C2);
+  *dphi = p.es4 * (-96.0 * iu4 * iu + 336.0 * iu8);
+  *V = p.eps * 4.0 * (iu6 - iu3 + p.C2 * u + p.C0);
+}
+
 // Force kick over the tile lists: lane per real, loop over its full list
 // with partners read from LDS (no reaction, so no write conflicts).
 //   MODE 0  p += F dt in place; block max |p|^2 to partial
 //   MODE 1  out[i] = F (dt = 1); p untouched
 //   MODE 2  MODE 0, plus the trailing half drift out of place:
 //           out[i] = q_i + p_i dt/2 (fcalculator.cc:26-28)
-template <int MODE>
+//   SCALED  the pair term of the differentiable form (pair_df_lj)
+template <int MODE, bool SCALED = false>
 __global__ void __launch_bounds__(kTile) k_kick_tiles(
     const double4 *__restrict__ pos, double4 *__restrict__ mom, const TileDesc *__restrict__ tiles,
     int ntiles, Geom g, Cells c, int n, int kcap, const unsigned short *__restrict__ list,
     const int *__restrict__ nn, double C2_8, double dt, double4 *__restrict__ out,
-    double *__restrict__ partial) {
+    double *__restrict__ partial, LJ lj = LJ{}, const int *__restrict__ halt = nullptr) {
   extern __shared__ double stage_mem[];
   __shared__ TileSmem sm;
+  if (halt && *halt) return;
   const int tile = tile_of_block(blockIdx.x, ntiles);
   if (tile >= ntiles) {
     if (MODE != 1 && threadIdx.x == 0) partial[blockIdx.x] = 0.0;
@@ -616,15 +711,49 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
     const double4 qi = pos[i];
     const int m = nn[i];
     const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-#pragma unroll 2
-    for (int k = 0; k < m; k++) {
+    // kILP independent partner chains per lane, summed in a fixed order;
+    // the offsets of the next group are loaded while this group computes
+    double ax[kILP], ay[kILP], az[kILP];
+#pragma unroll
+    for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
+    const int mg = m - m % kILP;
+    int nxt[kILP];
+#pragma unroll
+    for (int u = 0; u < kILP; u++) nxt[u] = u < mg ? lst[(size_t)u * kTile] : 0;
+    for (int k = 0; k < mg; k += kILP) {
+      int cur[kILP];
+#pragma unroll
+      for (int u = 0; u < kILP; u++) cur[u] = nxt[u];
+      if (k + kILP < mg) {
+#pragma unroll
+        for (int u = 0; u < kILP; u++) nxt[u] = lst[(size_t)(k + kILP + u) * kTile];
+      }
+#pragma unroll
+      for (int u = 0; u < kILP; u++) {
+        const int o = cur[u];
+        const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
+        const double s2 = dx * dx + dy * dy + dz * dz;
+        const double df = SCALED ? pair_df_lj(s2, lj, dt) : pair_df(s2, C2_8, dt, g.CL2);
+        ax[u] += df * dx;
+        ay[u] += df * dy;
+        az[u] += df * dz;
+      }
+    }
+    for (int k = mg; k < m; k++) {
       const int o = lst[(size_t)k * kTile];
       const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
-      const double df = pair_df(dx * dx + dy * dy + dz * dz, C2_8, dt, g.CL2);
-      fx += df * dx;
-      fy += df * dy;
-      fz += df * dz;
+      const double s2 = dx * dx + dy * dy + dz * dz;
+      const double df = SCALED ? pair_df_lj(s2, lj, dt) : pair_df(s2, C2_8, dt, g.CL2);
+      ax[0] += df * dx;
+      ay[0] += df * dy;
+      az[0] += df * dz;
+    }
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+#pragma unroll
+    for (int u = 0; u < kILP; u++) {
+      fx += ax[u];
+      fy += ay[u];
+      fz += az[u];
     }
     if (MODE == 1) {
       out[i] = make_double4(fx, fy, fz, 0.0);
@@ -634,7 +763,7 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
       p.y += fy;
       p.z += fz;
       mom[i] = p;
-      v2 = p.x * p.x + p.y * p.y + p.z * p.z;
+      v2 = speed2(p);
       if (MODE == 2) {
         const double dt2 = dt * 0.5;
         double4 q = qi;
@@ -652,10 +781,224 @@ __global__ void k_vmax(const double4 *__restrict__ mom, int n, double *__restric
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double v2 = 0.0;
   if (i < n) {
-    const double4 p = mom[i];
-    v2 = p.x * p.x + p.y * p.y + p.z * p.z;
+    v2 = speed2(mom[i]);
   }
   block_max_to(v2, partial);
+}
+
+// ------------------------------------------------------------ adjoint
+// Fixed-order block sum of K values per thread into partial[block * K + k].
+template <int K>
+__device__ __forceinline__ void block_sums_to(const double (&v)[K], double *__restrict__ partial) {
+  __shared__ double red[K][kTile];
+  const int t = threadIdx.x;
+  for (int k = 0; k < K; k++) red[k][t] = v[k];
+  __syncthreads();
+  for (int off = kTile / 2; off > 0; off >>= 1) {
+    if (t < off)
+      for (int k = 0; k < K; k++) red[k][t] += red[k][t + off];
+    __syncthreads();
+  }
+  if (t < K) partial[(size_t)blockIdx.x * K + t] = red[t][0];
+}
+
+template <int K>
+__global__ void k_sum_partials_k(const double *__restrict__ partial, int nblocks,
+                                 double *__restrict__ out, double scale) {
+  __shared__ double lds[K][kBlock];
+  const int t = threadIdx.x;
+  double acc[K];
+  for (int k = 0; k < K; k++) acc[k] = 0.0;
+  for (int b = t; b < nblocks; b += kBlock)
+    for (int k = 0; k < K; k++) acc[k] += partial[(size_t)b * K + k];
+  for (int k = 0; k < K; k++) lds[k][t] = acc[k];
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (t < off)
+      for (int k = 0; k < K; k++) lds[k][t] += lds[k][t + off];
+    __syncthreads();
+  }
+  if (t < K) out[t] = lds[t][0] * scale;
+}
+
+// Observables of the differentiable form at one state, per tile:
+//   GRAD = false: block sums (KE, PE, W), W = sum r V'(r) / 3 over pairs;
+//   GRAD = true:  adds the loss sensitivity of this observation,
+//     aq_i += cw gW_i + ce gPE_i,   gW_i = -(2/3) sum_j (phi + s phi') d_ij,
+//                                   gPE_i = -sum_j phi d_ij = -F_i,
+//     ap_i += ck p_i,
+//   and block sums (dPE/deps, dPE/dsigma, dW/deps, dW/dsigma).
+template <bool GRAD>
+__global__ void __launch_bounds__(kTile) k_lj_observe_tiles(
+    const double4 *__restrict__ pos, const double4 *__restrict__ mom,
+    const TileDesc *__restrict__ tiles, int ntiles, Geom g, Cells c, int n, int kcap,
+    const unsigned short *__restrict__ list, const int *__restrict__ nn, LJ lj, double cw,
+    double ce, double ck, double4 *__restrict__ aq, double4 *__restrict__ ap,
+    double *__restrict__ partial) {
+  extern __shared__ double stage_mem[];
+  __shared__ TileSmem sm;
+  constexpr int K = GRAD ? 4 : 3;
+  const int tile = tile_of_block(blockIdx.x, ntiles);
+  if (tile >= ntiles) {
+    if (threadIdx.x < K) partial[(size_t)blockIdx.x * K + threadIdx.x] = 0.0;
+    return;
+  }
+  const TileDesc t = tiles[tile];
+  double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+  tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
+  const int lane = threadIdx.x;
+  double acc[K];
+  for (int k = 0; k < K; k++) acc[k] = 0.0;
+  if (lane < t.ni) {
+    const int i = t.i0 + lane;
+    const double4 qi = pos[i];
+    const double4 p = mom[i];
+    const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
+    double pe = 0.0, w = 0.0, dpe_s = 0.0, dw_s = 0.0;
+    double gwx = 0.0, gwy = 0.0, gwz = 0.0, gex = 0.0, gey = 0.0, gez = 0.0;
+    for (int k = 0; k < nn[i]; k++) {
+      const int o = lst[(size_t)k * kTile];
+      const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
+      const double s2 = dx * dx + dy * dy + dz * dz;
+      double phi, dphi, V;
+      lj_terms(s2, lj, &phi, &dphi, &V);
+      const double gterm = s2 * phi;
+      pe += V;
+      w += gterm;
+      if (GRAD) {
+        const double h = phi + s2 * dphi;
+        dpe_s += -gterm;            // sigma dV/dsigma
+        dw_s += -2.0 * s2 * h;      // sigma dg/dsigma
+        gwx += h * dx;
+        gwy += h * dy;
+        gwz += h * dz;
+        gex += phi * dx;
+        gey += phi * dy;
+        gez += phi * dz;
+      }
+    }
+    if (!GRAD) {
+      acc[0] = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z;
+      acc[1] = 0.5 * pe;
+      acc[2] = 0.5 * w / 3.0;
+    } else {
+      // dPE/deps = PE/eps, dW/deps = W/eps; sigma terms divided by sigma
+      acc[0] = 0.5 * pe / lj.eps;
+      acc[1] = 0.5 * dpe_s / lj.sigma;
+      acc[2] = 0.5 * w / 3.0 / lj.eps;
+      acc[3] = 0.5 * dw_s / 3.0 / lj.sigma;
+      double4 a = aq[i];
+      a.x += cw * (-2.0 / 3.0) * gwx - ce * gex;
+      a.y += cw * (-2.0 / 3.0) * gwy - ce * gey;
+      a.z += cw * (-2.0 / 3.0) * gwz - ce * gez;
+      aq[i] = a;
+      double4 b = ap[i];
+      b.x += ck * p.x;
+      b.y += ck * p.y;
+      b.z += ck * p.z;
+      ap[i] = b;
+    }
+  }
+  block_sums_to<K>(acc, partial);
+}
+
+// Before the HVP of a step: ext[] = ap + aq dt/2 over reals and images (an
+// image carries its source's adjoint, as it carries its source's position).
+// ap itself is left alone here (images read it); k_adj_hvp_tiles writes the
+// reals' new ap from ext[].
+__global__ void k_adj_pre(const double4 *__restrict__ ap, const double4 *__restrict__ aq,
+                          const int *__restrict__ gsrc, int n, int g, double dt2,
+                          double4 *__restrict__ ext) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n + g) return;
+  const int src = i < n ? i : gsrc[i - n];
+  double4 a = ap[src];
+  const double4 b = aq[src];
+  a.x += b.x * dt2;
+  a.y += b.y * dt2;
+  a.z += b.z * dt2;
+  a.w = 0.0;
+  ext[i] = a;
+}
+
+// Adjoint of the kick p' = p + F(q_h) dt, then of the half drift into q_h:
+//   aq_i += dt (J ap)_i,  (J a)_i = sum_j [phi (a_j - a_i) + 2 phi' (d.(a_j - a_i)) d],
+//   ap_i += aq_i dt/2,
+//   block sums dt sum_i ap_i . dF_i/d(eps, sigma).
+// Partner adjoints come from ext[] through the staged slot index.
+__global__ void __launch_bounds__(kTile) k_adj_hvp_tiles(
+    const double4 *__restrict__ pos, const TileDesc *__restrict__ tiles, int ntiles, Geom g,
+    Cells c, int n, int kcap, const unsigned short *__restrict__ list, const int *__restrict__ nn,
+    LJ lj, double dt, const double4 *__restrict__ ext, double4 *__restrict__ aq,
+    double4 *__restrict__ ap, double *__restrict__ partial) {
+  extern __shared__ double stage_mem[];
+  __shared__ TileSmem sm;
+  const int tile = tile_of_block(blockIdx.x, ntiles);
+  if (tile >= ntiles) {
+    if (threadIdx.x < 2) partial[(size_t)blockIdx.x * 2 + threadIdx.x] = 0.0;
+    return;
+  }
+  const TileDesc t = tiles[tile];
+  double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+  int *sidx = reinterpret_cast<int *>(stage_mem + 3 * t.stage);
+  tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
+  for (int k = 0; k < kSeg; k++) {
+    const int b = sm.seg_base[k], len = sm.seg_base[k + 1] - b, st = sm.seg_start[k];
+    for (int u = threadIdx.x; u < len; u += blockDim.x) sidx[b + u] = st + u;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (lane < t.ni) {
+    const int i = t.i0 + lane;
+    const double4 qi = pos[i];
+    const double4 ai = ext[i];
+    const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
+    double jx = 0.0, jy = 0.0, jz = 0.0, te = 0.0, ts = 0.0;
+    for (int k = 0; k < nn[i]; k++) {
+      const int o = lst[(size_t)k * kTile];
+      const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
+      const double s2 = dx * dx + dy * dy + dz * dz;
+      double phi, dphi, V;
+      lj_terms(s2, lj, &phi, &dphi, &V);
+      const double4 aj = ext[sidx[o]];
+      const double wx = aj.x - ai.x, wy = aj.y - ai.y, wz = aj.z - ai.z;
+      const double dw = 2.0 * dphi * (dx * wx + dy * wy + dz * wz);
+      jx += phi * wx + dw * dx;
+      jy += phi * wy + dw * dy;
+      jz += phi * wz + dw * dz;
+      const double ad = ai.x * dx + ai.y * dy + ai.z * dz;
+      te += phi * ad;                         // eps dF/deps . a
+      ts += (phi + s2 * dphi) * ad;           // -(sigma/2) dF/dsigma . a
+    }
+    acc[0] = dt * te / lj.eps;
+    acc[1] = dt * (-2.0 / lj.sigma) * ts;
+    double4 a = aq[i];
+    a.x += dt * jx;
+    a.y += dt * jy;
+    a.z += dt * jz;
+    aq[i] = a;
+    double4 b = ai;
+    b.x += a.x * (dt * 0.5);
+    b.y += a.y * (dt * 0.5);
+    b.z += a.z * (dt * 0.5);
+    ap[i] = b;
+  }
+  block_sums_to<2>(acc, partial);
+}
+
+__global__ void k_scatter4_by_tag(const double4 *__restrict__ src, const int *__restrict__ tag, int n,
+                                  double4 *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[tag[i]] = src[i];
+}
+
+__global__ void k_gather4_by_tag(const double4 *__restrict__ by_id, const int *__restrict__ tag, int n,
+                                 double4 *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = by_id[tag[i]];
 }
 
 // ------------------------------------------------------------ observers
@@ -866,6 +1209,11 @@ __global__ void k_load_state(const double *__restrict__ q3, const double *__rest
   tag[i] = i;
 }
 
+__global__ void k_iota(int *__restrict__ v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = i;
+}
+
 inline int nblk(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 thread_local std::string g_create_error;
@@ -911,6 +1259,9 @@ struct mdx_system {
   bool have_state = false, list_valid = false;
   double buffer_length = 0;
   bool vmax_valid = false;
+  double vmax2_host = 0.0;          // max |p|^2 after the last kick
+  StepCtl *dctl = nullptr, *hctl = nullptr;
+  hipEvent_t evk[2 * kRing] = {};
   // timing
   bool timing = false;
   hipEvent_t ev[8] = {};
@@ -1042,7 +1393,7 @@ int ensure_partials(mdx_system *s, int blocks) {
   if (blocks <= s->partial_blocks) return MDX_OK;
   int rc;
   if ((rc = dev_alloc(s, &s->pmax, (size_t)blocks)) ||
-      (rc = dev_alloc(s, &s->partial, 3 * (size_t)blocks)))
+      (rc = dev_alloc(s, &s->partial, 4 * (size_t)blocks)))
     return rc;
   s->partial_blocks = blocks;
   return MDX_OK;
@@ -1175,32 +1526,63 @@ int kick(mdx_system *s) {
       s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), s->n, s->kcap, s->list16, s->nn,
       8.0 * s->C2, s->prm.dt, nullptr, s->pmax);
   ev_record(s, 3);
-  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, grid, &s->dstat->vmax2);
-  s->vmax_valid = true;
+  s->vmax_valid = false;   // recomputed from p when the expiry check needs it
   s->n_kick++;
   return launch_check(s, "kick");
 }
 
-// One NVE step after the expiry check, in two passes: drift + image
-// refresh into pos_alt (images also written back to pos), then kick +
-// trailing drift reading pos_alt and writing the reals of pos.  pos ends
-// as the reference's state after Calculate(): reals at the end of the step,
-// images at their mid-step refresh.  Same arithmetic as
-// drift(); refresh(); kick(); drift().
-int fused_step(mdx_system *s) {
+// Up to `want` NVE steps after the expiry check of the first, queued
+// without a host round trip: per step, drift + image refresh into pos_alt
+// (images also written back to pos), kick + trailing drift from pos_alt into
+// the reals of pos, and the budget guard.  pos ends as the reference's state
+// after Calculate(): reals at the end of the step, images at their mid-step
+// refresh.  Same arithmetic as drift(); refresh(); kick(); drift().
+// *ran receives the number of steps the device completed (>= 1).
+int run_batch(mdx_system *s, int want, int *ran) {
   const int n = s->n, g = s->g, grid = tile_grid(s->ntiles);
-  k_drift_refresh<<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->gsrc, s->gcode, n, g,
-                                                          s->prm.dt * 0.5, s->geom.L[0],
-                                                          s->geom.L[1], s->geom.L[2], s->pos_alt);
-  ev_record(s, 2);
-  k_kick_tiles<2><<<grid, kTile, stage_bytes(s), s->stream>>>(
-      s->pos_alt, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn,
-      8.0 * s->C2, s->prm.dt, s->pos, s->pmax);
-  ev_record(s, 3);
-  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, grid, &s->dstat->vmax2);
-  s->vmax_valid = true;
-  s->n_kick++;
-  return launch_check(s, "step");
+  const int B = std::max(1, std::min(want, kRing));
+  StepCtl *h = s->hctl;
+  h->budget = s->buffer_length;
+  h->dt = s->prm.dt;
+  h->buffer = s->prm.buffer_length;
+  h->halt = 0;
+  h->steps_done = 0;
+  h->too_fast = 0;
+  HIPCHK(s, hipMemcpyAsync(s->dctl, h, offsetof(StepCtl, ring), hipMemcpyHostToDevice, s->stream));
+  ev_record(s, 4);
+  for (int b = 0; b < B; b++) {
+    k_drift_refresh<<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->gsrc, s->gcode, n, g,
+                                                            s->prm.dt * 0.5, s->geom.L[0],
+                                                            s->geom.L[1], s->geom.L[2], s->pos_alt,
+                                                            &s->dctl->halt);
+    if (s->timing) (void)hipEventRecord(s->evk[2 * b], s->stream);
+    k_kick_tiles<2><<<grid, kTile, stage_bytes(s), s->stream>>>(
+        s->pos_alt, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16,
+        s->nn, 8.0 * s->C2, s->prm.dt, s->pos, s->pmax, LJ{}, &s->dctl->halt);
+    if (s->timing) (void)hipEventRecord(s->evk[2 * b + 1], s->stream);
+    k_step_guard<<<1, kBlock, 0, s->stream>>>(s->pmax, grid, s->dctl);
+  }
+  ev_record(s, 5);
+  int rc = launch_check(s, "step");
+  if (rc) return rc;
+  HIPCHK(s, hipMemcpyAsync(h, s->dctl, sizeof(StepCtl), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  *ran = h->steps_done;
+  if (*ran < 1 || *ran > B) return fail(s, MDX_ERR_STATE, "step guard ran %d of %d steps", *ran, B);
+  if (s->timing) {
+    double k = 0.0;
+    for (int b = 0; b < *ran; b++) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, s->evk[2 * b], s->evk[2 * b + 1]);
+      k += ms;
+    }
+    const float all = ev_ms(s, 4, 5);
+    s->t_force += k;
+    s->t_kick_only += k;
+    s->t_other += all - k;
+  }
+  s->n_kick += *ran;
+  return MDX_OK;
 }
 
 int drift(mdx_system *s) {
@@ -1228,10 +1610,11 @@ int expired(mdx_system *s, int *out) {
     if (!s->vmax_valid) {
       k_vmax<<<nblk(s->n), kBlock, 0, s->stream>>>(s->mom, s->n, s->pmax);
       k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(s->n), &s->dstat->vmax2);
+      if ((rc = sync_status(s))) return rc;
+      s->vmax2_host = s->hstat->vmax2;
       s->vmax_valid = true;
     }
-    if ((rc = sync_status(s))) return rc;
-    const double vmax = std::sqrt(s->hstat->vmax2);
+    const double vmax = std::sqrt(s->vmax2_host);
     if (vmax * 2.0 * s->prm.dt > s->prm.buffer_length)
       return fail(s, MDX_ERR_TOO_FAST, "Too fast particles exists. max_velocity = %f", vmax);
     s->buffer_length = s->buffer_length - vmax * 2.0 * s->prm.dt;
@@ -1278,6 +1661,241 @@ int reduce_observe(mdx_system *s, double *ke, double *pe, double *vir) {
   if (pe) *pe = s->hstat->sums[1];
   if (vir) *vir = s->hstat->sums[2] / 3.0;
   return MDX_OK;
+}
+
+
+// ---------------------------------------------------------------- adjoint host
+// Device scratch of one differentiable-trajectory call, freed on return.
+struct AdjScratch {
+  mdx_system *s = nullptr;
+  std::vector<void *> bufs;
+  template <class T>
+  int get(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) return fail(s, MDX_ERR_HIP, "hipMalloc(%zu B): %s", count * sizeof(T), hipGetErrorString(e));
+    bufs.push_back(*p);
+    return MDX_OK;
+  }
+  ~AdjScratch() {
+    for (void *b : bufs) (void)hipFree(b);
+  }
+};
+
+LJ make_lj(const mdx_system *s, double eps, double sigma) {
+  LJ p;
+  p.eps = eps;
+  p.sigma = sigma;
+  p.inv_sig2 = 1.0 / (sigma * sigma);
+  p.es2 = eps / (sigma * sigma);
+  p.es4 = eps / (sigma * sigma * sigma * sigma);
+  const double rc = s->prm.cutoff * sigma;
+  p.cl2 = rc * rc;
+  p.C0 = s->C0;
+  p.C2 = s->C2;
+  return p;
+}
+
+size_t adj_stage_bytes(const mdx_system *s) {
+  return (size_t)s->stage_max * (3 * sizeof(double) + sizeof(int)) + 8;
+}
+
+// one step of the differentiable form: drift + image refresh into mid,
+// kick + drift back into pos
+int lj_step(mdx_system *s, const LJ &lj, double4 *mid) {
+  const int n = s->n, g = s->g, grid = tile_grid(s->ntiles);
+  k_drift_refresh<<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->gsrc, s->gcode, n, g,
+                                                          s->prm.dt * 0.5, s->geom.L[0],
+                                                          s->geom.L[1], s->geom.L[2], mid, nullptr);
+  k_kick_tiles<2, true><<<grid, kTile, stage_bytes(s), s->stream>>>(
+      mid, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn,
+      8.0 * s->C2, s->prm.dt, s->pos, s->pmax, lj);
+  return launch_check(s, "lj step");
+}
+
+// the largest displacement of a real since the last build must stay below
+// half the skin, so the list of the checkpoint covers the whole segment
+int check_segment_list(mdx_system *s, const LJ &lj) {
+  k_disp<<<nblk(s->n), kBlock, 0, s->stream>>>(s->pos, s->ref_pos, s->n, s->disp, s->pmax);
+  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(s->n), &s->dstat->dmax2);
+  int rc = sync_status(s);
+  if (rc) return rc;
+  const double skin = s->SL - s->prm.cutoff * lj.sigma;
+  if (!(2.0 * std::sqrt(s->hstat->dmax2) < skin))
+    return fail(s, MDX_ERR_STATE,
+                "checkpoint interval too long: a particle moved %g, more than half the skin %g",
+                std::sqrt(s->hstat->dmax2), skin);
+  return MDX_OK;
+}
+
+int lj_trajectory(mdx_system *s, double eps, double sigma, int nsteps, int nobs, int seg,
+                  double p_target, double pe_target, bool grad, mdx_lj_result *res,
+                  double *dq0) {
+  if (!s->have_state) return fail(s, MDX_ERR_STATE, "no state");
+  if (nsteps < 1 || nobs < 1 || nobs > nsteps || seg < 1 || !(eps > 0) || !(sigma > 0))
+    return fail(s, MDX_ERR_ARG, "bad trajectory arguments");
+  if (!(s->prm.cutoff * sigma < s->SL))
+    return fail(s, MDX_ERR_ARG, "cutoff 3 sigma must stay inside the search length");
+  const LJ lj = make_lj(s, eps, sigma);
+  const int n = s->n;
+  const int nseg = (nsteps + seg - 1) / seg;
+  const double V = s->geom.L[0] * s->geom.L[1] * s->geom.L[2];
+  AdjScratch sc;
+  sc.s = s;
+  double4 *ck_pos, *ck_mom, *fin_q, *fin_p;
+  int *ck_tag;
+  double *obs;
+  int rc;
+  if ((rc = sc.get(&ck_pos, (size_t)nseg * n)) || (rc = sc.get(&ck_mom, (size_t)nseg * n)) ||
+      (rc = sc.get(&ck_tag, (size_t)nseg * n)) || (rc = sc.get(&obs, 3 * (size_t)nsteps)) ||
+      (rc = sc.get(&fin_q, (size_t)n)) || (rc = sc.get(&fin_p, (size_t)n)))
+    return rc;
+  HIPCHK(s, hipMemsetAsync(obs, 0, sizeof(double) * 3 * nsteps, s->stream));
+  const auto t0 = std::chrono::steady_clock::now();
+  // ---- forward, a list and a checkpoint every seg steps
+  for (int sgi = 0; sgi < nseg; sgi++) {
+    if ((rc = build_pairlist(s))) return rc;
+    HIPCHK(s, hipMemcpyAsync(ck_pos + (size_t)sgi * n, s->pos, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync(ck_mom + (size_t)sgi * n, s->mom, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync(ck_tag + (size_t)sgi * n, s->tag, sizeof(int) * n, hipMemcpyDeviceToDevice, s->stream));
+    const int k1 = std::min(nsteps, (sgi + 1) * seg);
+    for (int k = sgi * seg; k < k1; k++) {
+      if ((rc = lj_step(s, lj, s->pos_alt))) return rc;
+      if (k >= nsteps - nobs) {
+        if ((rc = refresh(s))) return rc;
+        const int nb = tile_grid(s->ntiles);
+        k_lj_observe_tiles<false><<<nb, kTile, stage_bytes(s), s->stream>>>(
+            s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16,
+            s->nn, lj, 0.0, 0.0, 0.0, nullptr, nullptr, s->partial);
+        k_sum_partials_k<3><<<1, kBlock, 0, s->stream>>>(s->partial, nb, obs + 3 * (size_t)k, 1.0);
+      }
+    }
+    if ((rc = check_segment_list(s, lj))) return rc;
+  }
+  k_scatter4_by_tag<<<nblk(n), kBlock, 0, s->stream>>>(s->pos, s->tag, n, fin_q);
+  k_scatter4_by_tag<<<nblk(n), kBlock, 0, s->stream>>>(s->mom, s->tag, n, fin_p);
+  std::vector<double> ho(3 * (size_t)nsteps);
+  HIPCHK(s, hipMemcpyAsync(ho.data(), obs, sizeof(double) * 3 * nsteps, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  double Psum = 0.0, Esum = 0.0;
+  for (int k = nsteps - nobs; k < nsteps; k++) {
+    const double ke = ho[3 * k], pe = ho[3 * k + 1], w = ho[3 * k + 2];
+    Psum += (2.0 * ke / 3.0 - w) / V;   // MDManager::Pressure (mdmanager.cc:563-570)
+    Esum += pe / n;
+  }
+  const double Pm = Psum / nobs, Em = Esum / nobs;
+  res->pressure_mean = Pm;
+  res->pe_mean = Em;
+  res->loss = (Pm - p_target) * (Pm - p_target) + (Em - pe_target) * (Em - pe_target);
+  res->dloss_deps = res->dloss_dsigma = 0.0;
+  const auto t1 = std::chrono::steady_clock::now();
+  res->forward_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  res->backward_ms = 0.0;
+  if (grad) {
+    // ---- reverse sweep: recompute each segment from its checkpoint, then
+    // walk its steps backwards
+    const int tot = n + std::max(s->g, 0);
+    const size_t totcap = (size_t)s->cap + s->gcap;
+    double4 *st_mid, *st_end, *st_mom, *aq, *ap, *aext, *aid_q, *aid_p;
+    double *theta;
+    const int nslots = nsteps + nobs;
+    if ((rc = sc.get(&st_mid, (size_t)seg * totcap)) || (rc = sc.get(&st_end, (size_t)seg * totcap)) ||
+        (rc = sc.get(&st_mom, (size_t)seg * n)) || (rc = sc.get(&aq, (size_t)n)) ||
+        (rc = sc.get(&ap, (size_t)n)) || (rc = sc.get(&aext, totcap)) ||
+        (rc = sc.get(&aid_q, (size_t)n)) || (rc = sc.get(&aid_p, (size_t)n)) ||
+        (rc = sc.get(&theta, 4 * (size_t)nslots)))
+      return rc;
+    (void)tot;
+    HIPCHK(s, hipMemsetAsync(aid_q, 0, sizeof(double4) * n, s->stream));
+    HIPCHK(s, hipMemsetAsync(aid_p, 0, sizeof(double4) * n, s->stream));
+    HIPCHK(s, hipMemsetAsync(theta, 0, sizeof(double) * 4 * nslots, s->stream));
+    const double cP = 2.0 * (Pm - p_target) / nobs, cE = 2.0 * (Em - pe_target) / nobs;
+    const double cw = -cP / V, ce = cE / n, ck = cP * 2.0 / (3.0 * V);
+    int slot = 0;
+    for (int sgi = nseg - 1; sgi >= 0; sgi--) {
+      HIPCHK(s, hipMemcpyAsync(s->pos, ck_pos + (size_t)sgi * n, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream));
+      HIPCHK(s, hipMemcpyAsync(s->mom, ck_mom + (size_t)sgi * n, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream));
+      HIPCHK(s, hipMemcpyAsync(s->tag, ck_tag + (size_t)sgi * n, sizeof(int) * n, hipMemcpyDeviceToDevice, s->stream));
+      // the checkpoint is already in cell order: the stable sort keeps it and
+      // the list, tiles and images come out as in the forward pass
+      if ((rc = build_pairlist(s))) return rc;
+      if (adj_stage_bytes(s) + 4096 > s->lds_limit)
+        return fail(s, MDX_ERR_CAPACITY, "adjoint tile staging exceeds LDS");
+      const int T = s->n + s->g;
+      k_gather4_by_tag<<<nblk(n), kBlock, 0, s->stream>>>(aid_q, s->tag, n, aq);
+      k_gather4_by_tag<<<nblk(n), kBlock, 0, s->stream>>>(aid_p, s->tag, n, ap);
+      const int k0 = sgi * seg, k1 = std::min(nsteps, (sgi + 1) * seg);
+      for (int k = k0; k < k1; k++) {
+        double4 *mid = st_mid + (size_t)(k - k0) * totcap;
+        if ((rc = lj_step(s, lj, mid))) return rc;
+        if (k >= nsteps - nobs) {
+          if ((rc = refresh(s))) return rc;
+          HIPCHK(s, hipMemcpyAsync(st_end + (size_t)(k - k0) * totcap, s->pos, sizeof(double4) * T, hipMemcpyDeviceToDevice, s->stream));
+          HIPCHK(s, hipMemcpyAsync(st_mom + (size_t)(k - k0) * n, s->mom, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream));
+        }
+      }
+      const int nb = tile_grid(s->ntiles);
+      for (int k = k1 - 1; k >= k0; k--) {
+        if (k >= nsteps - nobs) {
+          // sensitivity of the observation after step k
+          k_lj_observe_tiles<true><<<nb, kTile, stage_bytes(s), s->stream>>>(
+              st_end + (size_t)(k - k0) * totcap, st_mom + (size_t)(k - k0) * n, s->tiles,
+              s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, lj, cw, ce, ck, aq,
+              ap, s->partial);
+          k_sum_partials_k<4><<<1, kBlock, 0, s->stream>>>(s->partial, nb, theta + 4 * (size_t)slot++, 1.0);
+        }
+        k_adj_pre<<<nblk(T), kBlock, 0, s->stream>>>(ap, aq, s->gsrc, n, s->g, s->prm.dt * 0.5, aext);
+        k_adj_hvp_tiles<<<nb, kTile, adj_stage_bytes(s), s->stream>>>(
+            st_mid + (size_t)(k - k0) * totcap, s->tiles, s->ntiles, s->geom, cells_of(s), n,
+            s->kcap, s->list16, s->nn, lj, s->prm.dt, aext, aq, ap, s->partial);
+        k_sum_partials_k<2><<<1, kBlock, 0, s->stream>>>(s->partial, nb, theta + 4 * (size_t)slot++, 1.0);
+        if ((rc = launch_check(s, "adjoint step"))) return rc;
+      }
+      k_scatter4_by_tag<<<nblk(n), kBlock, 0, s->stream>>>(aq, s->tag, n, aid_q);
+      k_scatter4_by_tag<<<nblk(n), kBlock, 0, s->stream>>>(ap, s->tag, n, aid_p);
+    }
+    std::vector<double> ht(4 * (size_t)nslots);
+    HIPCHK(s, hipMemcpyAsync(ht.data(), theta, sizeof(double) * 4 * nslots, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    // slots in the order written: per step backwards, [observation], hvp
+    double ge = 0.0, gs = 0.0;
+    slot = 0;
+    for (int sgi = nseg - 1; sgi >= 0; sgi--) {
+      const int k0 = sgi * seg, k1 = std::min(nsteps, (sgi + 1) * seg);
+      for (int k = k1 - 1; k >= k0; k--) {
+        if (k >= nsteps - nobs) {
+          const double *v = &ht[4 * (size_t)slot++];
+          ge += ce * v[0] + cw * v[2];
+          gs += ce * v[1] + cw * v[3];
+        }
+        const double *v = &ht[4 * (size_t)slot++];
+        ge += v[0];
+        gs += v[1];
+      }
+    }
+    res->dloss_deps = ge;
+    res->dloss_dsigma = gs;
+    if (dq0) {
+      std::vector<double4> hq(n);
+      HIPCHK(s, hipMemcpyAsync(hq.data(), aid_q, sizeof(double4) * n, hipMemcpyDeviceToHost, s->stream));
+      HIPCHK(s, hipStreamSynchronize(s->stream));
+      for (int i = 0; i < n; i++) {
+        dq0[3 * (size_t)i] = hq[i].x;
+        dq0[3 * (size_t)i + 1] = hq[i].y;
+        dq0[3 * (size_t)i + 2] = hq[i].z;
+      }
+    }
+    res->backward_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+  }
+  // leave the system at the end of the trajectory (ids in order, no list)
+  HIPCHK(s, hipMemcpyAsync(s->pos, fin_q, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(s, hipMemcpyAsync(s->mom, fin_p, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream));
+  k_iota<<<nblk(n), kBlock, 0, s->stream>>>(s->tag, n);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  s->list_valid = false;
+  s->vmax_valid = false;
+  return launch_check(s, "trajectory");
 }
 
 }  // namespace
@@ -1353,6 +1971,12 @@ int mdx_create(const mdx_params *prm, mdx_system **out) {
     if (hipGetDeviceProperties(&prop, prm->device) == hipSuccess) s->lds_limit = prop.sharedMemPerBlock;
   }
   if (!rc && hipHostMalloc((void **)&s->hstat, sizeof(DevStatus)) != hipSuccess) rc = MDX_ERR_HIP;
+  if (!rc && hipHostMalloc((void **)&s->hctl, sizeof(StepCtl)) != hipSuccess) rc = MDX_ERR_HIP;
+  if (!rc && hipMalloc((void **)&s->dctl, sizeof(StepCtl)) != hipSuccess) rc = MDX_ERR_HIP;
+  if (!rc) {
+    for (auto &e : s->evk)
+      if (hipEventCreate(&e) != hipSuccess) rc = MDX_ERR_HIP;
+  }
   if (!rc && prm->capacity > 0) rc = ensure_capacity(s, prm->capacity);
   if (rc) {
     g_create_error = s->err.empty() ? "allocation failed" : s->err;
@@ -1375,6 +1999,10 @@ int mdx_destroy(mdx_system *s) {
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (s->hstat) (void)hipHostFree(s->hstat);
+  if (s->hctl) (void)hipHostFree(s->hctl);
+  if (s->dctl) (void)hipFree(s->dctl);
+  for (auto &e : s->evk)
+    if (e) (void)hipEventDestroy(e);
   for (auto &e : s->ev)
     if (e) (void)hipEventDestroy(e);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
@@ -1466,27 +2094,35 @@ int mdx_calculate_force(mdx_system *s) {
 int mdx_calculate(mdx_system *s, int nsteps, int *rebuilds) {
   if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no state");
   int nreb = 0, rc;
-  for (int step = 0; step < nsteps; step++) {
+  int done = 0;
+  while (done < nsteps) {
+    // start of step `done`: PairList::IsPairListExpired, then MakePairList
     int exp = 0;
     if ((rc = expired(s, &exp))) return rc;
     if (exp) {
       if ((rc = build_pairlist(s))) return rc;
       nreb++;
     }
-    ev_record(s, 4);
-    if ((rc = fused_step(s))) return rc;
-    ev_record(s, 5);
-    s->n_steps++;
-    if (s->timing) {
-      HIPCHK(s, hipEventSynchronize(s->ev[5]));
-      const float all = ev_ms(s, 4, 5), k = ev_ms(s, 2, 3);
-      s->t_force += k;
-      s->t_kick_only += k;
-      s->t_other += all - k;
+    int ran = 0;
+    if ((rc = run_batch(s, nsteps - done, &ran))) return rc;
+    // replay the budget at the starts of the batch's later steps exactly;
+    // the device only went on while the budget stayed positive
+    for (int b = 0; b + 1 < ran; b++) {
+      const double vmax = std::sqrt(s->hctl->ring[b]);
+      if (s->buffer_length > 0.0) {
+        if (vmax * 2.0 * s->prm.dt > s->prm.buffer_length)
+          return fail(s, MDX_ERR_TOO_FAST, "Too fast particles exists. max_velocity = %f", vmax);
+        s->buffer_length = s->buffer_length - vmax * 2.0 * s->prm.dt;
+      }
+      if (s->buffer_length < 0.0)
+        return fail(s, MDX_ERR_STATE, "step guard ran past a spent pairlist budget");
     }
+    s->vmax2_host = s->hctl->ring[ran - 1];
+    s->vmax_valid = true;
+    done += ran;
+    s->n_steps += ran;
   }
   if (rebuilds) *rebuilds = nreb;
-  HIPCHK(s, hipStreamSynchronize(s->stream));
   return MDX_OK;
 }
 
@@ -1542,6 +2178,14 @@ int mdx_pairs_canonical(mdx_system *s, int32_t *out, int64_t cap, int64_t *count
   for (long long k = 0; k < w; k++)
     for (int d = 0; d < 3; d++) out[3 * k + d] = h[3 * idx[k] + d];
   return MDX_OK;
+}
+
+int mdx_lj_trajectory(mdx_system *s, double eps, double sigma, int nsteps, int nobs,
+                      int checkpoint_every, double p_target, double pe_target, int grad,
+                      mdx_lj_result *out, double *dloss_dq0) {
+  if (!s || !out) return fail(s, MDX_ERR_ARG, "null argument");
+  return lj_trajectory(s, eps, sigma, nsteps, nobs, checkpoint_every, p_target, pe_target,
+                       grad != 0, out, dloss_dq0);
 }
 
 int mdx_pair_number(mdx_system *s, int64_t *count) {

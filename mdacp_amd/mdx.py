@@ -18,7 +18,7 @@ EXPORTS = (
     "mdx_get_state", "mdx_particle_number", "mdx_make_pairlist", "mdx_is_pairlist_expired",
     "mdx_update_position_half", "mdx_refresh_images", "mdx_calculate_force", "mdx_calculate",
     "mdx_observe", "mdx_forces", "mdx_pair_number", "mdx_pairs_canonical", "mdx_timing",
-    "mdx_timing_reset", "mdx_kick_profile",
+    "mdx_timing_reset", "mdx_kick_profile", "mdx_lj_trajectory",
 )
 
 STATUS = {0: "MDX_OK", -1: "MDX_ERR_ARG", -2: "MDX_ERR_CAPACITY", -3: "MDX_ERR_INVALID_POS",
@@ -29,6 +29,13 @@ class MdxParams(ctypes.Structure):
     _fields_ = [("L", ctypes.c_double * 3), ("dt", ctypes.c_double), ("cutoff", ctypes.c_double),
                 ("buffer_length", ctypes.c_double), ("check_list_length", ctypes.c_int),
                 ("capacity", ctypes.c_int), ("device", ctypes.c_int), ("stream", ctypes.c_void_p)]
+
+
+class MdxLjResult(ctypes.Structure):
+    _fields_ = [("loss", ctypes.c_double), ("pressure_mean", ctypes.c_double),
+                ("pe_mean", ctypes.c_double), ("dloss_deps", ctypes.c_double),
+                ("dloss_dsigma", ctypes.c_double), ("forward_ms", ctypes.c_double),
+                ("backward_ms", ctypes.c_double)]
 
 
 class MdxError(RuntimeError):
@@ -71,6 +78,9 @@ def load() -> ctypes.CDLL:
         "mdx_timing": (ctypes.c_int, [vp, dp, dp, dp, i64p, i64p]),
         "mdx_timing_reset": (ctypes.c_int, [vp, ctypes.c_int]),
         "mdx_kick_profile": (ctypes.c_int, [vp, dp, dp]),
+        "mdx_lj_trajectory": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_int, P(MdxLjResult), dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -209,3 +219,17 @@ class System:
         a, b = ctypes.c_double(), ctypes.c_double()
         self._chk(self.lib.mdx_kick_profile(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def lj_trajectory(self, eps=1.0, sigma=1.0, nsteps=200, nobs=100, checkpoint_every=20,
+                      p_target=1.5, pe_target=-4.0, grad=True, want_dq0=True):
+        """Loss of the differentiable trajectory and, with grad, its gradient
+        (dict with loss, pressure_mean, pe_mean, dloss_deps, dloss_dsigma,
+        dloss_dq0 [n][3] by id, forward_ms, backward_ms)."""
+        r = MdxLjResult()
+        dq0 = np.zeros((self.n, 3)) if (grad and want_dq0) else None
+        self._chk(self.lib.mdx_lj_trajectory(self.h, eps, sigma, nsteps, nobs, checkpoint_every,
+                                             p_target, pe_target, 1 if grad else 0,
+                                             ctypes.byref(r), _dp(dq0) if dq0 is not None else None))
+        out = {k: getattr(r, k) for k, _ in MdxLjResult._fields_}
+        out["dloss_dq0"] = dq0
+        return out
