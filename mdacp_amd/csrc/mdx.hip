@@ -2103,8 +2103,17 @@ int mdx_calculate(mdx_system *s, int nsteps, int *rebuilds) {
       if ((rc = build_pairlist(s))) return rc;
       nreb++;
     }
+    // queue as many steps as the budget is expected to last at the last
+    // |p|max, so the guard halts near the end of the batch
+    int want = nsteps - done;
+    if (!(s->buffer_length > 0.0)) {
+      want = 1;
+    } else if (s->vmax_valid && s->vmax2_host > 0.0) {
+      const double per = 2.0 * std::sqrt(s->vmax2_host) * s->prm.dt;
+      want = std::min(want, (int)std::min(s->buffer_length / per + 1.0, (double)kRing));
+    }
     int ran = 0;
-    if ((rc = run_batch(s, nsteps - done, &ran))) return rc;
+    if ((rc = run_batch(s, want, &ran))) return rc;
     // replay the budget at the starts of the batch's later steps exactly;
     // the device only went on while the budget stayed positive
     for (int b = 0; b + 1 < ran; b++) {
