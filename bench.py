@@ -15,6 +15,17 @@ kernel): algorithmic bytes per launch over its mean launch time from HIP
 events recorded on the library's stream during the timed region.
 `cpu_baseline` is the reference's own CPU path (oracle/_ref, compiled from
 the reference sources) on a bounded sample of the same workload.
+
+Two more objects ride on the same line:
+  `batched`  256 independent replicas (FCC n ~ U{5..20} cells, rho ~ U[0.3,
+             0.9], T ~ U[0.7, 1.5], seeds 4+r) advanced together, one graph
+             launch per step, tile shape autotuned; aggregate atom-steps/s
+             over 500 steps, against looping the single-system path over
+             the same replicas (each timed alone, summed).
+  `adjoint`  the differentiable trajectory at N=32,000 (FCC 20^3, rho 0.7,
+             T 1, seed 5): 200 steps, loss over the last 100, checkpoints
+             every 20; forward and reverse-sweep device time, fwd+adjoint
+             steps per second and the adjoint/forward cost ratio.
 """
 from __future__ import annotations
 
@@ -113,6 +124,61 @@ def copy_probe(nbytes: int):
     print(json.dumps({"copy_probe_bytes": a.numel() * 8, "launches": 3}), flush=True)
 
 
+def batched(steps: int, plain_steps: int):
+    import torch
+    from mdacp_amd import workloads as W
+    from mdacp_amd.mdx import Ensemble, System
+
+    states = W.replica_set()
+    ntot = sum(s.n for s in states)
+    e = Ensemble(states, device=0)
+    cfg = e.autotune(10)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.calculate(steps)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    st, done = e.status()
+    e.close()
+    # the same replicas through the single-system path, one after another
+    looped = 0.0
+    for s in states:
+        with System.from_state(s, device=0) as p:
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            p.calculate(plain_steps)
+            torch.cuda.synchronize()
+            looped += time.perf_counter() - t1
+    value = ntot * steps / sec
+    plain_value = ntot * plain_steps / looped
+    return {"replicas": len(states), "particles": ntot, "steps": steps,
+            "value": value, "unit": "atom-steps/s", "ms_per_step": sec * 1e3 / steps,
+            "all_replicas_completed": bool((st == 0).all() and (done == steps).all()),
+            "autotuned": {"tile": cfg["tile"], "grid_mult": cfg["grid_mult"],
+                          "step_grid": cfg["step_grid"], "stage_cap": cfg["stage_cap"]},
+            "looped_single_system": {"value": plain_value, "unit": "atom-steps/s",
+                                     "steps": plain_steps, "seconds": looped},
+            "speedup_vs_looped": value / plain_value}
+
+
+def adjoint():
+    from mdacp_amd import workloads as W
+    from mdacp_amd.mdx import System
+
+    s = W.fcc_state(20, 0.7, 1.0, 5)
+    opts = dict(nsteps=200, nobs=100, checkpoint_every=20, p_target=1.5, pe_target=-4.0)
+    with System.from_state(s, device=0) as sy:
+        sy.lj_trajectory(grad=True, **opts)            # warm-up
+    with System.from_state(s, device=0) as sy:
+        g = sy.lj_trajectory(grad=True, **opts)
+    fwd, bwd = g["forward_ms"], g["backward_ms"]
+    return {"particles": s.n, "steps": 200, "checkpoint_every": 20,
+            "forward_ms": fwd, "backward_ms": bwd,
+            "fwd_adjoint_steps_per_s": 200 / ((fwd + bwd) * 1e-3),
+            "adjoint_over_forward": bwd / fwd, "loss": g["loss"],
+            "dloss_deps": g["dloss_deps"], "dloss_dsigma": g["dloss_dsigma"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +186,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=500)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES between two device buffers (counter calibration)")
     args = ap.parse_args()
@@ -171,10 +240,14 @@ def main():
                      "other_step_kernels_total": tm["other_ms"], "builds": tm["builds"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_kick<0>", "avg_launch_ms": kick_ms,
+                     "kernel": "k_kick_tiles<2>", "avg_launch_ms": kick_ms,
                      "algorithmic_bytes_per_launch": kick_bytes},
     }
     sys_.close()
+    if not args.no_batched:
+        out["batched"] = batched(args.batched_steps, args.batched_steps)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(state, args.cpu_steps, 2)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -135,6 +135,57 @@ int mdx_lj_trajectory(mdx_system *sys, double eps, double sigma, int nsteps, int
                       int checkpoint_every, double p_target, double pe_target, int grad,
                       mdx_lj_result *out, double *dloss_dq0);
 
+/* ------------------------------------------------------------------
+ * Replica ensemble (no counterpart in the reference, which runs one
+ * system per MPI job): nrep independent periodic systems of uneven sizes,
+ * densities and temperatures advanced together on one GPU.  Each replica
+ * runs exactly the step of mdx_calculate on its own (binning, images,
+ * pairlist expiry, kick arithmetic), so its trajectory is the one a
+ * separate mdx_system would produce; one captured HIP graph per step
+ * advances all of them, and a replica whose pairlist expired is rebuilt
+ * inside that graph while the others keep stepping.
+ *
+ * Particles of all replicas are concatenated in replica order (replica r
+ * owns ids [sum_{s<r} n_s, sum_{s<=r} n_s)); L3 is [nrep][3].  prm gives
+ * dt, cutoff, buffer_length, check_list_length (<= 1024), device and
+ * stream; prm->L and prm->capacity are not used. */
+typedef struct mdx_ensemble mdx_ensemble;
+
+typedef struct mdx_ensemble_config {
+  int tile;            /* reals per tile: 64 or 128 */
+  int grid_mult;       /* persistent step grid = grid_mult x resident workgroups */
+  int step_grid;       /* workgroups of the step kernel */
+  int stage_cap;       /* particles one tile may stage in LDS */
+  double ms_per_step;  /* device time per step of the last mdx_ensemble_calculate */
+} mdx_ensemble_config;
+
+int mdx_ensemble_create(const mdx_params *prm, int nrep, const int *n, const double *L3,
+                        mdx_ensemble **out);
+int mdx_ensemble_destroy(mdx_ensemble *e);
+const char *mdx_ensemble_last_error(const mdx_ensemble *e);
+int mdx_ensemble_size(const mdx_ensemble *e, int *nrep, int *ntotal);
+/* loads every replica (q3, p3 are [ntotal][3]) and builds the first
+ * pairlists (the construction the first step of mdx_calculate makes) */
+int mdx_ensemble_set_state(mdx_ensemble *e, const double *q3, const double *p3);
+int mdx_ensemble_get_state(mdx_ensemble *e, double *q3, double *p3);
+/* nsteps NVE steps of every running replica; rebuilds (may be NULL,
+ * [nrep]) receives each replica's pairlist constructions since
+ * mdx_ensemble_set_state.  A replica that fails (too fast, invalid
+ * position, capacity) stops; the others go on; the first failure is
+ * returned and mdx_ensemble_status lists all. */
+int mdx_ensemble_calculate(mdx_ensemble *e, int nsteps, int *rebuilds);
+int mdx_ensemble_status(mdx_ensemble *e, int *status, int *steps);
+/* per replica KE, PE, virial ([nrep] each, any may be NULL), as
+ * mdx_observe of each replica alone */
+int mdx_ensemble_observe(mdx_ensemble *e, double *ke, double *pe, double *vir);
+int mdx_ensemble_get_config(const mdx_ensemble *e, mdx_ensemble_config *cfg);
+/* tile size changes only before the first step after mdx_ensemble_set_state */
+int mdx_ensemble_set_config(mdx_ensemble *e, int tile, int grid_mult);
+/* times trial_steps steps of every tile size and grid multiple from the
+ * loaded state, keeps the fastest; only before the first step; the loaded
+ * state is left in place, and results do not depend on the choice */
+int mdx_ensemble_autotune(mdx_ensemble *e, int trial_steps, mdx_ensemble_config *best);
+
 /* cumulative device time (ms) of the pairlist build, force kick and
  * remaining step kernels, and counts, measured with HIP events since the
  * last reset; any pointer may be NULL. */

@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmdx.so")
-SRC = os.path.join(HERE, "csrc", "mdx.hip")
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("mdx.hip", "ensemble.hip")]
 ARCH = os.environ.get("MDX_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
@@ -61,13 +61,14 @@ def build_host(verbose: bool = False) -> None:
 
 def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIP_FLAGS, SRC, "-o", tmp]
+    srcs = [f for f in SRCS if os.path.exists(f)]
+    cmd = [hipcc(), *HIP_FLAGS, *srcs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed for mdx.hip")
+        raise RuntimeError("hipcc failed for " + " ".join(os.path.basename(f) for f in srcs))
     os.replace(tmp, LIB)
     build_host(verbose)
     return LIB

@@ -1,0 +1,1633 @@
+// ensemble.hip — many independent periodic LJ systems ("replicas") of
+// uneven sizes, densities and temperatures, advanced together on one GPU.
+//
+// Every replica follows exactly the step of a single system (the same
+// binning, image rule, neighbour-list order, kick arithmetic and pairlist
+// expiry), so each replica's trajectory is the one it would have run alone.
+// What changes is the schedule: one captured graph per step advances all
+// replicas, and a replica whose pairlist expired is rebuilt inside that
+// graph while the others keep stepping.
+//
+// Layout in HBM: fixed per-replica regions of shared arrays.
+//   reals    slots [rbase, rbase + n)                        of NR = sum n
+//   images   slots NR + [ibase, ibase + 7n)  (a real has at most 7 images)
+//   cells    [cbase, cbase + m^3)           padded cell grid of the replica
+//   tiles    [tbase, tbase + rows + ceil(n / T))    T = tile size in reals
+//   lists    [lbase, lbase + tcap * kcap * T)  u16 LDS offsets per tile
+// so a rebuild of one replica writes its own regions only.
+//
+// Per step (one graph launch):
+//   rebuild   nine small kernels over the replicas flagged by the last guard
+//             (wrap, bin, images, counting sort, tiles, lists); they return
+//             at once when no replica is flagged
+//   step      one persistent kernel over the tiles of all replicas: stage the
+//             tile's partners in LDS (images formed from their source's
+//             mid-step position on the fly), kick, trailing half drift, and
+//             the next step's leading half drift into the other mid-step
+//             buffer
+//   guard     one workgroup per replica: |p|max, the velocity budget and
+//             CheckByDisplacement (pairlist.cc:26-121) for the next step
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mdx.h"
+#include "mdx_common.h"
+
+namespace {
+
+using namespace mdxk;
+
+constexpr int kRebuildBlocks = 1024;   // grid of the per-particle rebuild kernels
+constexpr int kReplicaThreads = 1024;  // one workgroup per replica: scans, guard
+constexpr int kMaxCand = 1024;         // CheckByDisplacement candidates held in LDS
+constexpr int kMaxImages = 7;          // images per real
+
+enum EnsFlags : int { EF_INVALID_POS = 1, EF_STAGE = 2, EF_LIST = 4, EF_TOO_FAST = 8, EF_IMAGES = 16 };
+
+struct Rep {
+  Geom g;
+  size_t lbase;      // first list entry of its tiles
+  int n, rbase;      // reals: slots [rbase, rbase + n)
+  int ibase, gcap;   // images: slots NR + [ibase, ibase + gcap)
+  int cbase, ncell;  // padded cell grid
+  int tbase, tcap;   // tile slots
+  int kcap, pad;     // list entries per real
+};
+
+// Device-resident state machine of one replica (PairList, pairlist.h:10-31,
+// plus the step bookkeeping).  Written by the rebuild, step and guard
+// kernels; read back by the host between calls.
+struct RepCtl {
+  double budget;     // PairList::buffer_length
+  double vmax2;      // max |p|^2 after the last step
+  int need_rebuild;  // pairlist expired: rebuild before the next step
+  int eflags;        // failure bits (EnsFlags) of the last rebuild / guard
+  int error;         // mdx_status that stopped the replica, 0 while running
+  int g;             // images in use
+  int ntiles, stage_max, kmax;
+  int steps, target;  // steps done since mdx_ensemble_set_state, steps requested
+  int rebuilds;       // pairlist constructions since mdx_ensemble_set_state
+  int par;            // mid-step buffer (qh0 / qh1) that holds the next step's q + p dt/2
+  int stepped;        // a step ran since the last rebuild
+  int cand;           // candidates of the last displacement check
+  int pad;
+};
+
+struct EnsDev {
+  const Rep *rep;
+  RepCtl *ctl;
+  int *meta;      // meta[0]: number of replicas flagged for a rebuild in this step
+  int *flagged;   // their ids
+  int R, NR, NI, NT;
+  int T;          // tile size (reals per tile)
+  int stage_cap;  // particles a tile may stage (dynamic LDS / 24 B)
+  int check_list_length;
+  double dt, buffer, SL, C2_8;
+  double4 *qe;    // [NR + NI] reals at the end of the step; images as built
+  double4 *qh0, *qh1;   // [NR] mid-step positions q + p dt/2 (two buffers)
+  double4 *mom, *ref;   // [NR], [NR + NI] snapshot at the last build
+  int *tag;
+  int *gsrc, *gcode;    // [NI] image -> real slot, shift code
+  double4 *pos_tmp, *mom_tmp, *gpos_tmp;
+  int *tag_tmp, *gsrc_tmp, *gcode_tmp;
+  int *cellid, *slot, *nimg, *goff, *order, *inv;
+  int *cell_count, *cnt_in, *cnt_out, *scan_in, *scan_out, *cell_start;
+  TileDesc *tiles;
+  unsigned short *list;
+  int *nn;
+  double *partial;      // [NT] max |p|^2 per tile
+  double *obs_partial;  // [3 NT] KE, PE, virial per tile
+  int *tile_kmax;       // [NT]
+};
+
+// flagged for a rebuild in this step: expired and about to step (a replica
+// that has finished its steps keeps its state until it is asked for more)
+__device__ __forceinline__ bool rebuild_due(const RepCtl *c) {
+  return c->need_rebuild && !c->error && c->steps < c->target;
+}
+
+__device__ __forceinline__ bool rebuilding(const RepCtl *c) { return rebuild_due(c) && !c->eflags; }
+
+__device__ __forceinline__ bool stepping(const RepCtl *c) {
+  return !c->error && !c->need_rebuild && c->steps < c->target;
+}
+
+// ------------------------------------------------------------ block helpers
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(v, o);
+    if (lane >= o) v += y;
+  }
+  return v;
+}
+
+// exclusive scan over the block (blockDim a multiple of 64, <= 1024)
+__device__ int block_excl_scan(int v, int *total) {
+  __shared__ int wsum[16];
+  __shared__ int tot;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int incl = wave_incl_scan(v);
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  if (w == 0) {
+    const int s = lane < nw ? wsum[lane] : 0;
+    const int si = wave_incl_scan(s);
+    if (lane < nw) wsum[lane] = si - s;
+    if (lane == 63) tot = si;
+  }
+  __syncthreads();
+  const int ex = incl - v + wsum[w];
+  *total = tot;
+  __syncthreads();
+  return ex;
+}
+
+__device__ double block_max(double v) {
+  __shared__ double wm[16];
+  __shared__ double res;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_max(v);
+  if (lane == 0) wm[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wm[0];
+    for (int k = 1; k < nw; k++) m = fmax(m, wm[k]);
+    res = m;
+  }
+  __syncthreads();
+  const double r = res;
+  __syncthreads();
+  return r;
+}
+
+__device__ int block_max_int(int v) {
+  __shared__ int wm[16];
+  __shared__ int res;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+  if (lane == 0) wm[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = wm[0];
+    for (int k = 1; k < nw; k++) m = max(m, wm[k]);
+    res = m;
+  }
+  __syncthreads();
+  const int r = res;
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------ tiles
+// [start, len) of segment s (0..17) of a tile of replica rp, as absolute
+// slots: even s reals (interior cells), odd s images (outer cells).
+__device__ __forceinline__ void e_segment(const EnsDev &d, const Rep &rp, int y, int z, int xf, int xl,
+                                          int s, int *start, int *len) {
+  const int r = s >> 1, dy = r % 3 - 1, dz = r / 3 - 1;
+  const int a = rp.cbase + cell_linear(rp.g, xf - 1, y + dy, z + dz);
+  const int b = rp.cbase + cell_linear(rp.g, xl + 1, y + dy, z + dz);
+  if ((s & 1) == 0) {
+    *start = d.scan_in[a];
+    *len = d.scan_in[b] + d.cnt_in[b] - *start;
+  } else {
+    *start = d.scan_out[a];
+    *len = d.scan_out[b] + d.cnt_out[b] - *start;
+  }
+}
+
+// Stage the 18 segments of tile t into LDS as x[], y[], z[].  Reals come
+// from `reals`; images from the positions stored at the build (mid ==
+// nullptr) or as their source's mid-step position plus the shift, which is
+// the image refresh of MDUnit::MakeBufferForBorderParticles
+// (mdunit.cc:181-191) done while staging.  Every thread calls it.
+__device__ void e_stage(const EnsDev &d, const Rep &rp, const TileDesc &t, const double4 *reals,
+                        const double4 *mid, TileSmem &sm, double *sx, double *sy, double *sz) {
+  if (threadIdx.x < kSeg) {
+    int st, ln;
+    e_segment(d, rp, t.y, t.z, t.xf, t.xl, threadIdx.x, &st, &ln);
+    sm.seg_start[threadIdx.x] = st;
+    sm.seg_base[threadIdx.x + 1] = ln;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sm.seg_base[0] = 0;
+    for (int k = 0; k < kSeg; k++) sm.seg_base[k + 1] += sm.seg_base[k];
+  }
+  __syncthreads();
+  for (int k = 0; k < kSeg; k++) {
+    const int b = sm.seg_base[k], len = sm.seg_base[k + 1] - b, st = sm.seg_start[k];
+    for (int u = threadIdx.x; u < len; u += blockDim.x) {
+      double4 q;
+      if ((k & 1) == 0) {
+        q = reals[st + u];
+      } else if (mid) {
+        const int idx = st + u - d.NR;
+        q = image_of(mid[d.gsrc[idx]], d.gcode[idx], rp.g.L);
+      } else {
+        q = d.qe[st + u];
+      }
+      sx[b + u] = q.x;
+      sy[b + u] = q.y;
+      sz[b + u] = q.z;
+    }
+  }
+  __syncthreads();
+}
+
+// Persistent tile loop: workgroup b runs on XCD b % 8; the workgroups of one
+// XCD walk one contiguous eighth of the tile slots, so neighbouring rows of
+// a replica share that XCD's L2.
+struct SlotWalk {
+  int s, end, stride;
+};
+__device__ __forceinline__ SlotWalk slot_walk(int NT) {
+  const int chunk = (NT + 7) >> 3;
+  const int x = blockIdx.x & 7, j = blockIdx.x >> 3, nw = gridDim.x >> 3;
+  SlotWalk w;
+  w.s = x * chunk + j;
+  w.end = min(NT, (x + 1) * chunk);
+  w.stride = nw;
+  return w;
+}
+
+// ------------------------------------------------------------ rebuild
+// MDManager::MakePairList (mdmanager.cc:388-417) for the flagged replicas.
+__global__ void k_e_begin(EnsDev d) {
+  const int r = blockIdx.x;
+  RepCtl *c = d.ctl + r;
+  if (!rebuild_due(c)) return;
+  const Rep &rp = d.rep[r];
+  for (int k = threadIdx.x; k < rp.ncell; k += blockDim.x) d.cell_count[rp.cbase + k] = 0;
+  if (threadIdx.x == 0) {
+    c->eflags = 0;
+    d.flagged[atomicAdd(d.meta, 1)] = r;
+  }
+}
+
+// wrap into the box (variables.cc:64-84), bin, count images (mdrect.cc:24-47);
+// the pre-sort copies go to the *_tmp arrays
+__global__ void k_e_wrap(EnsDev d) {
+  const int nf = d.meta[0];
+  for (int f = 0; f < nf; f++) {
+    const int r = d.flagged[f];
+    const Rep &rp = d.rep[r];
+    const Geom &g = rp.g;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < rp.n; k += gridDim.x * blockDim.x) {
+      const int i = rp.rbase + k;
+      double4 q = d.qe[i];
+      double x[3] = {q.x, q.y, q.z};
+      bool bad = false;
+      int c[3], cnt = 1;
+      for (int a = 0; a < 3; a++) {
+        if (x[a] < 0.0) x[a] += g.L[a];
+        else if (x[a] > g.L[a]) x[a] -= g.L[a];
+        if (!(x[a] >= 0.0 && x[a] <= g.L[a])) bad = true;
+        c[a] = bad ? 1 : interior_coord(x[a], g.inv_w[a], g.ms[a]);
+        const bool left = x[a] < d.SL && x[a] > 0.0;
+        const bool right = x[a] > g.L[a] - d.SL && x[a] < g.L[a];
+        cnt *= (left || right) ? 2 : 1;
+      }
+      if (bad) atomicOr(&d.ctl[r].eflags, EF_INVALID_POS);
+      q.x = x[0];
+      q.y = x[1];
+      q.z = x[2];
+      d.pos_tmp[i] = q;
+      d.mom_tmp[i] = d.mom[i];
+      d.tag_tmp[i] = d.tag[i];
+      const int cl = rp.cbase + cell_linear(g, c[0], c[1], c[2]);
+      d.cellid[i] = cl;
+      d.slot[i] = atomicAdd(&d.cell_count[cl], 1);
+      d.nimg[i] = cnt - 1;
+    }
+  }
+}
+
+// image offsets: exclusive scan of the image counts of each flagged replica
+__global__ void __launch_bounds__(kReplicaThreads) k_e_img_scan(EnsDev d) {
+  const int r = blockIdx.x;
+  RepCtl *c = d.ctl + r;
+  if (!rebuilding(c)) return;
+  const Rep &rp = d.rep[r];
+  int carry = 0;
+  for (int base = 0; base < rp.n; base += blockDim.x) {
+    const int k = base + threadIdx.x;
+    const int v = k < rp.n ? d.nimg[rp.rbase + k] : 0;
+    int tot;
+    const int ex = block_excl_scan(v, &tot);
+    if (k < rp.n) d.goff[rp.rbase + k] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    c->g = carry;
+    if (carry > rp.gcap) c->eflags |= EF_IMAGES;
+  }
+}
+
+// images of each real in the fixed (sz, sy, sx) enumeration, with their
+// outer-layer cells (MDUnit::FindBorderParticles, mdunit.cc:153-192)
+__global__ void k_e_images(EnsDev d) {
+  const int nf = d.meta[0];
+  for (int f = 0; f < nf; f++) {
+    const int r = d.flagged[f];
+    if (d.ctl[r].eflags) continue;
+    const Rep &rp = d.rep[r];
+    const Geom &g = rp.g;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < rp.n; k += gridDim.x * blockDim.x) {
+      const int i = rp.rbase + k;
+      if (d.nimg[i] == 0) continue;
+      const double4 q = d.pos_tmp[i];
+      const double x[3] = {q.x, q.y, q.z};
+      int sh[3];
+      for (int a = 0; a < 3; a++) {
+        const bool left = x[a] < d.SL && x[a] > 0.0;
+        const bool right = x[a] > g.L[a] - d.SL && x[a] < g.L[a];
+        sh[a] = left ? 1 : (right ? -1 : 0);
+      }
+      const int cl = d.cellid[i] - rp.cbase;
+      const int cx = cl % g.m[0], cy = (cl / g.m[0]) % g.m[1], cz = cl / (g.m[0] * g.m[1]);
+      int idx = rp.ibase + d.goff[i];
+      for (int bz = 0; bz <= (sh[2] != 0); bz++)
+        for (int by = 0; by <= (sh[1] != 0); by++)
+          for (int bx = 0; bx <= (sh[0] != 0); bx++) {
+            if (bx == 0 && by == 0 && bz == 0) continue;
+            const int s[3] = {bx ? sh[0] : 0, by ? sh[1] : 0, bz ? sh[2] : 0};
+            double4 im;
+            im.x = shifted(x[0], s[0], g.L[0]);
+            im.y = shifted(x[1], s[1], g.L[1]);
+            im.z = shifted(x[2], s[2], g.L[2]);
+            im.w = 0.0;
+            d.gpos_tmp[idx] = im;
+            d.gsrc_tmp[idx] = i;
+            d.gcode_tmp[idx] = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
+            const int gx = s[0] > 0 ? g.m[0] - 1 : (s[0] < 0 ? 0 : cx);
+            const int gy = s[1] > 0 ? g.m[1] - 1 : (s[1] < 0 ? 0 : cy);
+            const int gz = s[2] > 0 ? g.m[2] - 1 : (s[2] < 0 ? 0 : cz);
+            const int gcl = rp.cbase + cell_linear(g, gx, gy, gz);
+            d.cellid[d.NR + idx] = gcl;
+            d.slot[d.NR + idx] = atomicAdd(&d.cell_count[gcl], 1);
+            idx++;
+          }
+    }
+  }
+}
+
+// Per flagged replica: split the cell counts into reals (interior) and
+// images (outer layer), scan each into absolute slots (the counting sort of
+// MeshList::MakeMesh, meshlist.cc:176-228), then cut every interior cell
+// row into tiles of at most T consecutive reals and record each tile's
+// staging size.
+__global__ void __launch_bounds__(kReplicaThreads) k_e_cells(EnsDev d) {
+  const int r = blockIdx.x;
+  RepCtl *c = d.ctl + r;
+  if (!rebuilding(c)) return;
+  const Rep &rp = d.rep[r];
+  const Geom &g = rp.g;
+  int carry_in = 0, carry_out = 0;
+  for (int base = 0; base < rp.ncell; base += blockDim.x) {
+    const int k = base + threadIdx.x;
+    int ci = 0, co = 0;
+    if (k < rp.ncell) {
+      const int x = k % g.m[0], y = (k / g.m[0]) % g.m[1], z = k / (g.m[0] * g.m[1]);
+      const bool inner = x >= 1 && x <= g.ms[0] && y >= 1 && y <= g.ms[1] && z >= 1 && z <= g.ms[2];
+      const int cnt = d.cell_count[rp.cbase + k];
+      ci = inner ? cnt : 0;
+      co = inner ? 0 : cnt;
+    }
+    int tin, tout;
+    const int ex_in = block_excl_scan(ci, &tin);
+    const int ex_out = block_excl_scan(co, &tout);
+    if (k < rp.ncell) {
+      const int cl = rp.cbase + k;
+      const int si = rp.rbase + carry_in + ex_in;
+      const int so = d.NR + rp.ibase + carry_out + ex_out;
+      d.cnt_in[cl] = ci;
+      d.cnt_out[cl] = co;
+      d.scan_in[cl] = si;
+      d.scan_out[cl] = so;
+      d.cell_start[cl] = ci > 0 ? si : so;
+    }
+    carry_in += tin;
+    carry_out += tout;
+  }
+  __syncthreads();
+  const int T = d.T;
+  const int rows = g.ms[1] * g.ms[2];
+  int tcarry = 0, smax = 0;
+  for (int base = 0; base < rows; base += blockDim.x) {
+    const int w = base + threadIdx.x;
+    int y = 0, z = 0, first = 0, cnt = 0, nt = 0;
+    if (w < rows) {
+      y = w % g.ms[1] + 1;
+      z = w / g.ms[1] + 1;
+      const int a = rp.cbase + cell_linear(g, 1, y, z), b = rp.cbase + cell_linear(g, g.ms[0], y, z);
+      first = d.scan_in[a];
+      cnt = d.scan_in[b] + d.cnt_in[b] - first;
+      nt = (cnt + T - 1) / T;
+    }
+    int tot;
+    const int ex = block_excl_scan(nt, &tot);
+    for (int k = 0; k < nt; k++) {
+      TileDesc t;
+      t.i0 = first + k * T;
+      t.ni = min(T, cnt - k * T);
+      t.y = y;
+      t.z = z;
+      // cells holding the tile's first and last real (row-local slots)
+      const int lo = k * T, hi = k * T + t.ni - 1;
+      int acc = 0, xf = 1, xl = 1;
+      for (int x = 1; x <= g.ms[0]; x++) {
+        const int cc = d.cnt_in[rp.cbase + cell_linear(g, x, y, z)];
+        if (cc > 0 && lo >= acc && lo < acc + cc) xf = x;
+        if (cc > 0 && hi >= acc && hi < acc + cc) xl = x;
+        acc += cc;
+      }
+      t.xf = xf;
+      t.xl = xl;
+      int stage = 0;
+      for (int sgi = 0; sgi < kSeg; sgi++) {
+        int st, ln;
+        e_segment(d, rp, y, z, xf, xl, sgi, &st, &ln);
+        stage += ln;
+      }
+      t.stage = stage;
+      t.pad = r;
+      d.tiles[rp.tbase + tcarry + ex + k] = t;
+      smax = max(smax, stage);
+    }
+    tcarry += tot;
+  }
+  // unused slots of the region: empty tiles
+  for (int s = tcarry + threadIdx.x; s < rp.tcap; s += blockDim.x) {
+    TileDesc t{};
+    t.pad = r;
+    d.tiles[rp.tbase + s] = t;
+  }
+  smax = block_max_int(smax);
+  if (threadIdx.x == 0) {
+    c->ntiles = tcarry;
+    c->stage_max = smax;
+    if (smax > d.stage_cap || tcarry > rp.tcap) c->eflags |= EF_STAGE;
+  }
+}
+
+// place pre-sort key k (real slot, or NR + image index) into its cell
+__global__ void k_e_scatter(EnsDev d) {
+  const int nf = d.meta[0];
+  for (int f = 0; f < nf; f++) {
+    const int r = d.flagged[f];
+    if (d.ctl[r].eflags) continue;
+    const Rep &rp = d.rep[r];
+    const int g = d.ctl[r].g;
+    const int stride = gridDim.x * blockDim.x;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < rp.n + g; k += stride) {
+      const int key = k < rp.n ? rp.rbase + k : d.NR + rp.ibase + (k - rp.n);
+      d.order[d.cell_start[d.cellid[key]] + d.slot[key]] = key;
+    }
+  }
+}
+
+// atomics fixed the cell, not the order in it: sort each cell by pre-sort
+// key, which makes the result a stable counting sort
+__global__ void k_e_cell_sort(EnsDev d) {
+  const int nf = d.meta[0];
+  for (int f = 0; f < nf; f++) {
+    const int r = d.flagged[f];
+    if (d.ctl[r].eflags) continue;
+    const Rep &rp = d.rep[r];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < rp.ncell; k += gridDim.x * blockDim.x) {
+      const int cl = rp.cbase + k;
+      int *seg = d.order + d.cell_start[cl];
+      const int len = d.cell_count[cl];
+      for (int a = 1; a < len; a++) {
+        const int v = seg[a];
+        int b = a - 1;
+        while (b >= 0 && seg[b] > v) {
+          seg[b + 1] = seg[b];
+          b--;
+        }
+        seg[b + 1] = v;
+      }
+    }
+  }
+}
+
+// reals into cell order, snapshot (PairList::Init, pairlist.cc:11-23), and
+// the first half drift of the coming step into the current mid-step buffer
+__global__ void k_e_gather_reals(EnsDev d) {
+  const int nf = d.meta[0];
+  const double dt2 = d.dt * 0.5;
+  for (int f = 0; f < nf; f++) {
+    const int r = d.flagged[f];
+    if (d.ctl[r].eflags) continue;
+    const Rep &rp = d.rep[r];
+    double4 *mid = d.ctl[r].par ? d.qh1 : d.qh0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < rp.n; k += gridDim.x * blockDim.x) {
+      const int s = rp.rbase + k;
+      const int key = d.order[s];
+      const double4 q = d.pos_tmp[key];
+      const double4 p = d.mom_tmp[key];
+      d.qe[s] = q;
+      d.ref[s] = q;
+      d.mom[s] = p;
+      d.tag[s] = d.tag_tmp[key];
+      d.inv[key] = s;
+      mid[s] = drift_half(q, p, dt2);
+    }
+  }
+}
+
+__global__ void k_e_gather_images(EnsDev d) {
+  const int nf = d.meta[0];
+  for (int f = 0; f < nf; f++) {
+    const int r = d.flagged[f];
+    if (d.ctl[r].eflags) continue;
+    const Rep &rp = d.rep[r];
+    const int g = d.ctl[r].g;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < g; k += gridDim.x * blockDim.x) {
+      const int s = d.NR + rp.ibase + k;
+      const int pre = d.order[s] - d.NR;
+      const double4 q = d.gpos_tmp[pre];
+      d.qe[s] = q;
+      d.ref[s] = q;
+      d.gsrc[s - d.NR] = d.inv[d.gsrc_tmp[pre]];
+      d.gcode[s - d.NR] = d.gcode_tmp[pre];
+    }
+  }
+}
+
+// Full neighbour list per tile (as the single-system build): lane = real,
+// every partner in its 27 cells with r^2 <= SL^2 (meshlist.cc:354-362),
+// stored as LDS offsets at list[lbase + ((tile * kcap) + k) * T + lane].
+template <int T>
+__global__ void __launch_bounds__(T) k_e_build(EnsDev d) {
+  extern __shared__ double stage_mem[];
+  __shared__ TileSmem sm;
+  const int nf = d.meta[0];
+  for (int f = 0; f < nf; f++) {
+    const int r = d.flagged[f];
+    if (d.ctl[r].eflags) continue;
+    const Rep &rp = d.rep[r];
+    const Geom &g = rp.g;
+    const int nt = d.ctl[r].ntiles;
+    for (int s = blockIdx.x; s < nt; s += gridDim.x) {
+      const TileDesc t = d.tiles[rp.tbase + s];
+      double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+      e_stage(d, rp, t, d.qe, nullptr, sm, sx, sy, sz);
+      const int lane = threadIdx.x;
+      int cnt = 0;
+      if (lane < t.ni) {
+        const int i = t.i0 + lane;
+        const int self = sm.seg_base[8] + (i - sm.seg_start[8]);
+        const double qx = sx[self], qy = sy[self], qz = sz[self];
+        const int cx = interior_coord(qx, g.inv_w[0], g.ms[0]);
+        unsigned short *out = d.list + rp.lbase + (size_t)s * rp.kcap * T + lane;
+        for (int rr = 0; rr < 9; rr++) {
+          const int y = t.y + rr % 3 - 1, z = t.z + rr / 3 - 1;
+          for (int dx = -1; dx <= 1; dx++) {
+            const int x = cx + dx;
+            const int cl = rp.cbase + cell_linear(g, x, y, z);
+            int lo, len;
+            if (is_outer(g, x, y, z)) {
+              lo = sm.seg_base[2 * rr + 1] + (d.scan_out[cl] - sm.seg_start[2 * rr + 1]);
+              len = d.cnt_out[cl];
+            } else {
+              lo = sm.seg_base[2 * rr] + (d.scan_in[cl] - sm.seg_start[2 * rr]);
+              len = d.cnt_in[cl];
+            }
+            for (int o = lo; o < lo + len; o++) {
+              if (o == self) continue;
+              if (list_r2(sx[o], sy[o], sz[o], qx, qy, qz) <= g.SL2) {
+                if (cnt < rp.kcap) out[(size_t)cnt * T] = (unsigned short)o;
+                cnt++;
+              }
+            }
+          }
+        }
+        d.nn[i] = cnt < rp.kcap ? cnt : rp.kcap;
+      }
+      const int km = block_max_int(cnt);
+      if (threadIdx.x == 0) d.tile_kmax[rp.tbase + s] = km;
+      __syncthreads();
+    }
+  }
+}
+
+// close the rebuild: longest list against the capacity, then the reset of
+// the velocity budget (pairlist.cc:11-23)
+__global__ void k_e_end(EnsDev d) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.meta[0] = 0;
+  const int r = blockIdx.x;
+  RepCtl *c = d.ctl + r;
+  if (!rebuild_due(c)) return;
+  const Rep &rp = d.rep[r];
+  int km = 0;
+  if (!c->eflags)
+    for (int s = threadIdx.x; s < c->ntiles; s += blockDim.x) km = max(km, d.tile_kmax[rp.tbase + s]);
+  km = block_max_int(km);
+  if (threadIdx.x == 0) {
+    int ef = c->eflags;
+    c->kmax = km;
+    if (km > rp.kcap) ef |= EF_LIST;
+    c->eflags = ef;
+    if (ef) {
+      c->error = (ef & EF_INVALID_POS) ? MDX_ERR_INVALID_POS : MDX_ERR_CAPACITY;
+    } else {
+      c->budget = d.buffer;
+      c->need_rebuild = 0;
+      c->rebuilds++;
+      c->stepped = 0;
+    }
+  }
+}
+
+// ------------------------------------------------------------ step
+// One NVE step (MDManager::Calculate, mdmanager.cc:154-186) of every tile
+// of every stepping replica: partners staged from the mid-step positions
+// q + p dt/2 (images formed on the fly), p += F dt over the tile lists,
+// q_end = q_mid + p dt/2 (fcalculator.cc:18-30), and the next step's
+// q_end + p dt/2 into the other mid-step buffer.  The arithmetic is the
+// single-system kick's (kick_term, kILP chains, drift_half).
+template <int T>
+__global__ void __launch_bounds__(T) k_e_step(EnsDev d) {
+  extern __shared__ double stage_mem[];
+  __shared__ TileSmem sm;
+  const SlotWalk w = slot_walk(d.NT);
+  const double dt2 = d.dt * 0.5;
+  for (int slot = w.s; slot < w.end; slot += w.stride) {
+    const TileDesc t = d.tiles[slot];
+    if (t.ni == 0) continue;
+    const int r = t.pad;
+    const RepCtl *c = d.ctl + r;
+    if (!stepping(c)) continue;
+    const Rep &rp = d.rep[r];
+    const double4 *qh = c->par ? d.qh1 : d.qh0;
+    double4 *qn = c->par ? d.qh0 : d.qh1;
+    double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+    e_stage(d, rp, t, qh, qh, sm, sx, sy, sz);
+    const int lane = threadIdx.x;
+    double v2 = 0.0;
+    if (lane < t.ni) {
+      const int i = t.i0 + lane;
+      const double4 qi = qh[i];
+      const int m = d.nn[i];
+      const unsigned short *lst = d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * T + lane;
+      const double CL2 = rp.g.CL2;
+      double ax[kILP], ay[kILP], az[kILP];
+#pragma unroll
+      for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
+      const int mg = m - m % kILP;
+      int nxt[kILP];
+#pragma unroll
+      for (int u = 0; u < kILP; u++) nxt[u] = u < mg ? lst[(size_t)u * T] : 0;
+      for (int k = 0; k < mg; k += kILP) {
+        int cur[kILP];
+#pragma unroll
+        for (int u = 0; u < kILP; u++) cur[u] = nxt[u];
+        if (k + kILP < mg) {
+#pragma unroll
+          for (int u = 0; u < kILP; u++) nxt[u] = lst[(size_t)(k + kILP + u) * T];
+        }
+#pragma unroll
+        for (int u = 0; u < kILP; u++) {
+          const int o = cur[u];
+          kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, d.C2_8, d.dt, CL2, ax[u], ay[u], az[u]);
+        }
+      }
+      for (int k = mg; k < m; k++) {
+        const int o = lst[(size_t)k * T];
+        kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, d.C2_8, d.dt, CL2, ax[0], ay[0], az[0]);
+      }
+      double fx = 0.0, fy = 0.0, fz = 0.0;
+#pragma unroll
+      for (int u = 0; u < kILP; u++) {
+        fx += ax[u];
+        fy += ay[u];
+        fz += az[u];
+      }
+      double4 p = d.mom[i];
+      p.x += fx;
+      p.y += fy;
+      p.z += fz;
+      d.mom[i] = p;
+      v2 = speed2(p);
+      const double4 q = drift_half(qi, p, dt2);
+      d.qe[i] = q;
+      qn[i] = drift_half(q, p, dt2);
+    }
+    v2 = T == 64 ? wave_max(v2) : block_max(v2);
+    if (threadIdx.x == 0) d.partial[slot] = v2;
+    __syncthreads();
+  }
+}
+
+// Expiry check for the next step, one workgroup per stepping replica:
+// PairList::IsPairListExpired / UpdatePairListValidity /
+// CheckByDisplacement (pairlist.cc:26-121).  Images enter the displacement
+// test at their mid-step positions, as the reference's ghosts do after a
+// step.  The budget arithmetic is rounded operation by operation like the
+// host's, and sqrt is the correctly rounded one.
+__global__ void __launch_bounds__(kReplicaThreads) k_e_guard(EnsDev d) {
+#pragma clang fp contract(off)
+  __shared__ int cand[kMaxCand];
+  __shared__ int ncand, hit;
+  const int r = blockIdx.x;
+  RepCtl *c = d.ctl + r;
+  if (!stepping(c)) return;
+  const Rep &rp = d.rep[r];
+  const int tid = threadIdx.x;
+  double v = 0.0;
+  for (int s = tid; s < c->ntiles; s += blockDim.x) v = fmax(v, d.partial[rp.tbase + s]);
+  const double v2max = block_max(v);
+  const int par = c->par;
+  double budget = c->budget;
+  const int g = c->g, tn = rp.n + g;
+  __syncthreads();
+  if (tid == 0) {
+    c->steps = c->steps + 1;
+    c->par = par ^ 1;
+    c->stepped = 1;
+    c->vmax2 = v2max;
+  }
+  if (budget > 0.0) {
+    const double vmax = __dsqrt_rn(v2max);
+    const double move = vmax * 2.0 * d.dt;
+    if (move > d.buffer) {
+      if (tid == 0) {
+        c->eflags |= EF_TOO_FAST;
+        c->error = MDX_ERR_TOO_FAST;
+      }
+      return;
+    }
+    budget = budget - move;
+  }
+  int expired = 0;
+  if (budget < 0.0) {
+    const double4 *mid = par ? d.qh1 : d.qh0;   // this step's mid-step positions
+    double m = 0.0;
+    for (int k = tid; k < tn; k += blockDim.x) {
+      const int s = k < rp.n ? rp.rbase + k : d.NR + rp.ibase + (k - rp.n);
+      const double4 now = k < rp.n ? d.qe[s] : image_of(mid[d.gsrc[s - d.NR]], d.gcode[s - d.NR], rp.g.L);
+      m = fmax(m, disp2(d.ref[s], now));
+    }
+    const double dmax2 = block_max(m);
+    const int Nl = tn > d.check_list_length ? d.check_list_length : tn;
+    const double c_distance = d.buffer - __dsqrt_rn(dmax2);
+    const double c_d2 = c_distance * c_distance;
+    if (tid == 0) {
+      ncand = 0;
+      hit = 0;
+    }
+    __syncthreads();
+    for (int k = tid; k < tn; k += blockDim.x) {
+      const int s = k < rp.n ? rp.rbase + k : d.NR + rp.ibase + (k - rp.n);
+      const double4 now = k < rp.n ? d.qe[s] : image_of(mid[d.gsrc[s - d.NR]], d.gcode[s - d.NR], rp.g.L);
+      if (disp2(d.ref[s], now) > c_d2) {
+        const int at = atomicAdd(&ncand, 1);
+        if (at < kMaxCand) cand[at] = s;
+      }
+    }
+    __syncthreads();
+    const int l = ncand;
+    if (l >= Nl) {
+      expired = 1;
+    } else if (l > 1) {
+      for (long long p = tid; p < (long long)l * l; p += blockDim.x) {
+        const int a = (int)(p / l), b = (int)(p % l);
+        if (b <= a) continue;
+        const int s1 = cand[a], s2 = cand[b];
+        if (disp2(d.ref[s1], d.ref[s2]) < rp.g.SL2) continue;
+        const double4 n1 = s1 < d.NR ? d.qe[s1]
+                                     : image_of(mid[d.gsrc[s1 - d.NR]], d.gcode[s1 - d.NR], rp.g.L);
+        const double4 n2 = s2 < d.NR ? d.qe[s2]
+                                     : image_of(mid[d.gsrc[s2 - d.NR]], d.gcode[s2 - d.NR], rp.g.L);
+        if (disp2(n1, n2) < rp.g.CL2) hit = 1;
+      }
+      __syncthreads();
+      expired = hit;
+    }
+    if (tid == 0) c->cand = l;
+  }
+  if (tid == 0) {
+    c->budget = budget;
+    c->need_rebuild = expired;
+  }
+}
+
+// ------------------------------------------------------------ observe
+// KE, PE and virial per tile (observer.cc:5-79 over the full list, halves
+// for pairs counted from both sides); images at the positions the reference's
+// ghosts hold: the last step's mid-step refresh, or the build if no step ran.
+template <int T>
+__global__ void __launch_bounds__(T) k_e_observe(EnsDev d, double C0, double C2) {
+  extern __shared__ double stage_mem[];
+  __shared__ TileSmem sm;
+  __shared__ double red[3][T];
+  const int slot = blockIdx.x;
+  const TileDesc t = d.tiles[slot];
+  const int r = t.pad;
+  const RepCtl *c = d.ctl + r;
+  const int lane = threadIdx.x;
+  if (t.ni == 0 || c->error || c->need_rebuild) {
+    if (lane < 3) d.obs_partial[3 * (size_t)slot + lane] = 0.0;
+    return;
+  }
+  const Rep &rp = d.rep[r];
+  const double4 *mid = c->stepped ? (c->par ? d.qh0 : d.qh1) : nullptr;
+  double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+  e_stage(d, rp, t, d.qe, mid, sm, sx, sy, sz);
+  double ke = 0.0, pe = 0.0, vir = 0.0;
+  if (lane < t.ni) {
+    const int i = t.i0 + lane;
+    const double4 p = d.mom[i];
+    ke = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z;
+    const double4 qi = d.qe[i];
+    const unsigned short *lst = d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * T + lane;
+    for (int k = 0; k < d.nn[i]; k++) {
+      const int o = lst[(size_t)k * T];
+      const double dx = qi.x - sx[o], dy = qi.y - sy[o], dz = qi.z - sz[o];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 > rp.g.CL2) continue;
+      observe_term(r2, C0, C2, pe, vir);
+    }
+    pe *= 0.5;
+    vir *= 0.5;
+  }
+  red[0][lane] = ke;
+  red[1][lane] = pe;
+  red[2][lane] = vir;
+  __syncthreads();
+  for (int off = T / 2; off > 0; off >>= 1) {
+    if (lane < off) {
+      red[0][lane] += red[0][lane + off];
+      red[1][lane] += red[1][lane + off];
+      red[2][lane] += red[2][lane + off];
+    }
+    __syncthreads();
+  }
+  if (lane < 3) d.obs_partial[3 * (size_t)slot + lane] = red[lane][0];
+}
+
+// per replica, the tile sums in tile order (fixed tree)
+__global__ void k_e_observe_fold(EnsDev d, double *__restrict__ out) {
+  __shared__ double lds[3][kBlock];
+  const int r = blockIdx.x, t = threadIdx.x;
+  const Rep &rp = d.rep[r];
+  const int nt = d.ctl[r].ntiles;
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int k = t; k < nt; k += kBlock) {
+    const size_t b = 3 * (size_t)(rp.tbase + k);
+    s0 += d.obs_partial[b];
+    s1 += d.obs_partial[b + 1];
+    s2 += d.obs_partial[b + 2];
+  }
+  lds[0][t] = s0;
+  lds[1][t] = s1;
+  lds[2][t] = s2;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (t < off) {
+      lds[0][t] += lds[0][t + off];
+      lds[1][t] += lds[1][t + off];
+      lds[2][t] += lds[2][t + off];
+    }
+    __syncthreads();
+  }
+  if (t < 3) out[3 * (size_t)r + t] = lds[t][0];
+}
+
+// ------------------------------------------------------------ state i/o
+__global__ void k_e_load(const double *__restrict__ q3, const double *__restrict__ p3, int n,
+                         double4 *__restrict__ qe, double4 *__restrict__ mom, int *__restrict__ tag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  qe[i] = make_double4(q3[3 * (size_t)i], q3[3 * (size_t)i + 1], q3[3 * (size_t)i + 2], 0.0);
+  mom[i] = make_double4(p3[3 * (size_t)i], p3[3 * (size_t)i + 1], p3[3 * (size_t)i + 2], 0.0);
+  tag[i] = i;
+}
+
+__global__ void k_e_scatter_by_tag(const double4 *__restrict__ src, const int *__restrict__ tag, int n,
+                                   double *__restrict__ out3) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 v = src[i];
+  const int t = tag[i];
+  out3[3 * (size_t)t + 0] = v.x;
+  out3[3 * (size_t)t + 1] = v.y;
+  out3[3 * (size_t)t + 2] = v.z;
+}
+
+inline int nblk(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+thread_local std::string g_ens_create_error;
+
+}  // namespace
+
+// ====================================================================== host
+struct mdx_ensemble {
+  mdx_params prm{};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  double SL = 0, C0 = 0, C2 = 0;
+  int R = 0, NR = 0, NI = 0, NC = 0, NT = 0;
+  int tile = 128;          // reals per tile
+  int grid_mult = 2;       // persistent step grid = grid_mult * resident workgroups
+  int step_grid = 0;
+  int stage_cap = 0;       // particles per tile staging (dynamic LDS / 24 B)
+  size_t lds_limit = 65536;
+  int num_cu = 256;
+  std::vector<Rep> hrep;
+  std::vector<RepCtl> hctl;
+  std::vector<int> kcap;   // per replica list capacity
+  Rep *rep = nullptr;
+  RepCtl *ctl = nullptr;
+  int *meta = nullptr, *flagged = nullptr;
+  double4 *qe = nullptr, *qh0 = nullptr, *qh1 = nullptr, *mom = nullptr, *ref = nullptr,
+          *pos_tmp = nullptr, *mom_tmp = nullptr, *gpos_tmp = nullptr, *init_q = nullptr,
+          *init_p = nullptr;
+  int *tag = nullptr, *tag_tmp = nullptr, *gsrc = nullptr, *gcode = nullptr, *gsrc_tmp = nullptr,
+      *gcode_tmp = nullptr, *cellid = nullptr, *slot = nullptr, *nimg = nullptr, *goff = nullptr,
+      *order = nullptr, *inv = nullptr;
+  int *cell_count = nullptr, *cnt_in = nullptr, *cnt_out = nullptr, *scan_in = nullptr,
+      *scan_out = nullptr, *cell_start = nullptr;
+  TileDesc *tiles = nullptr;
+  unsigned short *list = nullptr;
+  size_t list_elems = 0;
+  int *nn = nullptr, *tile_kmax = nullptr;
+  double *partial = nullptr, *obs_partial = nullptr, *obs_out = nullptr, *io = nullptr;
+  bool have_state = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool graph_dirty = true;
+  hipEvent_t ev[2] = {};
+  double last_ms = 0.0;
+  long long last_steps = 0;
+  std::string err;
+};
+
+namespace {
+
+int efail(mdx_ensemble *e, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (e) e->err = buf;
+  else g_ens_create_error = buf;
+  return code;
+}
+
+#define EHIP(e, call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return efail((e), MDX_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));   \
+  } while (0)
+
+template <class T>
+int ealloc(mdx_ensemble *e, T **p, size_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (count == 0) count = 1;
+  EHIP(e, hipMalloc((void **)p, count * sizeof(T)));
+  return MDX_OK;
+}
+
+EnsDev dev_view(const mdx_ensemble *e) {
+  EnsDev d{};
+  d.rep = e->rep;
+  d.ctl = e->ctl;
+  d.meta = e->meta;
+  d.flagged = e->flagged;
+  d.R = e->R;
+  d.NR = e->NR;
+  d.NI = e->NI;
+  d.NT = e->NT;
+  d.T = e->tile;
+  d.stage_cap = e->stage_cap;
+  d.check_list_length = e->prm.check_list_length;
+  d.dt = e->prm.dt;
+  d.buffer = e->prm.buffer_length;
+  d.SL = e->SL;
+  d.C2_8 = 8.0 * e->C2;
+  d.qe = e->qe;
+  d.qh0 = e->qh0;
+  d.qh1 = e->qh1;
+  d.mom = e->mom;
+  d.ref = e->ref;
+  d.tag = e->tag;
+  d.gsrc = e->gsrc;
+  d.gcode = e->gcode;
+  d.pos_tmp = e->pos_tmp;
+  d.mom_tmp = e->mom_tmp;
+  d.gpos_tmp = e->gpos_tmp;
+  d.tag_tmp = e->tag_tmp;
+  d.gsrc_tmp = e->gsrc_tmp;
+  d.gcode_tmp = e->gcode_tmp;
+  d.cellid = e->cellid;
+  d.slot = e->slot;
+  d.nimg = e->nimg;
+  d.goff = e->goff;
+  d.order = e->order;
+  d.inv = e->inv;
+  d.cell_count = e->cell_count;
+  d.cnt_in = e->cnt_in;
+  d.cnt_out = e->cnt_out;
+  d.scan_in = e->scan_in;
+  d.scan_out = e->scan_out;
+  d.cell_start = e->cell_start;
+  d.tiles = e->tiles;
+  d.list = e->list;
+  d.nn = e->nn;
+  d.partial = e->partial;
+  d.obs_partial = e->obs_partial;
+  d.tile_kmax = e->tile_kmax;
+  return d;
+}
+
+size_t stage_bytes(const mdx_ensemble *e) { return (size_t)e->stage_cap * 3 * sizeof(double); }
+
+int launch_check(mdx_ensemble *e, const char *what) {
+  hipError_t er = hipGetLastError();
+  if (er != hipSuccess) return efail(e, MDX_ERR_HIP, "launch %s: %s", what, hipGetErrorString(er));
+  return MDX_OK;
+}
+
+// Tile slots and list regions for tile size T: tcap = rows + ceil(n / T)
+// bounds the tiles of any binning; lists hold kcap entries per real.
+int layout_tiles(mdx_ensemble *e, int T) {
+  size_t tb = 0, lb = 0;
+  for (int r = 0; r < e->R; r++) {
+    Rep &rp = e->hrep[r];
+    rp.tbase = (int)tb;
+    rp.tcap = rp.g.ms[1] * rp.g.ms[2] + (rp.n + T - 1) / T;
+    rp.kcap = e->kcap[r];
+    rp.lbase = lb;
+    tb += rp.tcap;
+    lb += (size_t)rp.tcap * rp.kcap * T;
+  }
+  if (tb > (size_t)0x7fffffff) return efail(e, MDX_ERR_CAPACITY, "too many tile slots");
+  e->NT = (int)tb;
+  e->tile = T;
+  int rc;
+  if ((rc = ealloc(e, &e->tiles, tb)) || (rc = ealloc(e, &e->partial, tb)) ||
+      (rc = ealloc(e, &e->obs_partial, 3 * tb)) || (rc = ealloc(e, &e->tile_kmax, tb)) ||
+      (rc = ealloc(e, &e->list, lb)))
+    return rc;
+  e->list_elems = lb;
+  EHIP(e, hipMemsetAsync(e->tiles, 0, tb * sizeof(TileDesc), e->stream));
+  EHIP(e, hipMemcpyAsync(e->rep, e->hrep.data(), sizeof(Rep) * e->R, hipMemcpyHostToDevice, e->stream));
+  EHIP(e, hipStreamSynchronize(e->stream));
+  e->graph_dirty = true;
+  return MDX_OK;
+}
+
+// Grow the list capacity of the replicas in `grow` (their rebuild failed
+// for it) and move every other replica's lists to the new layout.
+int regrow_lists(mdx_ensemble *e, const std::vector<int> &grow) {
+  std::vector<Rep> old = e->hrep;
+  for (int r : grow) e->kcap[r] = ((std::max(e->kcap[r], e->hctl[r].kmax) * 5 / 4 + 16) + 7) & ~7;
+  size_t lb = 0;
+  for (int r = 0; r < e->R; r++) {
+    e->hrep[r].kcap = e->kcap[r];
+    e->hrep[r].lbase = lb;
+    lb += (size_t)e->hrep[r].tcap * e->kcap[r] * e->tile;
+  }
+  unsigned short *nl = nullptr;
+  EHIP(e, hipMalloc((void **)&nl, std::max<size_t>(lb, 1) * sizeof(unsigned short)));
+  for (int r = 0; r < e->R; r++) {
+    if (old[r].kcap != e->hrep[r].kcap) continue;
+    const size_t len = (size_t)old[r].tcap * old[r].kcap * e->tile;
+    EHIP(e, hipMemcpyAsync(nl + e->hrep[r].lbase, e->list + old[r].lbase, len * sizeof(unsigned short),
+                           hipMemcpyDeviceToDevice, e->stream));
+  }
+  EHIP(e, hipStreamSynchronize(e->stream));
+  (void)hipFree(e->list);
+  e->list = nl;
+  e->list_elems = lb;
+  EHIP(e, hipMemcpyAsync(e->rep, e->hrep.data(), sizeof(Rep) * e->R, hipMemcpyHostToDevice, e->stream));
+  e->graph_dirty = true;
+  return MDX_OK;
+}
+
+int set_stage_cap(mdx_ensemble *e, int cap) {
+  const size_t need = (size_t)cap * 3 * sizeof(double) + 1024;
+  if (need > e->lds_limit)
+    return efail(e, MDX_ERR_CAPACITY, "tile staging of %d particles exceeds LDS (%zu B)", cap,
+                 e->lds_limit);
+  e->stage_cap = cap;
+  e->graph_dirty = true;
+  return MDX_OK;
+}
+
+template <int T>
+int enqueue_rebuild_t(mdx_ensemble *e) {
+  const EnsDev d = dev_view(e);
+  k_e_begin<<<e->R, kBlock, 0, e->stream>>>(d);
+  k_e_wrap<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
+  k_e_img_scan<<<e->R, kReplicaThreads, 0, e->stream>>>(d);
+  k_e_images<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
+  k_e_cells<<<e->R, kReplicaThreads, 0, e->stream>>>(d);
+  k_e_scatter<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
+  k_e_cell_sort<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
+  k_e_gather_reals<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
+  k_e_gather_images<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
+  k_e_build<T><<<e->step_grid, T, stage_bytes(e), e->stream>>>(d);
+  k_e_end<<<e->R, kBlock, 0, e->stream>>>(d);
+  return launch_check(e, "rebuild");
+}
+
+template <int T>
+int enqueue_step_t(mdx_ensemble *e) {
+  int rc = enqueue_rebuild_t<T>(e);
+  if (rc) return rc;
+  const EnsDev d = dev_view(e);
+  k_e_step<T><<<e->step_grid, T, stage_bytes(e), e->stream>>>(d);
+  k_e_guard<<<e->R, kReplicaThreads, 0, e->stream>>>(d);
+  return launch_check(e, "step");
+}
+
+int enqueue_rebuild(mdx_ensemble *e) {
+  return e->tile == 64 ? enqueue_rebuild_t<64>(e) : enqueue_rebuild_t<128>(e);
+}
+
+int enqueue_step(mdx_ensemble *e) {
+  return e->tile == 64 ? enqueue_step_t<64>(e) : enqueue_step_t<128>(e);
+}
+
+// resident workgroups of the step kernel times grid_mult, a multiple of 8
+int size_step_grid(mdx_ensemble *e) {
+  int per_cu = 0;
+  hipError_t er = e->tile == 64
+                      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_e_step<64>, 64, stage_bytes(e))
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_e_step<128>, 128, stage_bytes(e));
+  if (er != hipSuccess || per_cu < 1) per_cu = 1;
+  long long g = (long long)per_cu * e->num_cu * e->grid_mult;
+  g = std::min<long long>(g, ((long long)e->NT + 7) / 8 * 8);
+  g = std::max<long long>(8, (g + 7) / 8 * 8);
+  e->step_grid = (int)g;
+  return MDX_OK;
+}
+
+int read_ctl(mdx_ensemble *e) {
+  EHIP(e, hipMemcpyAsync(e->hctl.data(), e->ctl, sizeof(RepCtl) * e->R, hipMemcpyDeviceToHost, e->stream));
+  EHIP(e, hipStreamSynchronize(e->stream));
+  return MDX_OK;
+}
+
+int write_ctl(mdx_ensemble *e) {
+  EHIP(e, hipMemcpyAsync(e->ctl, e->hctl.data(), sizeof(RepCtl) * e->R, hipMemcpyHostToDevice, e->stream));
+  return MDX_OK;
+}
+
+// Replicas whose rebuild failed on a capacity that can grow: grow it and
+// clear the failure so the rebuild is retried.  Returns whether any was fixed.
+int fix_capacity(mdx_ensemble *e, bool *fixed) {
+  *fixed = false;
+  std::vector<int> grow;
+  int need_stage = 0;
+  for (int r = 0; r < e->R; r++) {
+    RepCtl &c = e->hctl[r];
+    if (c.error != MDX_ERR_CAPACITY) continue;
+    if (c.eflags & EF_LIST) grow.push_back(r);
+    if (c.eflags & EF_STAGE) need_stage = std::max(need_stage, c.stage_max);
+    if (c.eflags & EF_IMAGES) return efail(e, MDX_ERR_CAPACITY, "replica %d: image capacity", r);
+  }
+  int rc;
+  if (need_stage > 0) {
+    if ((rc = set_stage_cap(e, need_stage + need_stage / 8 + 64))) return rc;
+    if ((rc = size_step_grid(e))) return rc;
+  }
+  if (!grow.empty() && (rc = regrow_lists(e, grow))) return rc;
+  if (need_stage > 0 || !grow.empty()) {
+    for (int r = 0; r < e->R; r++) {
+      RepCtl &c = e->hctl[r];
+      if (c.error == MDX_ERR_CAPACITY && !(c.eflags & EF_IMAGES)) {
+        c.error = 0;
+        c.eflags = 0;
+        c.need_rebuild = 1;
+      }
+    }
+    if ((rc = write_ctl(e))) return rc;
+    *fixed = true;
+  }
+  return MDX_OK;
+}
+
+int capture(mdx_ensemble *e) {
+  if (!e->graph_dirty && e->exec) return MDX_OK;
+  if (e->exec) {
+    (void)hipGraphExecDestroy(e->exec);
+    e->exec = nullptr;
+  }
+  if (e->graph) {
+    (void)hipGraphDestroy(e->graph);
+    e->graph = nullptr;
+  }
+  EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_step(e);
+  hipGraph_t g = nullptr;
+  hipError_t er = hipStreamEndCapture(e->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (er != hipSuccess) return efail(e, MDX_ERR_HIP, "graph capture: %s", hipGetErrorString(er));
+  e->graph = g;
+  EHIP(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+  e->graph_dirty = false;
+  return MDX_OK;
+}
+
+// the loaded state again, every pairlist expired, then the first build
+int reset_state(mdx_ensemble *e) {
+  EHIP(e, hipMemcpyAsync(e->qe, e->init_q, sizeof(double4) * e->NR, hipMemcpyDeviceToDevice, e->stream));
+  EHIP(e, hipMemcpyAsync(e->mom, e->init_p, sizeof(double4) * e->NR, hipMemcpyDeviceToDevice, e->stream));
+  std::vector<int> ids(e->NR);
+  for (int i = 0; i < e->NR; i++) ids[i] = i;
+  EHIP(e, hipMemcpyAsync(e->tag, ids.data(), sizeof(int) * e->NR, hipMemcpyHostToDevice, e->stream));
+  for (int r = 0; r < e->R; r++) {
+    RepCtl c{};
+    c.budget = e->prm.buffer_length;
+    c.need_rebuild = 1;
+    c.target = 1;   // admits the build; no step runs here
+    e->hctl[r] = c;
+  }
+  int rc = write_ctl(e);
+  if (rc) return rc;
+  EHIP(e, hipMemsetAsync(e->meta, 0, sizeof(int) * 4, e->stream));
+  // the first step's pairlist construction, retried while a capacity grows
+  for (int attempt = 0; attempt < 4; attempt++) {
+    if ((rc = enqueue_rebuild(e))) return rc;
+    if ((rc = read_ctl(e))) return rc;
+    bool fixed = false;
+    if ((rc = fix_capacity(e, &fixed))) return rc;
+    if (!fixed) break;
+  }
+  int smax = 0;
+  for (int r = 0; r < e->R; r++) {
+    RepCtl &c = e->hctl[r];
+    if (c.error) return efail(e, c.error, "replica %d: first pairlist build failed (flags %d)", r, c.eflags);
+    c.target = 0;
+    smax = std::max(smax, c.stage_max);
+  }
+  if ((rc = write_ctl(e))) return rc;
+  // LDS per tile sized to the tiles as built, with room to grow
+  const int cap = smax + smax / 8 + 64;
+  if (cap < e->stage_cap) {
+    if ((rc = set_stage_cap(e, cap)) || (rc = size_step_grid(e))) return rc;
+  }
+  return MDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdx_ensemble_create(const mdx_params *prm, int nrep, const int *n, const double *L3,
+                        mdx_ensemble **out) {
+  if (!prm || !out || !n || !L3 || nrep < 1) return efail(nullptr, MDX_ERR_ARG, "bad arguments");
+  *out = nullptr;
+  const double SL = prm->cutoff + prm->buffer_length;
+  if (!(prm->dt > 0.0) || !(prm->cutoff > 0.0) || !(prm->buffer_length > 0.0))
+    return efail(nullptr, MDX_ERR_ARG, "dt, cutoff and buffer_length must be positive");
+  if (prm->check_list_length < 1 || prm->check_list_length > kMaxCand)
+    return efail(nullptr, MDX_ERR_ARG, "check_list_length must be in [1, %d]", kMaxCand);
+  long long NR = 0, NI = 0;
+  for (int r = 0; r < nrep; r++) {
+    if (n[r] < 1) return efail(nullptr, MDX_ERR_ARG, "replica %d has no particles", r);
+    for (int a = 0; a < 3; a++)
+      if (!(L3[3 * r + a] >= 2.0 * SL))
+        return efail(nullptr, MDX_ERR_BOX, "replica %d: box edge %g < 2 x search length %g", r,
+                     L3[3 * r + a], SL);
+    NR += n[r];
+    NI += (long long)kMaxImages * n[r];
+  }
+  if (NR + NI > 0x7fffffffLL) return efail(nullptr, MDX_ERR_CAPACITY, "too many particles");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= prm->device)
+    return efail(nullptr, MDX_ERR_HIP, "HIP device %d not available", prm->device);
+  if (hipSetDevice(prm->device) != hipSuccess)
+    return efail(nullptr, MDX_ERR_HIP, "hipSetDevice(%d) failed", prm->device);
+  mdx_ensemble *e = new mdx_ensemble();
+  e->prm = *prm;
+  e->SL = SL;
+  {
+    const double s2 = 1.0 / (prm->cutoff * prm->cutoff);
+    const double s6 = s2 * s2 * s2, s8 = s6 * s2, s12 = s6 * s6, s14 = s12 * s2;
+    e->C2 = 6.0 * s14 - 3.0 * s8;
+    e->C0 = -s12 + s6 - e->C2 / s2;
+  }
+  e->R = nrep;
+  e->NR = (int)NR;
+  e->NI = (int)NI;
+  e->hrep.resize(nrep);
+  e->hctl.resize(nrep);
+  e->kcap.resize(nrep);
+  int rb = 0, ib = 0, cb = 0;
+  for (int r = 0; r < nrep; r++) {
+    Rep &rp = e->hrep[r];
+    rp = Rep{};
+    double V = 1.0;
+    for (int a = 0; a < 3; a++) {
+      const double L = L3[3 * r + a];
+      rp.g.L[a] = L;
+      rp.g.ms[a] = (int)(L / SL);   // MeshList::ChangeScale (meshlist.cc:41-65)
+      rp.g.inv_w[a] = 1.0 / (L / (double)rp.g.ms[a]);
+      rp.g.m[a] = rp.g.ms[a] + 2;
+      V *= L;
+    }
+    rp.g.SL2 = SL * SL;
+    rp.g.CL2 = prm->cutoff * prm->cutoff;
+    rp.n = n[r];
+    rp.rbase = rb;
+    rp.ibase = ib;
+    rp.gcap = kMaxImages * n[r];
+    rp.cbase = cb;
+    rp.ncell = rp.g.m[0] * rp.g.m[1] * rp.g.m[2];
+    rb += n[r];
+    ib += rp.gcap;
+    cb += rp.ncell;
+    // partners within SL at the mean density, with room for fluctuations
+    const double mean = 4.18879 * SL * SL * SL * (n[r] / V);
+    e->kcap[r] = ((int)(mean * 1.4) + 24 + 7) & ~7;
+  }
+  e->NC = cb;
+  int rc = MDX_OK;
+  if (prm->stream) {
+    e->stream = (hipStream_t)prm->stream;
+  } else {
+    if (hipStreamCreate(&e->stream) != hipSuccess) rc = efail(e, MDX_ERR_HIP, "hipStreamCreate failed");
+    e->own_stream = true;
+  }
+  hipDeviceProp_t prop;
+  if (!rc && hipGetDeviceProperties(&prop, prm->device) == hipSuccess) {
+    e->lds_limit = prop.sharedMemPerBlock;
+    e->num_cu = prop.multiProcessorCount;
+  }
+  for (auto &ev : e->ev)
+    if (!rc && hipEventCreate(&ev) != hipSuccess) rc = efail(e, MDX_ERR_HIP, "hipEventCreate failed");
+  const size_t tot = (size_t)NR + NI;
+  if (!rc && ((rc = ealloc(e, &e->rep, nrep)) || (rc = ealloc(e, &e->ctl, nrep)) ||
+              (rc = ealloc(e, &e->meta, 4)) || (rc = ealloc(e, &e->flagged, nrep)) ||
+              (rc = ealloc(e, &e->qe, tot)) || (rc = ealloc(e, &e->ref, tot)) ||
+              (rc = ealloc(e, &e->qh0, NR)) || (rc = ealloc(e, &e->qh1, NR)) ||
+              (rc = ealloc(e, &e->mom, NR)) || (rc = ealloc(e, &e->pos_tmp, NR)) ||
+              (rc = ealloc(e, &e->mom_tmp, NR)) || (rc = ealloc(e, &e->gpos_tmp, NI)) ||
+              (rc = ealloc(e, &e->init_q, NR)) || (rc = ealloc(e, &e->init_p, NR)) ||
+              (rc = ealloc(e, &e->tag, NR)) || (rc = ealloc(e, &e->tag_tmp, NR)) ||
+              (rc = ealloc(e, &e->gsrc, NI)) || (rc = ealloc(e, &e->gcode, NI)) ||
+              (rc = ealloc(e, &e->gsrc_tmp, NI)) || (rc = ealloc(e, &e->gcode_tmp, NI)) ||
+              (rc = ealloc(e, &e->cellid, tot)) || (rc = ealloc(e, &e->slot, tot)) ||
+              (rc = ealloc(e, &e->order, tot)) || (rc = ealloc(e, &e->nimg, NR)) ||
+              (rc = ealloc(e, &e->goff, NR)) || (rc = ealloc(e, &e->inv, NR)) ||
+              (rc = ealloc(e, &e->nn, NR)) || (rc = ealloc(e, &e->cell_count, cb)) ||
+              (rc = ealloc(e, &e->cnt_in, cb)) || (rc = ealloc(e, &e->cnt_out, cb)) ||
+              (rc = ealloc(e, &e->scan_in, cb)) || (rc = ealloc(e, &e->scan_out, cb)) ||
+              (rc = ealloc(e, &e->cell_start, cb)) || (rc = ealloc(e, &e->obs_out, 3 * (size_t)nrep)) ||
+              (rc = ealloc(e, &e->io, 3 * (size_t)NR))))
+    ;
+  if (!rc && hipMemsetAsync(e->meta, 0, 4 * sizeof(int), e->stream) != hipSuccess)
+    rc = efail(e, MDX_ERR_HIP, "hipMemsetAsync failed");
+  if (!rc) rc = layout_tiles(e, 128);
+  // staging of the densest replica: its 9 cell rows by 4 cells, doubled
+  if (!rc) {
+    double dmax = 0.0;
+    for (const Rep &rp : e->hrep) {
+      const double cellv = (rp.g.L[0] / rp.g.ms[0]) * (rp.g.L[1] / rp.g.ms[1]) * (rp.g.L[2] / rp.g.ms[2]);
+      dmax = std::max(dmax, rp.n / (rp.g.L[0] * rp.g.L[1] * rp.g.L[2]) * cellv);
+    }
+    const int want = (int)(dmax * 36.0 * 1.25) + 256;
+    const int lim = (int)((e->lds_limit - 1024) / 24);
+    rc = set_stage_cap(e, std::min(want, lim));
+  }
+  if (!rc) rc = size_step_grid(e);
+  if (rc) {
+    g_ens_create_error = e->err.empty() ? "allocation failed" : e->err;
+    mdx_ensemble_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return MDX_OK;
+}
+
+int mdx_ensemble_destroy(mdx_ensemble *e) {
+  if (!e) return MDX_OK;
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  void *bufs[] = {e->rep, e->ctl, e->meta, e->flagged, e->qe, e->qh0, e->qh1, e->mom, e->ref,
+                  e->pos_tmp, e->mom_tmp, e->gpos_tmp, e->init_q, e->init_p, e->tag, e->tag_tmp,
+                  e->gsrc, e->gcode, e->gsrc_tmp, e->gcode_tmp, e->cellid, e->slot, e->nimg, e->goff,
+                  e->order, e->inv, e->cell_count, e->cnt_in, e->cnt_out, e->scan_in, e->scan_out,
+                  e->cell_start, e->tiles, e->list, e->nn, e->tile_kmax, e->partial, e->obs_partial,
+                  e->obs_out, e->io};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  for (auto &ev : e->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return MDX_OK;
+}
+
+const char *mdx_ensemble_last_error(const mdx_ensemble *e) {
+  return e ? e->err.c_str() : g_ens_create_error.c_str();
+}
+
+int mdx_ensemble_size(const mdx_ensemble *e, int *nrep, int *ntotal) {
+  if (!e) return MDX_ERR_ARG;
+  if (nrep) *nrep = e->R;
+  if (ntotal) *ntotal = e->NR;
+  return MDX_OK;
+}
+
+int mdx_ensemble_set_state(mdx_ensemble *e, const double *q3, const double *p3) {
+  if (!e || !q3 || !p3) return efail(e, MDX_ERR_ARG, "bad arguments to mdx_ensemble_set_state");
+  double *pbuf = reinterpret_cast<double *>(e->pos_tmp);   // >= 3 NR doubles
+  EHIP(e, hipMemcpyAsync(e->io, q3, sizeof(double) * 3 * (size_t)e->NR, hipMemcpyHostToDevice, e->stream));
+  EHIP(e, hipMemcpyAsync(pbuf, p3, sizeof(double) * 3 * (size_t)e->NR, hipMemcpyHostToDevice, e->stream));
+  k_e_load<<<nblk(e->NR), kBlock, 0, e->stream>>>(e->io, pbuf, e->NR, e->init_q, e->init_p, e->tag);
+  int rc = launch_check(e, "load");
+  if (rc) return rc;
+  e->have_state = true;
+  return reset_state(e);
+}
+
+int mdx_ensemble_get_state(mdx_ensemble *e, double *q3, double *p3) {
+  if (!e || !e->have_state) return efail(e, MDX_ERR_STATE, "no state");
+  if (q3) {
+    k_e_scatter_by_tag<<<nblk(e->NR), kBlock, 0, e->stream>>>(e->qe, e->tag, e->NR, e->io);
+    EHIP(e, hipMemcpyAsync(q3, e->io, sizeof(double) * 3 * (size_t)e->NR, hipMemcpyDeviceToHost, e->stream));
+    EHIP(e, hipStreamSynchronize(e->stream));
+  }
+  if (p3) {
+    k_e_scatter_by_tag<<<nblk(e->NR), kBlock, 0, e->stream>>>(e->mom, e->tag, e->NR, e->io);
+    EHIP(e, hipMemcpyAsync(p3, e->io, sizeof(double) * 3 * (size_t)e->NR, hipMemcpyDeviceToHost, e->stream));
+    EHIP(e, hipStreamSynchronize(e->stream));
+  }
+  return launch_check(e, "get_state");
+}
+
+int mdx_ensemble_calculate(mdx_ensemble *e, int nsteps, int *rebuilds) {
+  if (!e || nsteps < 0) return efail(e, MDX_ERR_ARG, "bad arguments to mdx_ensemble_calculate");
+  if (!e->have_state) return efail(e, MDX_ERR_STATE, "no state");
+  int rc;
+  if ((rc = read_ctl(e))) return rc;
+  for (auto &c : e->hctl)
+    if (!c.error) c.target = c.steps + nsteps;
+  if ((rc = write_ctl(e))) return rc;
+  if ((rc = capture(e))) return rc;
+  EHIP(e, hipEventRecord(e->ev[0], e->stream));
+  int launches = nsteps;
+  for (int round = 0; round < 8 && launches > 0; round++) {
+    for (int k = 0; k < launches; k++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
+    if ((rc = read_ctl(e))) return rc;
+    bool fixed = false;
+    if ((rc = fix_capacity(e, &fixed))) return rc;
+    if ((rc = capture(e))) return rc;
+    launches = 0;
+    for (const auto &c : e->hctl)
+      if (!c.error) launches = std::max(launches, c.target - c.steps);
+  }
+  EHIP(e, hipEventRecord(e->ev[1], e->stream));
+  EHIP(e, hipEventSynchronize(e->ev[1]));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e->ev[0], e->ev[1]);
+  e->last_ms = ms;
+  e->last_steps = nsteps;
+  int worst = MDX_OK, first_bad = -1;
+  for (int r = 0; r < e->R; r++) {
+    const RepCtl &c = e->hctl[r];
+    if (rebuilds) rebuilds[r] = c.rebuilds;
+    if (c.error && first_bad < 0) {
+      first_bad = r;
+      worst = c.error;
+    }
+  }
+  if (first_bad >= 0) {
+    const RepCtl &c = e->hctl[first_bad];
+    if (c.error == MDX_ERR_TOO_FAST)
+      return efail(e, c.error, "replica %d: Too fast particles exists. max_velocity = %f", first_bad,
+                   std::sqrt(c.vmax2));
+    return efail(e, c.error, "replica %d stopped (status %d, flags %d)", first_bad, c.error, c.eflags);
+  }
+  return worst;
+}
+
+int mdx_ensemble_status(mdx_ensemble *e, int *status, int *steps) {
+  if (!e) return MDX_ERR_ARG;
+  int rc = read_ctl(e);
+  if (rc) return rc;
+  for (int r = 0; r < e->R; r++) {
+    if (status) status[r] = e->hctl[r].error;
+    if (steps) steps[r] = e->hctl[r].steps;
+  }
+  return MDX_OK;
+}
+
+int mdx_ensemble_observe(mdx_ensemble *e, double *ke, double *pe, double *vir) {
+  if (!e || !e->have_state) return efail(e, MDX_ERR_STATE, "no state");
+  const EnsDev d = dev_view(e);
+  if (e->tile == 64)
+    k_e_observe<64><<<e->NT, 64, stage_bytes(e), e->stream>>>(d, e->C0, e->C2);
+  else
+    k_e_observe<128><<<e->NT, 128, stage_bytes(e), e->stream>>>(d, e->C0, e->C2);
+  k_e_observe_fold<<<e->R, kBlock, 0, e->stream>>>(d, e->obs_out);
+  int rc = launch_check(e, "observe");
+  if (rc) return rc;
+  std::vector<double> h(3 * (size_t)e->R);
+  EHIP(e, hipMemcpyAsync(h.data(), e->obs_out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, e->stream));
+  EHIP(e, hipStreamSynchronize(e->stream));
+  for (int r = 0; r < e->R; r++) {
+    if (ke) ke[r] = h[3 * r];
+    if (pe) pe[r] = h[3 * r + 1];
+    if (vir) vir[r] = h[3 * r + 2] / 3.0;
+  }
+  return MDX_OK;
+}
+
+int mdx_ensemble_get_config(const mdx_ensemble *e, mdx_ensemble_config *cfg) {
+  if (!e || !cfg) return MDX_ERR_ARG;
+  cfg->tile = e->tile;
+  cfg->grid_mult = e->grid_mult;
+  cfg->step_grid = e->step_grid;
+  cfg->stage_cap = e->stage_cap;
+  cfg->ms_per_step = e->last_steps > 0 ? e->last_ms / e->last_steps : 0.0;
+  return MDX_OK;
+}
+
+// Tile size and persistent grid.  Changing the tile size re-tiles every
+// replica, so it is accepted only before the first step after
+// mdx_ensemble_set_state, and rebuilds from the loaded state.
+int mdx_ensemble_set_config(mdx_ensemble *e, int tile, int grid_mult) {
+  if (!e || (tile != 64 && tile != 128) || grid_mult < 1 || grid_mult > 16)
+    return efail(e, MDX_ERR_ARG, "tile must be 64 or 128, grid_mult in [1, 16]");
+  int rc;
+  const bool retile = tile != e->tile;
+  if (retile && e->have_state) {
+    if ((rc = read_ctl(e))) return rc;
+    for (const auto &c : e->hctl)
+      if (c.steps > 0) return efail(e, MDX_ERR_STATE, "tile size changes only before the first step");
+  }
+  if (retile && (rc = layout_tiles(e, tile))) return rc;
+  e->grid_mult = grid_mult;
+  if ((rc = size_step_grid(e))) return rc;
+  e->graph_dirty = true;
+  if (retile && e->have_state && (rc = reset_state(e))) return rc;
+  return MDX_OK;
+}
+
+// Times trial_steps steps for every tile size and grid multiple, each from
+// the loaded state, keeps the fastest, and leaves the loaded state (with its
+// first pairlist) in place.  Results do not depend on the choice: the tile
+// size and the grid change which workgroup computes a real, not its
+// arithmetic or the order of its partners.
+int mdx_ensemble_autotune(mdx_ensemble *e, int trial_steps, mdx_ensemble_config *best) {
+  if (!e || trial_steps < 1) return efail(e, MDX_ERR_ARG, "bad arguments to mdx_ensemble_autotune");
+  if (!e->have_state) return efail(e, MDX_ERR_STATE, "autotune needs a loaded state");
+  int rc;
+  if ((rc = read_ctl(e))) return rc;
+  for (const auto &c : e->hctl)
+    if (c.steps > 0) return efail(e, MDX_ERR_STATE, "autotune runs only before the first step");
+  const int tiles[2] = {64, 128};
+  const int mults[3] = {1, 2, 4};
+  double best_ms = 1e300;
+  int bt = e->tile, bm = e->grid_mult;
+  for (int T : tiles)
+    for (int gm : mults) {
+      if (T != e->tile && (rc = layout_tiles(e, T))) return rc;
+      e->grid_mult = gm;
+      if ((rc = size_step_grid(e)) || (rc = reset_state(e))) return rc;
+      e->graph_dirty = true;
+      if ((rc = mdx_ensemble_calculate(e, trial_steps, nullptr))) return rc;
+      const double ms = e->last_ms / trial_steps;
+      if (ms < best_ms) {
+        best_ms = ms;
+        bt = T;
+        bm = gm;
+      }
+    }
+  if (bt != e->tile && (rc = layout_tiles(e, bt))) return rc;
+  e->grid_mult = bm;
+  if ((rc = size_step_grid(e)) || (rc = reset_state(e))) return rc;
+  e->graph_dirty = true;
+  if (best) {
+    mdx_ensemble_get_config(e, best);
+    best->ms_per_step = best_ms;
+  }
+  return MDX_OK;
+}
+
+}  // extern "C"

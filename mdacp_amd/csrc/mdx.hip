@@ -31,13 +31,13 @@
 #include <vector>
 
 #include "../../include/mdx.h"
+#include "mdx_common.h"
 
 namespace {
 
-constexpr int kBlock = 256;
+using namespace mdxk;
 constexpr int kScanItems = 4;                       // items per thread in a scan block
 constexpr int kScanTile = kBlock * kScanItems;      // 1024 elements per scan block
-constexpr int kCodeNone = 13;                       // image code of "no shift"
 
 enum StatusBits : int {
   ST_INVALID_POS = 1,
@@ -62,15 +62,6 @@ struct DevStatus {
   double sums[4];                  // KE, PE, virial, spare
 };
 
-// |p|^2 rounded as written (pairlist.cc:52), identical in every kernel
-__device__ __forceinline__ double speed2(const double4 &p) {
-  return __dadd_rn(__dadd_rn(__dmul_rn(p.x, p.x), __dmul_rn(p.y, p.y)), __dmul_rn(p.z, p.z));
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-  return v;
-}
 
 // Block maximum of a non-negative value written to partial[blockIdx.x];
 // every thread of the block must call it.  A one-block kernel folds the
@@ -145,15 +136,6 @@ __global__ void k_step_guard(const double *__restrict__ partial, int nb, StepCtl
   }
 }
 
-// 1/x from the hardware estimate and two Newton steps (each squares the
-// relative error); within an ulp or two of the IEEE quotient.
-__device__ __forceinline__ double recip(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = __fma_rn(-x, r, 1.0);
-  r = __fma_rn(r, e, r);
-  e = __fma_rn(-x, r, 1.0);
-  return __fma_rn(r, e, r);
-}
 
 // ---------------------------------------------------------------- scans
 // Exclusive scan of an int array, three passes, each block staging a
@@ -214,22 +196,6 @@ __global__ void k_scan_add(int *__restrict__ out, const int *__restrict__ tile_s
 }
 
 // ---------------------------------------------------------------- geometry
-struct Geom {
-  double L[3];
-  double inv_w[3];
-  int ms[3];   // interior cells per axis
-  int m[3];    // with the image layer on both sides
-  double SL2, CL2;
-};
-
-__device__ __forceinline__ int cell_linear(const Geom &g, int ix, int iy, int iz) {
-  return ix + g.m[0] * (iy + g.m[1] * iz);
-}
-
-__device__ __forceinline__ int interior_coord(double x, double inv_w, int ms) {
-  int c = (int)floor(x * inv_w) + 1;
-  return c < 1 ? 1 : (c > ms ? ms : c);
-}
 
 // ------------------------------------------------------------ rebuild
 // Wrap reals into the box exactly as Variables::AdjustPeriodicBoundary
@@ -263,10 +229,6 @@ __global__ void k_wrap_bin_count(double4 *__restrict__ pos, int n, Geom g, doubl
   nimg[i] = cnt - 1;
 }
 
-__device__ __forceinline__ double shifted(double x, int s, double L) {
-  // identical rounding to the reference's image q + diff (mdunit.cc:183)
-  return s == 0 ? x : (s > 0 ? x + L : x + (-L));
-}
 
 // Images of each real, in a fixed (sz, sy, sx) enumeration; their cells are
 // fixed by the shift (outer layer) so no image can land in an interior cell.
@@ -398,21 +360,11 @@ __global__ void k_gather_images(const int *__restrict__ order, int n, int g,
 // (interior cells, slots scan_in) and one of images (outer cells, slots
 // n + scan_out): 18 segments, staged once into LDS as x[], y[], z[].
 // Neighbour lists hold 16-bit offsets into that staging.
-constexpr int kTile = 128;
-constexpr int kSeg = 18;
-constexpr int kILP = 8;   // independent partner chains per lane in the kick
-
-struct TileDesc {
-  int i0, ni, y, z, xf, xl, stage, pad;
-};
 
 struct Cells {
   const int *scan_in, *cnt_in, *scan_out, *cnt_out;
 };
 
-__device__ __forceinline__ bool is_outer(const Geom &g, int x, int y, int z) {
-  return x == 0 || y == 0 || z == 0 || x == g.m[0] - 1 || y == g.m[1] - 1 || z == g.m[2] - 1;
-}
 
 // [start, end) of segment s (0..17) of a tile, as global slots
 __device__ __forceinline__ void tile_segment(const Geom &g, const Cells &c, int n, int y, int z,
@@ -437,10 +389,6 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles) {
 }
 __host__ __device__ inline int tile_grid(int ntiles) { return ((ntiles + 7) >> 3) << 3; }
 
-struct TileSmem {
-  int seg_start[kSeg];
-  int seg_base[kSeg + 1];
-};
 
 // segments and staging; every thread of the block calls it
 __device__ void tile_stage(const TileDesc &t, const Geom &g, const Cells &c, int n,
@@ -557,8 +505,7 @@ __global__ void __launch_bounds__(kTile) k_build_tiles(
         }
         for (int o = lo; o < lo + len; o++) {
           if (o == self) continue;
-          const double ddx = sx[o] - qx, ddy = sy[o] - qy, ddz = sz[o] - qz;
-          const double r2 = __fma_rn(ddx, ddx, __fma_rn(ddy, ddy, __dmul_rn(ddz, ddz)));
+          const double r2 = list_r2(sx[o], sy[o], sz[o], qx, qy, qz);
           if (r2 <= g.SL2) {
             if (cnt < kcap) out[(size_t)cnt * kTile] = (unsigned short)o;
             cnt++;
@@ -578,12 +525,7 @@ __global__ void k_drift(double4 *__restrict__ pos, const double4 *__restrict__ m
                         double dt2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double4 q = pos[i];
-  const double4 p = mom[i];
-  q.x = __dadd_rn(q.x, __dmul_rn(p.x, dt2));
-  q.y = __dadd_rn(q.y, __dmul_rn(p.y, dt2));
-  q.z = __dadd_rn(q.z, __dmul_rn(p.z, dt2));
-  pos[i] = q;
+  pos[i] = drift_half(pos[i], mom[i], dt2);
 }
 
 // image refresh (MDUnit::MakeBufferForBorderParticles, mdunit.cc:181-191)
@@ -592,14 +534,8 @@ __global__ void k_refresh_images(double4 *__restrict__ pos, const int *__restric
                                  double Lz) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= g) return;
-  const double4 q = pos[gsrc[k]];
-  const int c = gcode[k];
-  double4 r;
-  r.x = shifted(q.x, c % 3 - 1, Lx);
-  r.y = shifted(q.y, (c / 3) % 3 - 1, Ly);
-  r.z = shifted(q.z, c / 9 - 1, Lz);
-  r.w = 0.0;
-  pos[n + k] = r;
+  const double L[3] = {Lx, Ly, Lz};
+  pos[n + k] = image_of(pos[gsrc[k]], gcode[k], L);
 }
 
 // Drift by half a step fused with the image refresh, out of place:
@@ -616,30 +552,16 @@ __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__rest
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n + g) return;
   const int src = i < n ? i : gsrc[i - n];
-  double4 q = pos[src];
-  const double4 p = mom[src];
-  q.x = __dadd_rn(q.x, __dmul_rn(p.x, dt2));
-  q.y = __dadd_rn(q.y, __dmul_rn(p.y, dt2));
-  q.z = __dadd_rn(q.z, __dmul_rn(p.z, dt2));
+  double4 q = drift_half(pos[src], mom[src], dt2);
   if (i >= n) {
-    const int c = gcode[i - n];
-    q.x = shifted(q.x, c % 3 - 1, Lx);
-    q.y = shifted(q.y, (c / 3) % 3 - 1, Ly);
-    q.z = shifted(q.z, c / 9 - 1, Lz);
+    const double L[3] = {Lx, Ly, Lz};
+    q = image_of(q, gcode[i - n], L);
   }
   q.w = 0.0;
   out[i] = q;
   if (i >= n) pos[i] = q;
 }
 
-// Pair term of ForceCalculator::CalculateForceSorted (fcalculator.cc:203-217):
-// df = ((24 r^6 - 48) / (r^6 r^6 r^2) + 8 C2) dt, zero beyond the cutoff;
-// the quotient uses recip().
-__device__ __forceinline__ double pair_df(double r2, double C2_8, double dt, double CL2) {
-  const double r6 = r2 * r2 * r2;
-  const double df = ((24.0 * r6 - 48.0) * recip(r6 * r6 * r2) + C2_8) * dt;
-  return r2 > CL2 ? 0.0 : df;
-}
 
 // Parameters of the differentiable form V(r) = eps v(r / sigma), v the
 // reference's shifted LJ cut at r / sigma = 3 (so the cutoff is 3 sigma).
@@ -731,22 +653,28 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
 #pragma unroll
       for (int u = 0; u < kILP; u++) {
         const int o = cur[u];
-        const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
-        const double s2 = dx * dx + dy * dy + dz * dz;
-        const double df = SCALED ? pair_df_lj(s2, lj, dt) : pair_df(s2, C2_8, dt, g.CL2);
-        ax[u] += df * dx;
-        ay[u] += df * dy;
-        az[u] += df * dz;
+        if (SCALED) {
+          const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
+          const double df = pair_df_lj(dx * dx + dy * dy + dz * dz, lj, dt);
+          ax[u] += df * dx;
+          ay[u] += df * dy;
+          az[u] += df * dz;
+        } else {
+          kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, C2_8, dt, g.CL2, ax[u], ay[u], az[u]);
+        }
       }
     }
     for (int k = mg; k < m; k++) {
       const int o = lst[(size_t)k * kTile];
-      const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
-      const double s2 = dx * dx + dy * dy + dz * dz;
-      const double df = SCALED ? pair_df_lj(s2, lj, dt) : pair_df(s2, C2_8, dt, g.CL2);
-      ax[0] += df * dx;
-      ay[0] += df * dy;
-      az[0] += df * dz;
+      if (SCALED) {
+        const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
+        const double df = pair_df_lj(dx * dx + dy * dy + dz * dz, lj, dt);
+        ax[0] += df * dx;
+        ay[0] += df * dy;
+        az[0] += df * dz;
+      } else {
+        kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, C2_8, dt, g.CL2, ax[0], ay[0], az[0]);
+      }
     }
     double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll
@@ -764,14 +692,7 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
       p.z += fz;
       mom[i] = p;
       v2 = speed2(p);
-      if (MODE == 2) {
-        const double dt2 = dt * 0.5;
-        double4 q = qi;
-        q.x = __dadd_rn(q.x, __dmul_rn(p.x, dt2));
-        q.y = __dadd_rn(q.y, __dmul_rn(p.y, dt2));
-        q.z = __dadd_rn(q.z, __dmul_rn(p.z, dt2));
-        out[i] = q;
-      }
+      if (MODE == 2) out[i] = drift_half(qi, p, dt * 0.5);
     }
   }
   if (MODE != 1) block_max_to(v2, partial);
@@ -1035,9 +956,7 @@ __global__ void __launch_bounds__(kTile) k_observe_tiles(
       const double dx = qi.x - sx[o], dy = qi.y - sy[o], dz = qi.z - sz[o];
       const double r2 = dx * dx + dy * dy + dz * dz;
       if (r2 > g.CL2) continue;
-      const double r6 = r2 * r2 * r2;
-      pe += 4.0 * (1.0 / (r2 * r2 * r2 * r2 * r2 * r2) - 1.0 / r6 + C2 * r2 + C0);
-      vir += ((24.0 * r6 - 48.0) / (r6 * r6 * r2) + C2 * 8.0) * r2;
+      observe_term(r2, C0, C2, pe, vir);
     }
     pe *= 0.5;
     vir *= 0.5;
@@ -1089,9 +1008,7 @@ __global__ void k_disp(const double4 *__restrict__ pos, const double4 *__restric
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
   if (i < tn) {
-    const double4 a = ref[i], b = pos[i];
-    const double dx = __dadd_rn(a.x, -b.x), dy = __dadd_rn(a.y, -b.y), dz = __dadd_rn(a.z, -b.z);
-    d2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+    d2 = disp2(ref[i], pos[i]);
     disp[i] = d2;
   }
   block_max_to(d2, partial);
@@ -1113,16 +1030,8 @@ __global__ void k_candidate_pairs(const double4 *__restrict__ pos, const double4
   const int a = (int)(t / l), b = (int)(t % l);
   if (b <= a) return;
   const int i1 = cand[a], i2 = cand[b];
-  const double4 o1 = ref[i1], o2 = ref[i2];
-  double dx = __dadd_rn(o1.x, -o2.x), dy = __dadd_rn(o1.y, -o2.y), dz = __dadd_rn(o1.z, -o2.z);
-  double dr2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-  if (dr2 < SL2) return;
-  const double4 n1 = pos[i1], n2 = pos[i2];
-  dx = __dadd_rn(n1.x, -n2.x);
-  dy = __dadd_rn(n1.y, -n2.y);
-  dz = __dadd_rn(n1.z, -n2.z);
-  dr2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-  if (dr2 < CL2) atomicOr(hit, 1);
+  if (disp2(ref[i1], ref[i2]) < SL2) return;
+  if (disp2(pos[i1], pos[i2]) < CL2) atomicOr(hit, 1);
 }
 
 // ------------------------------------------------------------ export

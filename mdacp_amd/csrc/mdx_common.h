@@ -1,0 +1,146 @@
+// mdx_common.h — device definitions shared by the single-system step
+// (mdx.hip) and the replica ensemble (ensemble.hip): constants, cell
+// geometry, the pair term, tile descriptors and wave reductions.
+#ifndef MDX_COMMON_H
+#define MDX_COMMON_H
+
+#include <hip/hip_runtime.h>
+
+namespace mdxk {
+
+constexpr int kBlock = 256;
+
+constexpr int kCodeNone = 13;                       // image code of "no shift"
+
+// |p|^2 rounded as written (pairlist.cc:52), identical in every kernel
+__device__ __forceinline__ double speed2(const double4 &p) {
+  return __dadd_rn(__dadd_rn(__dmul_rn(p.x, p.x), __dmul_rn(p.y, p.y)), __dmul_rn(p.z, p.z));
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  return v;
+}
+
+// 1/x from the hardware estimate and two Newton steps (each squares the
+// relative error); within an ulp or two of the IEEE quotient.
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __fma_rn(-x, r, 1.0);
+  r = __fma_rn(r, e, r);
+  e = __fma_rn(-x, r, 1.0);
+  return __fma_rn(r, e, r);
+}
+
+struct Geom {
+  double L[3];
+  double inv_w[3];
+  int ms[3];   // interior cells per axis
+  int m[3];    // with the image layer on both sides
+  double SL2, CL2;
+};
+
+__device__ __forceinline__ int cell_linear(const Geom &g, int ix, int iy, int iz) {
+  return ix + g.m[0] * (iy + g.m[1] * iz);
+}
+
+__device__ __forceinline__ int interior_coord(double x, double inv_w, int ms) {
+  int c = (int)floor(x * inv_w) + 1;
+  return c < 1 ? 1 : (c > ms ? ms : c);
+}
+
+__device__ __forceinline__ double shifted(double x, int s, double L) {
+  // identical rounding to the reference's image q + diff (mdunit.cc:183)
+  return s == 0 ? x : (s > 0 ? x + L : x + (-L));
+}
+
+constexpr int kTile = 128;
+constexpr int kSeg = 18;
+constexpr int kILP = 8;   // independent partner chains per lane in the kick
+
+struct TileDesc {
+  int i0, ni, y, z, xf, xl, stage, pad;
+};
+
+__device__ __forceinline__ bool is_outer(const Geom &g, int x, int y, int z) {
+  return x == 0 || y == 0 || z == 0 || x == g.m[0] - 1 || y == g.m[1] - 1 || z == g.m[2] - 1;
+}
+
+struct TileSmem {
+  int seg_start[kSeg];
+  int seg_base[kSeg + 1];
+};
+
+// Pair term of ForceCalculator::CalculateForceSorted (fcalculator.cc:203-217):
+// df = ((24 r^6 - 48) / (r^6 r^6 r^2) + 8 C2) dt, zero beyond the cutoff;
+// the quotient uses recip().
+__device__ __forceinline__ double pair_df(double r2, double C2_8, double dt, double CL2) {
+  const double r6 = r2 * r2 * r2;
+  const double df = ((24.0 * r6 - 48.0) * recip(r6 * r6 * r2) + C2_8) * dt;
+  return r2 > CL2 ? 0.0 : df;
+}
+
+
+// ------------------------------------------------------------------
+// Per-partner and per-lane arithmetic shared by every kernel that must
+// produce identical bits (single system and replica ensemble): one inline
+// definition gives one expression tree, so the compiler forms the same
+// fused operations wherever it is used.
+
+// r^2 of the list build, as the reference's AVX2 search forms it:
+// fma(dx,dx, fma(dy,dy, dz*dz)) with dq = q_j - q_i (meshlist.cc:354-362)
+__device__ __forceinline__ double list_r2(double xj, double yj, double zj, double xi, double yi,
+                                          double zi) {
+  const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+  return __fma_rn(dx, dx, __fma_rn(dy, dy, __dmul_rn(dz, dz)));
+}
+
+// one partner of the kick: a += df (q_j - q_i)   (fcalculator.cc:203-217)
+__device__ __forceinline__ void kick_term(double xj, double yj, double zj, double xi, double yi,
+                                          double zi, double C2_8, double dt, double CL2,
+                                          double &ax, double &ay, double &az) {
+  const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+  const double s2 = dx * dx + dy * dy + dz * dz;
+  const double df = pair_df(s2, C2_8, dt, CL2);
+  ax += df * dx;
+  ay += df * dy;
+  az += df * dz;
+}
+
+// q + p dt/2 (ForceCalculator::UpdatePositionHalf, fcalculator.cc:26-28)
+__device__ __forceinline__ double4 drift_half(double4 q, const double4 &p, double dt2) {
+  q.x = __dadd_rn(q.x, __dmul_rn(p.x, dt2));
+  q.y = __dadd_rn(q.y, __dmul_rn(p.y, dt2));
+  q.z = __dadd_rn(q.z, __dmul_rn(p.z, dt2));
+  return q;
+}
+
+// image of a position under shift code (sx+1) + 3(sy+1) + 9(sz+1)
+__device__ __forceinline__ double4 image_of(double4 q, int code, const double *L) {
+  q.x = shifted(q.x, code % 3 - 1, L[0]);
+  q.y = shifted(q.y, (code / 3) % 3 - 1, L[1]);
+  q.z = shifted(q.z, code / 9 - 1, L[2]);
+  q.w = 0.0;
+  return q;
+}
+
+// squared displacement old - new (pairlist.cc:73-80), also the pair
+// distances of the candidate test (pairlist.cc:98-112)
+__device__ __forceinline__ double disp2(const double4 &a, const double4 &b) {
+  const double dx = __dadd_rn(a.x, -b.x), dy = __dadd_rn(a.y, -b.y), dz = __dadd_rn(a.z, -b.z);
+  return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+}
+
+// observer terms of one partner within the cutoff: potential energy
+// 4 (r^-12 - r^-6 + C2 r^2 + C0) (observer.cc:18-47) and r V'(r)
+// (observer.cc:50-79)
+__device__ __forceinline__ void observe_term(double r2, double C0, double C2, double &pe,
+                                             double &vir) {
+  const double r6 = r2 * r2 * r2;
+  pe += 4.0 * (1.0 / (r2 * r2 * r2 * r2 * r2 * r2) - 1.0 / r6 + C2 * r2 + C0);
+  vir += ((24.0 * r6 - 48.0) / (r6 * r6 * r2) + C2 * 8.0) * r2;
+}
+
+}  // namespace mdxk
+
+#endif

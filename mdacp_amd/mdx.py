@@ -19,6 +19,10 @@ EXPORTS = (
     "mdx_update_position_half", "mdx_refresh_images", "mdx_calculate_force", "mdx_calculate",
     "mdx_observe", "mdx_forces", "mdx_pair_number", "mdx_pairs_canonical", "mdx_timing",
     "mdx_timing_reset", "mdx_kick_profile", "mdx_lj_trajectory",
+    "mdx_ensemble_create", "mdx_ensemble_destroy", "mdx_ensemble_last_error", "mdx_ensemble_size",
+    "mdx_ensemble_set_state", "mdx_ensemble_get_state", "mdx_ensemble_calculate",
+    "mdx_ensemble_status", "mdx_ensemble_observe", "mdx_ensemble_get_config",
+    "mdx_ensemble_set_config", "mdx_ensemble_autotune",
 )
 
 STATUS = {0: "MDX_OK", -1: "MDX_ERR_ARG", -2: "MDX_ERR_CAPACITY", -3: "MDX_ERR_INVALID_POS",
@@ -36,6 +40,11 @@ class MdxLjResult(ctypes.Structure):
                 ("pe_mean", ctypes.c_double), ("dloss_deps", ctypes.c_double),
                 ("dloss_dsigma", ctypes.c_double), ("forward_ms", ctypes.c_double),
                 ("backward_ms", ctypes.c_double)]
+
+
+class MdxEnsembleConfig(ctypes.Structure):
+    _fields_ = [("tile", ctypes.c_int), ("grid_mult", ctypes.c_int), ("step_grid", ctypes.c_int),
+                ("stage_cap", ctypes.c_int), ("ms_per_step", ctypes.c_double)]
 
 
 class MdxError(RuntimeError):
@@ -81,6 +90,18 @@ def load() -> ctypes.CDLL:
         "mdx_lj_trajectory": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                              ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_int, P(MdxLjResult), dp]),
+        "mdx_ensemble_create": (ctypes.c_int, [P(MdxParams), ctypes.c_int, ip, dp, P(vp)]),
+        "mdx_ensemble_destroy": (ctypes.c_int, [vp]),
+        "mdx_ensemble_last_error": (ctypes.c_char_p, [vp]),
+        "mdx_ensemble_size": (ctypes.c_int, [vp, ip, ip]),
+        "mdx_ensemble_set_state": (ctypes.c_int, [vp, dp, dp]),
+        "mdx_ensemble_get_state": (ctypes.c_int, [vp, dp, dp]),
+        "mdx_ensemble_calculate": (ctypes.c_int, [vp, ctypes.c_int, ip]),
+        "mdx_ensemble_status": (ctypes.c_int, [vp, ip, ip]),
+        "mdx_ensemble_observe": (ctypes.c_int, [vp, dp, dp, dp]),
+        "mdx_ensemble_get_config": (ctypes.c_int, [vp, P(MdxEnsembleConfig)]),
+        "mdx_ensemble_set_config": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "mdx_ensemble_autotune": (ctypes.c_int, [vp, ctypes.c_int, P(MdxEnsembleConfig)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -233,3 +254,95 @@ class System:
         out = {k: getattr(r, k) for k, _ in MdxLjResult._fields_}
         out["dloss_dq0"] = dq0
         return out
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class Ensemble:
+    """Independent periodic systems ("replicas") advanced together on one
+    GPU; each replica's trajectory is the one :class:`System` gives it alone.
+    ``states`` is a sequence of objects with ``q``, ``p`` ([n][3]) and ``L``."""
+
+    def __init__(self, states, dt=0.001, cutoff=3.0, buffer_length=0.3, check_list_length=500,
+                 device=0, stream=None):
+        self.lib = load()
+        prm = MdxParams()
+        self.lib.mdx_default_params(ctypes.byref(prm))
+        prm.dt, prm.cutoff, prm.buffer_length = dt, cutoff, buffer_length
+        prm.check_list_length, prm.device, prm.stream = check_list_length, device, stream
+        self.sizes = np.array([s.q.shape[0] for s in states], dtype=np.int32)
+        L3 = np.ascontiguousarray([[float(x) for x in s.L] for s in states], dtype=np.float64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)])
+        self.R = len(states)
+        h = ctypes.c_void_p()
+        rc = self.lib.mdx_ensemble_create(ctypes.byref(prm), self.R, _ip(self.sizes), _dp(L3),
+                                          ctypes.byref(h))
+        if rc != 0:
+            raise MdxError(rc, self.lib.mdx_ensemble_last_error(None).decode())
+        self.h = h
+        self.set_state([s.q for s in states], [s.p for s in states])
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise MdxError(rc, self.lib.mdx_ensemble_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mdx_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_state(self, qs, ps):
+        q = np.ascontiguousarray(np.concatenate(qs), dtype=np.float64)
+        p = np.ascontiguousarray(np.concatenate(ps), dtype=np.float64)
+        assert q.shape == (self.offsets[-1], 3) and p.shape == q.shape
+        self._chk(self.lib.mdx_ensemble_set_state(self.h, _dp(q), _dp(p)))
+
+    def state(self):
+        """[(q, p)] per replica, by particle id."""
+        n = int(self.offsets[-1])
+        q = np.empty((n, 3))
+        p = np.empty((n, 3))
+        self._chk(self.lib.mdx_ensemble_get_state(self.h, _dp(q), _dp(p)))
+        o = self.offsets
+        return [(q[o[r]:o[r + 1]], p[o[r]:o[r + 1]]) for r in range(self.R)]
+
+    def calculate(self, nsteps: int):
+        """Runs nsteps; returns each replica's pairlist constructions so far."""
+        reb = np.zeros(self.R, dtype=np.int32)
+        self._chk(self.lib.mdx_ensemble_calculate(self.h, nsteps, _ip(reb)))
+        return reb
+
+    def status(self):
+        st = np.zeros(self.R, dtype=np.int32)
+        steps = np.zeros(self.R, dtype=np.int32)
+        self._chk(self.lib.mdx_ensemble_status(self.h, _ip(st), _ip(steps)))
+        return st, steps
+
+    def observe(self):
+        ke, pe, vir = (np.zeros(self.R) for _ in range(3))
+        self._chk(self.lib.mdx_ensemble_observe(self.h, _dp(ke), _dp(pe), _dp(vir)))
+        return ke, pe, vir
+
+    def config(self):
+        c = MdxEnsembleConfig()
+        self._chk(self.lib.mdx_ensemble_get_config(self.h, ctypes.byref(c)))
+        return {k: getattr(c, k) for k, _ in MdxEnsembleConfig._fields_}
+
+    def set_config(self, tile: int, grid_mult: int = 2):
+        self._chk(self.lib.mdx_ensemble_set_config(self.h, tile, grid_mult))
+
+    def autotune(self, trial_steps: int = 20):
+        c = MdxEnsembleConfig()
+        self._chk(self.lib.mdx_ensemble_autotune(self.h, trial_steps, ctypes.byref(c)))
+        return {k: getattr(c, k) for k, _ in MdxEnsembleConfig._fields_}

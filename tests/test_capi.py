@@ -53,3 +53,17 @@ def test_create_rejects_small_box_before_touching_device():
     h = ctypes.c_void_p()
     assert lib.mdx_create(ctypes.byref(p), ctypes.byref(h)) == -5
     assert b"search length" in lib.mdx_last_error(None)
+
+
+def test_ensemble_create_rejects_small_box_before_touching_device():
+    lib = mdx.load()
+    p = mdx.MdxParams()
+    lib.mdx_default_params(ctypes.byref(p))
+    n = (ctypes.c_int * 2)(500, 500)
+    L3 = (ctypes.c_double * 6)(10.0, 10.0, 10.0, 10.0, 5.0, 10.0)
+    h = ctypes.c_void_p()
+    assert lib.mdx_ensemble_create(ctypes.byref(p), 2, n, L3, ctypes.byref(h)) == -5
+    assert b"replica 1" in lib.mdx_ensemble_last_error(None)
+    p.check_list_length = 5000
+    L3[4] = 10.0
+    assert lib.mdx_ensemble_create(ctypes.byref(p), 2, n, L3, ctypes.byref(h)) == -1

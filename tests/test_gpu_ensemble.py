@@ -1,0 +1,134 @@
+"""Batched replicas (libmdx mdx_ensemble_*): every replica of an ensemble
+must end bit for bit where the single-system path (mdx_calculate) takes it
+alone, with the same number of pairlist constructions, whatever tile size
+and grid the autotuner picks; observables agree with mdx_observe.
+
+Replica sets follow the batched workload: FCC blocks with n ~ U{5..20}
+cells, density ~ U[0.3, 0.9], T ~ U[0.7, 1.5] (mdacp_amd.workloads.replica_set).
+"""
+import numpy as np
+import pytest
+
+from mdacp_amd import workloads as W
+from mdacp_amd.mdx import Ensemble, MdxError, System
+
+pytestmark = pytest.mark.gpu
+
+
+def small_set():
+    """Six uneven replicas: 2-3 cells per axis, densities 0.3-0.9,
+    temperatures 0.7-1.5, one perturbed off the lattice."""
+    specs = [(5, 0.9, 1.5, 11), (6, 0.3, 0.7, 12), (5, 0.6, 1.0, 13), (7, 0.8, 1.2, 14),
+             (6, 0.45, 1.4, 15), (5, 0.75, 0.9, 16)]
+    out = [W.fcc_state(n, rho, T, seed) for n, rho, T, seed in specs]
+    out[2] = W.perturbed(out[2], 0.05, 17)
+    return out
+
+
+def plain_run(s, nsteps):
+    with System.from_state(s) as p:
+        reb = p.calculate(nsteps)
+        q, pm = p.state()
+        obs = p.observe()
+    return q, pm, reb, obs
+
+
+def assert_same_as_plain(ens_states, rebuilds, states, nsteps, which):
+    for r in which:
+        q, p, reb, _ = plain_run(states[r], nsteps)
+        eq, ep = ens_states[r]
+        assert rebuilds[r] == reb, (r, rebuilds[r], reb)
+        np.testing.assert_array_equal(eq, q, err_msg=f"replica {r} positions")
+        np.testing.assert_array_equal(ep, p, err_msg=f"replica {r} momenta")
+
+
+def test_small_set_matches_single_system_bitwise():
+    states = small_set()
+    print("ensemble of", [s.n for s in states], flush=True)
+    with Ensemble(states) as e:
+        reb = e.calculate(150)
+        reb = e.calculate(150)          # a second call continues the same trajectories
+        st, steps = e.status()
+        assert (st == 0).all() and (steps == 300).all()
+        got = e.state()
+    assert (reb >= 2).all(), reb        # the lists expired and were rebuilt on the way
+    assert_same_as_plain(got, reb, states, 300, range(len(states)))
+
+
+def test_tile_sizes_and_grids_give_identical_trajectories():
+    states = small_set()
+    outs = []
+    for tile, gm in ((64, 1), (128, 2), (64, 4)):
+        with Ensemble(states) as e:
+            e.set_config(tile, gm)
+            e.calculate(120)
+            cfg = e.config()
+            assert cfg["tile"] == tile and cfg["grid_mult"] == gm
+            outs.append(e.state())
+    for other in outs[1:]:
+        for (q0, p0), (q1, p1) in zip(outs[0], other):
+            np.testing.assert_array_equal(q0, q1)
+            np.testing.assert_array_equal(p0, p1)
+
+
+def test_observables_match_single_system():
+    states = small_set()
+    with Ensemble(states) as e:
+        e.calculate(80)
+        ke, pe, vir = e.observe()
+    for r, s in enumerate(states):
+        _, _, _, (k1, p1, v1) = plain_run(s, 80)
+        assert ke[r] == pytest.approx(k1, rel=1e-12)
+        assert pe[r] == pytest.approx(p1, rel=1e-12)
+        assert vir[r] == pytest.approx(v1, rel=1e-12)
+
+
+def test_replica_set_256_sampled_against_single_system():
+    states = W.replica_set()
+    n = [s.n for s in states]
+    print(f"256 replicas, {sum(n)} particles", flush=True)
+    with Ensemble(states) as e:
+        cfg = e.autotune(5)
+        print("autotuned", cfg, flush=True)
+        reb = e.calculate(60)
+        got = e.state()
+        st, steps = e.status()
+    assert (st == 0).all() and (steps == 60).all()
+    order = np.argsort(n)
+    dens = np.argsort([s.n / float(np.prod(s.L)) for s in states])
+    which = sorted({int(order[0]), int(order[1]), int(order[-1]), int(order[128]), int(dens[0]),
+                    int(dens[-1]), 7, 200})
+    assert_same_as_plain(got, reb, states, 60, which)
+
+
+def test_failing_replica_stops_alone():
+    states = small_set()
+    bad = states[1].copy()
+    bad.p[0] = [400.0, 0.0, 0.0]        # |p| 2 dt > buffer: "Too fast particles"
+    states[1] = bad
+    with Ensemble(states) as e:
+        with pytest.raises(MdxError) as ei:
+            e.calculate(40)
+        assert ei.value.code == -4
+        st, steps = e.status()
+        got = e.state()
+    assert st[1] == -4 and steps[1] == 1
+    assert (np.delete(st, 1) == 0).all() and (np.delete(steps, 1) == 40).all()
+    with System.from_state(bad) as p:
+        with pytest.raises(MdxError) as ei:
+            p.calculate(40)
+        assert ei.value.code == -4
+    others = [r for r in range(len(states)) if r != 1]
+    for r in others:
+        q, p, _, _ = plain_run(states[r], 40)
+        np.testing.assert_array_equal(got[r][0], q)
+        np.testing.assert_array_equal(got[r][1], p)
+
+
+def test_autotune_refuses_after_stepping():
+    states = small_set()[:2]
+    with Ensemble(states) as e:
+        e.calculate(3)
+        with pytest.raises(MdxError) as ei:
+            e.autotune(2)
+        assert ei.value.code == -7
