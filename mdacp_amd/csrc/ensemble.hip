@@ -69,6 +69,7 @@ struct Rep {
 struct RepCtl {
   double budget;     // PairList::buffer_length
   double vmax2;      // max |p|^2 after the last step
+  double cd2;        // candidate threshold of the running displacement check
   int need_rebuild;  // pairlist expired: rebuild before the next step
   int eflags;        // failure bits (EnsFlags) of the last rebuild / guard
   int error;         // mdx_status that stopped the replica, 0 while running
@@ -79,19 +80,24 @@ struct RepCtl {
   int par;            // mid-step buffer (qh0 / qh1) that holds the next step's q + p dt/2
   int stepped;        // a step ran since the last rebuild
   int cand;           // candidates of the last displacement check
-  int pad;
+  int check;          // the budget is spent: a displacement check runs after this step
+  int check_slot;     // its position in the check list
+  int mid;            // mid-step buffer of the step just done
+  int pad[3];
 };
 
 struct EnsDev {
   const Rep *rep;
   RepCtl *ctl;
-  int *meta;      // meta[0]: number of replicas flagged for a rebuild in this step
+  int *meta;      // meta[0]: replicas flagged for a rebuild in this step; meta[1]: checks
   int *flagged;   // their ids
+  int *checklist; // replicas that run a displacement check after this step
   int R, NR, NI, NT;
   int T;          // tile size (reals per tile)
   int stage_cap;  // particles a tile may stage (dynamic LDS / 24 B)
   int check_list_length;
-  double dt, buffer, SL, C2_8;
+  double dt, buffer, SL;
+  PairK pk;       // pair-term constants with dt folded in
   double4 *qe;    // [NR + NI] reals at the end of the step; images as built
   double4 *qh0, *qh1;   // [NR] mid-step positions q + p dt/2 (two buffers)
   double4 *mom, *ref;   // [NR], [NR + NI] snapshot at the last build
@@ -105,6 +111,9 @@ struct EnsDev {
   unsigned short *list;
   int *nn;
   double *partial;      // [NT] max |p|^2 per tile
+  double *disp;         // [NR + NI] squared displacement since the build
+  double *dpart;        // [R][kRebuildBlocks] block maxima of disp
+  int *cand_buf;        // [R][kMaxCand] candidates of each check
   double *obs_partial;  // [3 NT] KE, PE, virial per tile
   int *tile_kmax;       // [NT]
 };
@@ -205,7 +214,8 @@ __device__ __forceinline__ void e_segment(const EnsDev &d, const Rep &rp, int y,
   }
 }
 
-// Stage the 18 segments of tile t into LDS as x[], y[], z[].  Reals come
+// Stage the 18 segments of tile t into LDS as interleaved (x, y, z) triples
+// (list entries are 3 x the staged index).  Reals come
 // from `reals`; images from the positions stored at the build (mid ==
 // nullptr) or as their source's mid-step position plus the shift, which is
 // the image refresh of MDUnit::MakeBufferForBorderParticles
@@ -224,24 +234,13 @@ __device__ void e_stage(const EnsDev &d, const Rep &rp, const TileDesc &t, const
     for (int k = 0; k < kSeg; k++) sm.seg_base[k + 1] += sm.seg_base[k];
   }
   __syncthreads();
-  for (int k = 0; k < kSeg; k++) {
-    const int b = sm.seg_base[k], len = sm.seg_base[k + 1] - b, st = sm.seg_start[k];
-    for (int u = threadIdx.x; u < len; u += blockDim.x) {
-      double4 q;
-      if ((k & 1) == 0) {
-        q = reals[st + u];
-      } else if (mid) {
-        const int idx = st + u - d.NR;
-        q = image_of(mid[d.gsrc[idx]], d.gcode[idx], rp.g.L);
-      } else {
-        q = d.qe[st + u];
-      }
-      sx[b + u] = q.x;
-      sy[b + u] = q.y;
-      sz[b + u] = q.z;
-    }
-  }
-  __syncthreads();
+  stage_segments(sm, sx, [&](int k, int slot) {
+    if ((k & 1) == 0) return reals[slot];
+    if (mid) return image_of(mid[d.gsrc[slot - d.NR]], d.gcode[slot - d.NR], rp.g.L);
+    return d.qe[slot];
+  });
+  (void)sy;
+  (void)sz;
 }
 
 // Persistent tile loop: workgroup b runs on XCD b % 8; the workgroups of one
@@ -499,24 +498,16 @@ __global__ void k_e_scatter(EnsDev d) {
 // atomics fixed the cell, not the order in it: sort each cell by pre-sort
 // key, which makes the result a stable counting sort
 __global__ void k_e_cell_sort(EnsDev d) {
+  __shared__ int buf[kBlock / 64][kSortBuf];
   const int nf = d.meta[0];
+  const int w = threadIdx.x >> 6, nw = gridDim.x * (kBlock / 64);
   for (int f = 0; f < nf; f++) {
     const int r = d.flagged[f];
     if (d.ctl[r].eflags) continue;
     const Rep &rp = d.rep[r];
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < rp.ncell; k += gridDim.x * blockDim.x) {
+    for (int k = blockIdx.x * (kBlock / 64) + w; k < rp.ncell; k += nw) {
       const int cl = rp.cbase + k;
-      int *seg = d.order + d.cell_start[cl];
-      const int len = d.cell_count[cl];
-      for (int a = 1; a < len; a++) {
-        const int v = seg[a];
-        int b = a - 1;
-        while (b >= 0 && seg[b] > v) {
-          seg[b + 1] = seg[b];
-          b--;
-        }
-        seg[b + 1] = v;
-      }
+      wave_rank_sort(d.order + d.cell_start[cl], d.cell_count[cl], buf[w]);
     }
   }
 }
@@ -573,22 +564,33 @@ __global__ void __launch_bounds__(T) k_e_build(EnsDev d) {
   extern __shared__ double stage_mem[];
   __shared__ TileSmem sm;
   const int nf = d.meta[0];
+  int total = 0;
   for (int f = 0; f < nf; f++) {
+    const RepCtl *c = d.ctl + d.flagged[f];
+    total += c->eflags ? 0 : c->ntiles;
+  }
+  // work items: the tiles of every flagged replica, spread over all blocks
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    int f = 0, s = item;
+    for (;; f++) {
+      const RepCtl *c = d.ctl + d.flagged[f];
+      const int nt = c->eflags ? 0 : c->ntiles;
+      if (s < nt) break;
+      s -= nt;
+    }
     const int r = d.flagged[f];
-    if (d.ctl[r].eflags) continue;
     const Rep &rp = d.rep[r];
     const Geom &g = rp.g;
-    const int nt = d.ctl[r].ntiles;
-    for (int s = blockIdx.x; s < nt; s += gridDim.x) {
+    {
       const TileDesc t = d.tiles[rp.tbase + s];
-      double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+      double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
       e_stage(d, rp, t, d.qe, nullptr, sm, sx, sy, sz);
       const int lane = threadIdx.x;
       int cnt = 0;
       if (lane < t.ni) {
         const int i = t.i0 + lane;
         const int self = sm.seg_base[8] + (i - sm.seg_start[8]);
-        const double qx = sx[self], qy = sy[self], qz = sz[self];
+        const double qx = sx[3 * self], qy = sy[3 * self], qz = sz[3 * self];
         const int cx = interior_coord(qx, g.inv_w[0], g.ms[0]);
         unsigned short *out = d.list + rp.lbase + (size_t)s * rp.kcap * T + lane;
         for (int rr = 0; rr < 9; rr++) {
@@ -604,10 +606,10 @@ __global__ void __launch_bounds__(T) k_e_build(EnsDev d) {
               lo = sm.seg_base[2 * rr] + (d.scan_in[cl] - sm.seg_start[2 * rr]);
               len = d.cnt_in[cl];
             }
-            for (int o = lo; o < lo + len; o++) {
-              if (o == self) continue;
-              if (list_r2(sx[o], sy[o], sz[o], qx, qy, qz) <= g.SL2) {
-                if (cnt < rp.kcap) out[(size_t)cnt * T] = (unsigned short)o;
+            for (int e = 3 * lo; e < 3 * (lo + len); e += 3) {
+              if (e == 3 * self) continue;
+              if (list_r2(sx[e], sy[e], sz[e], qx, qy, qz) <= g.SL2) {
+                if (cnt < rp.kcap) out[(size_t)cnt * T] = (unsigned short)e;
                 cnt++;
               }
             }
@@ -672,7 +674,7 @@ __global__ void __launch_bounds__(T) k_e_step(EnsDev d) {
     const Rep &rp = d.rep[r];
     const double4 *qh = c->par ? d.qh1 : d.qh0;
     double4 *qn = c->par ? d.qh0 : d.qh1;
-    double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+    double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
     e_stage(d, rp, t, qh, qh, sm, sx, sy, sz);
     const int lane = threadIdx.x;
     double v2 = 0.0;
@@ -700,12 +702,12 @@ __global__ void __launch_bounds__(T) k_e_step(EnsDev d) {
 #pragma unroll
         for (int u = 0; u < kILP; u++) {
           const int o = cur[u];
-          kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, d.C2_8, d.dt, CL2, ax[u], ay[u], az[u]);
+          kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, d.pk, CL2, ax[u], ay[u], az[u]);
         }
       }
       for (int k = mg; k < m; k++) {
         const int o = lst[(size_t)k * T];
-        kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, d.C2_8, d.dt, CL2, ax[0], ay[0], az[0]);
+        kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, d.pk, CL2, ax[0], ay[0], az[0]);
       }
       double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll
@@ -730,97 +732,155 @@ __global__ void __launch_bounds__(T) k_e_step(EnsDev d) {
   }
 }
 
-// Expiry check for the next step, one workgroup per stepping replica:
-// PairList::IsPairListExpired / UpdatePairListValidity /
-// CheckByDisplacement (pairlist.cc:26-121).  Images enter the displacement
-// test at their mid-step positions, as the reference's ghosts do after a
-// step.  The budget arithmetic is rounded operation by operation like the
-// host's, and sqrt is the correctly rounded one.
-__global__ void __launch_bounds__(kReplicaThreads) k_e_guard(EnsDev d) {
+// Expiry check for the next step: PairList::IsPairListExpired /
+// UpdatePairListValidity / CheckByDisplacement (pairlist.cc:26-121), in five
+// kernels so that the displacement and candidate passes of a large replica
+// spread over many workgroups:
+//   budget   per replica: |p|max of the step, the velocity budget, the
+//            Too-fast test; a spent budget queues a displacement check
+//   disp     all workgroups: |old - now|^2 of every real and image of the
+//            queued replicas, with block maxima
+//   dfold    per replica: max displacement -> candidate threshold
+//   cand     all workgroups: candidates above the threshold
+//   decide   per replica: too many candidates, or the pair test
+// Images enter at their mid-step positions, as the reference's ghosts do
+// after a step.  The budget arithmetic is rounded operation by operation as
+// the host does it, and sqrt is the correctly rounded one.
+__global__ void k_e_budget(EnsDev d) {
 #pragma clang fp contract(off)
-  __shared__ int cand[kMaxCand];
-  __shared__ int ncand, hit;
   const int r = blockIdx.x;
   RepCtl *c = d.ctl + r;
-  if (!stepping(c)) return;
+  if (!stepping(c)) {
+    // no step, no check: the later kernels of this step leave it alone
+    __syncthreads();
+    if (threadIdx.x == 0) c->check = 0;
+    return;
+  }
   const Rep &rp = d.rep[r];
-  const int tid = threadIdx.x;
   double v = 0.0;
-  for (int s = tid; s < c->ntiles; s += blockDim.x) v = fmax(v, d.partial[rp.tbase + s]);
+  for (int s = threadIdx.x; s < c->ntiles; s += blockDim.x) v = fmax(v, d.partial[rp.tbase + s]);
   const double v2max = block_max(v);
+  if (threadIdx.x != 0) return;
   const int par = c->par;
   double budget = c->budget;
-  const int g = c->g, tn = rp.n + g;
-  __syncthreads();
-  if (tid == 0) {
-    c->steps = c->steps + 1;
-    c->par = par ^ 1;
-    c->stepped = 1;
-    c->vmax2 = v2max;
-  }
+  c->steps = c->steps + 1;
+  c->mid = par;
+  c->par = par ^ 1;
+  c->stepped = 1;
+  c->vmax2 = v2max;
+  c->check = 0;
   if (budget > 0.0) {
     const double vmax = __dsqrt_rn(v2max);
     const double move = vmax * 2.0 * d.dt;
     if (move > d.buffer) {
-      if (tid == 0) {
-        c->eflags |= EF_TOO_FAST;
-        c->error = MDX_ERR_TOO_FAST;
-      }
+      c->eflags |= EF_TOO_FAST;
+      c->error = MDX_ERR_TOO_FAST;
       return;
     }
     budget = budget - move;
   }
-  int expired = 0;
+  c->budget = budget;
+  c->need_rebuild = 0;
   if (budget < 0.0) {
-    const double4 *mid = par ? d.qh1 : d.qh0;   // this step's mid-step positions
+    c->check = 1;
+    c->cand = 0;
+    const int f = atomicAdd(d.meta + 1, 1);
+    c->check_slot = f;
+    d.checklist[f] = r;
+  }
+}
+
+// position a displacement check sees for slot s of replica rp
+__device__ __forceinline__ double4 check_pos(const EnsDev &d, const Rep &rp, const double4 *mid, int s) {
+  return s < d.NR ? d.qe[s] : image_of(mid[d.gsrc[s - d.NR]], d.gcode[s - d.NR], rp.g.L);
+}
+
+__device__ __forceinline__ int check_slot_of(const EnsDev &d, const Rep &rp, int k) {
+  return k < rp.n ? rp.rbase + k : d.NR + rp.ibase + (k - rp.n);
+}
+
+__global__ void k_e_disp(EnsDev d) {
+  const int nf = d.meta[1];
+  for (int f = 0; f < nf; f++) {
+    const int r = d.checklist[f];
+    const RepCtl *c = d.ctl + r;
+    const Rep &rp = d.rep[r];
+    const double4 *mid = c->mid ? d.qh1 : d.qh0;
+    const int tn = rp.n + c->g;
     double m = 0.0;
-    for (int k = tid; k < tn; k += blockDim.x) {
-      const int s = k < rp.n ? rp.rbase + k : d.NR + rp.ibase + (k - rp.n);
-      const double4 now = k < rp.n ? d.qe[s] : image_of(mid[d.gsrc[s - d.NR]], d.gcode[s - d.NR], rp.g.L);
-      m = fmax(m, disp2(d.ref[s], now));
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < tn; k += gridDim.x * blockDim.x) {
+      const int sl = check_slot_of(d, rp, k);
+      const double d2 = disp2(d.ref[sl], check_pos(d, rp, mid, sl));
+      d.disp[sl] = d2;
+      m = fmax(m, d2);
     }
-    const double dmax2 = block_max(m);
-    const int Nl = tn > d.check_list_length ? d.check_list_length : tn;
+    m = block_max(m);
+    if (threadIdx.x == 0) d.dpart[(size_t)f * gridDim.x + blockIdx.x] = m;
+  }
+}
+
+__global__ void k_e_dfold(EnsDev d, int nblocks) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x;
+  RepCtl *c = d.ctl + r;
+  if (c->error || !c->check) return;
+  const int f = c->check_slot;
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) m = fmax(m, d.dpart[(size_t)f * nblocks + b]);
+  const double dmax2 = block_max(m);
+  if (threadIdx.x == 0) {
     const double c_distance = d.buffer - __dsqrt_rn(dmax2);
-    const double c_d2 = c_distance * c_distance;
-    if (tid == 0) {
-      ncand = 0;
-      hit = 0;
-    }
-    __syncthreads();
-    for (int k = tid; k < tn; k += blockDim.x) {
-      const int s = k < rp.n ? rp.rbase + k : d.NR + rp.ibase + (k - rp.n);
-      const double4 now = k < rp.n ? d.qe[s] : image_of(mid[d.gsrc[s - d.NR]], d.gcode[s - d.NR], rp.g.L);
-      if (disp2(d.ref[s], now) > c_d2) {
-        const int at = atomicAdd(&ncand, 1);
-        if (at < kMaxCand) cand[at] = s;
+    c->cd2 = c_distance * c_distance;
+  }
+}
+
+__global__ void k_e_cand(EnsDev d) {
+  const int nf = d.meta[1];
+  for (int f = 0; f < nf; f++) {
+    const int r = d.checklist[f];
+    RepCtl *c = d.ctl + r;
+    const Rep &rp = d.rep[r];
+    const int tn = rp.n + c->g;
+    const double cd2 = c->cd2;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < tn; k += gridDim.x * blockDim.x) {
+      const int sl = check_slot_of(d, rp, k);
+      if (d.disp[sl] > cd2) {
+        const int at = atomicAdd(&c->cand, 1);
+        if (at < kMaxCand) d.cand_buf[(size_t)f * kMaxCand + at] = sl;
       }
     }
+  }
+}
+
+__global__ void __launch_bounds__(kReplicaThreads) k_e_decide(EnsDev d) {
+  __shared__ int hit;
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.meta[1] = 0;
+  const int r = blockIdx.x;
+  RepCtl *c = d.ctl + r;
+  if (c->error || !c->check) return;
+  const Rep &rp = d.rep[r];
+  const int tn = rp.n + c->g;
+  const int Nl = tn > d.check_list_length ? d.check_list_length : tn;
+  const int l = c->cand;
+  int expired = 0;
+  if (l >= Nl) {
+    expired = 1;
+  } else if (l > 1) {
+    const double4 *mid = c->mid ? d.qh1 : d.qh0;
+    const int *cand = d.cand_buf + (size_t)c->check_slot * kMaxCand;
+    if (threadIdx.x == 0) hit = 0;
     __syncthreads();
-    const int l = ncand;
-    if (l >= Nl) {
-      expired = 1;
-    } else if (l > 1) {
-      for (long long p = tid; p < (long long)l * l; p += blockDim.x) {
-        const int a = (int)(p / l), b = (int)(p % l);
-        if (b <= a) continue;
-        const int s1 = cand[a], s2 = cand[b];
-        if (disp2(d.ref[s1], d.ref[s2]) < rp.g.SL2) continue;
-        const double4 n1 = s1 < d.NR ? d.qe[s1]
-                                     : image_of(mid[d.gsrc[s1 - d.NR]], d.gcode[s1 - d.NR], rp.g.L);
-        const double4 n2 = s2 < d.NR ? d.qe[s2]
-                                     : image_of(mid[d.gsrc[s2 - d.NR]], d.gcode[s2 - d.NR], rp.g.L);
-        if (disp2(n1, n2) < rp.g.CL2) hit = 1;
-      }
-      __syncthreads();
-      expired = hit;
+    for (long long p = threadIdx.x; p < (long long)l * l; p += blockDim.x) {
+      const int a = (int)(p / l), b = (int)(p % l);
+      if (b <= a) continue;
+      const int s1 = cand[a], s2 = cand[b];
+      if (disp2(d.ref[s1], d.ref[s2]) < rp.g.SL2) continue;
+      if (disp2(check_pos(d, rp, mid, s1), check_pos(d, rp, mid, s2)) < rp.g.CL2) hit = 1;
     }
-    if (tid == 0) c->cand = l;
+    __syncthreads();
+    expired = hit;
   }
-  if (tid == 0) {
-    c->budget = budget;
-    c->need_rebuild = expired;
-  }
+  if (threadIdx.x == 0) c->need_rebuild = expired;
 }
 
 // ------------------------------------------------------------ observe
@@ -843,7 +903,7 @@ __global__ void __launch_bounds__(T) k_e_observe(EnsDev d, double C0, double C2)
   }
   const Rep &rp = d.rep[r];
   const double4 *mid = c->stepped ? (c->par ? d.qh0 : d.qh1) : nullptr;
-  double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+  double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
   e_stage(d, rp, t, d.qe, mid, sm, sx, sy, sz);
   double ke = 0.0, pe = 0.0, vir = 0.0;
   if (lane < t.ni) {
@@ -950,7 +1010,8 @@ struct mdx_ensemble {
   std::vector<int> kcap;   // per replica list capacity
   Rep *rep = nullptr;
   RepCtl *ctl = nullptr;
-  int *meta = nullptr, *flagged = nullptr;
+  int *meta = nullptr, *flagged = nullptr, *checklist = nullptr, *cand_buf = nullptr;
+  double *disp = nullptr, *dpart = nullptr;
   double4 *qe = nullptr, *qh0 = nullptr, *qh1 = nullptr, *mom = nullptr, *ref = nullptr,
           *pos_tmp = nullptr, *mom_tmp = nullptr, *gpos_tmp = nullptr, *init_q = nullptr,
           *init_p = nullptr;
@@ -1011,6 +1072,10 @@ EnsDev dev_view(const mdx_ensemble *e) {
   d.ctl = e->ctl;
   d.meta = e->meta;
   d.flagged = e->flagged;
+  d.checklist = e->checklist;
+  d.disp = e->disp;
+  d.dpart = e->dpart;
+  d.cand_buf = e->cand_buf;
   d.R = e->R;
   d.NR = e->NR;
   d.NI = e->NI;
@@ -1021,7 +1086,7 @@ EnsDev dev_view(const mdx_ensemble *e) {
   d.dt = e->prm.dt;
   d.buffer = e->prm.buffer_length;
   d.SL = e->SL;
-  d.C2_8 = 8.0 * e->C2;
+  d.pk = make_pairk(e->C2, e->prm.dt);
   d.qe = e->qe;
   d.qh0 = e->qh0;
   d.qh1 = e->qh1;
@@ -1155,7 +1220,11 @@ int enqueue_step_t(mdx_ensemble *e) {
   if (rc) return rc;
   const EnsDev d = dev_view(e);
   k_e_step<T><<<e->step_grid, T, stage_bytes(e), e->stream>>>(d);
-  k_e_guard<<<e->R, kReplicaThreads, 0, e->stream>>>(d);
+  k_e_budget<<<e->R, kBlock, 0, e->stream>>>(d);
+  k_e_disp<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
+  k_e_dfold<<<e->R, kBlock, 0, e->stream>>>(d, kRebuildBlocks);
+  k_e_cand<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
+  k_e_decide<<<e->R, kReplicaThreads, 0, e->stream>>>(d);
   return launch_check(e, "step");
 }
 
@@ -1382,6 +1451,9 @@ int mdx_ensemble_create(const mdx_params *prm, int nrep, const int *n, const dou
   const size_t tot = (size_t)NR + NI;
   if (!rc && ((rc = ealloc(e, &e->rep, nrep)) || (rc = ealloc(e, &e->ctl, nrep)) ||
               (rc = ealloc(e, &e->meta, 4)) || (rc = ealloc(e, &e->flagged, nrep)) ||
+              (rc = ealloc(e, &e->checklist, nrep)) || (rc = ealloc(e, &e->disp, tot)) ||
+              (rc = ealloc(e, &e->dpart, (size_t)nrep * kRebuildBlocks)) ||
+              (rc = ealloc(e, &e->cand_buf, (size_t)nrep * kMaxCand)) ||
               (rc = ealloc(e, &e->qe, tot)) || (rc = ealloc(e, &e->ref, tot)) ||
               (rc = ealloc(e, &e->qh0, NR)) || (rc = ealloc(e, &e->qh1, NR)) ||
               (rc = ealloc(e, &e->mom, NR)) || (rc = ealloc(e, &e->pos_tmp, NR)) ||
@@ -1427,7 +1499,7 @@ int mdx_ensemble_destroy(mdx_ensemble *e) {
   if (!e) return MDX_OK;
   if (e->exec) (void)hipGraphExecDestroy(e->exec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
-  void *bufs[] = {e->rep, e->ctl, e->meta, e->flagged, e->qe, e->qh0, e->qh1, e->mom, e->ref,
+  void *bufs[] = {e->rep, e->ctl, e->meta, e->flagged, e->checklist, e->disp, e->dpart, e->cand_buf, e->qe, e->qh0, e->qh1, e->mom, e->ref,
                   e->pos_tmp, e->mom_tmp, e->gpos_tmp, e->init_q, e->init_p, e->tag, e->tag_tmp,
                   e->gsrc, e->gcode, e->gsrc_tmp, e->gcode_tmp, e->cellid, e->slot, e->nimg, e->goff,
                   e->order, e->inv, e->cell_count, e->cnt_in, e->cnt_out, e->scan_in, e->scan_out,

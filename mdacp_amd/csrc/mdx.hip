@@ -9,7 +9,8 @@
 //         interior cells hold reals only, the outer layer images only
 //   tiles TileDesc[]      <= kTile consecutive reals of one cell row
 //   list16 u16[tile][K][kTile]  full neighbour list of every real (partners
-//         real or image), as offsets into its tile's LDS staging; nn[n]
+//         real or image), as 3 x the partner's index in its tile's LDS
+//         staging of interleaved (x, y, z); nn[n]
 //
 // The reference keeps q/p as AoS double[N][4] on the host, materialises
 // images with an MPI halo exchange and builds a half list on the CPU
@@ -309,21 +310,13 @@ __global__ void k_scatter(const int *__restrict__ cellid, const int *__restrict_
 
 // atomics fixed the cell, not the order inside it: sort each cell's
 // segment by pre-sort index so the result equals a stable counting sort
+// (one wavefront per cell)
 __global__ void k_cell_sort(int *__restrict__ order, const int *__restrict__ cell_start,
                             const int *__restrict__ cell_count, int ncell) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  int *seg = order + cell_start[c];
-  const int len = cell_count[c];
-  for (int a = 1; a < len; a++) {
-    const int v = seg[a];
-    int b = a - 1;
-    while (b >= 0 && seg[b] > v) {
-      seg[b + 1] = seg[b];
-      b--;
-    }
-    seg[b + 1] = v;
-  }
+  __shared__ int buf[kBlock / 64][kSortBuf];
+  const int w = threadIdx.x >> 6;
+  for (int c = blockIdx.x * (kBlock / 64) + w; c < ncell; c += gridDim.x * (kBlock / 64))
+    wave_rank_sort(order + cell_start[c], cell_count[c], buf[w]);
 }
 
 __global__ void k_gather_reals(const int *__restrict__ order, int n,
@@ -358,8 +351,9 @@ __global__ void k_gather_images(const int *__restrict__ order, int n, int g,
 // spanning cells xf..xl.  Its partners all lie in 9 cell-row ranges
 // (y+dy, z+dz, xf-1..xl+1), each of which is one contiguous run of reals
 // (interior cells, slots scan_in) and one of images (outer cells, slots
-// n + scan_out): 18 segments, staged once into LDS as x[], y[], z[].
-// Neighbour lists hold 16-bit offsets into that staging.
+// n + scan_out): 18 segments, staged once into LDS as interleaved (x, y, z)
+// triples.  Neighbour lists hold 16-bit entries e = 3 x (staged index), so a
+// partner is three LDS loads at fixed offsets from one address.
 
 struct Cells {
   const int *scan_in, *cnt_in, *scan_out, *cnt_out;
@@ -406,25 +400,19 @@ __device__ void tile_stage(const TileDesc &t, const Geom &g, const Cells &c, int
     for (int k = 0; k < kSeg; k++) sm.seg_base[k + 1] += sm.seg_base[k];
   }
   __syncthreads();
-  for (int k = 0; k < kSeg; k++) {
-    const int b = sm.seg_base[k], len = sm.seg_base[k + 1] - b, st = sm.seg_start[k];
-    for (int u = threadIdx.x; u < len; u += blockDim.x) {
-      const double4 q = pos[st + u];
-      sx[b + u] = q.x;
-      sy[b + u] = q.y;
-      sz[b + u] = q.z;
-    }
-  }
+  stage_segments(sm, sx, [&](int, int slot) { return pos[slot]; });
+  (void)sy;
+  (void)sz;
   (void)stride;
-  __syncthreads();
 }
 
-// rows -> number of tiles
+// rows (in row_of order) -> number of tiles
 __global__ void k_row_tiles(Geom g, Cells c, int *__restrict__ row_tiles) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int rows = g.ms[1] * g.ms[2];
   if (r >= rows) return;
-  const int y = r % g.ms[1] + 1, z = r / g.ms[1] + 1;
+  int y, z;
+  row_of(g, r, &y, &z);
   const int a = cell_linear(g, 1, y, z), b = cell_linear(g, g.ms[0], y, z);
   const int cnt = c.scan_in[b] + c.cnt_in[b] - c.scan_in[a];
   row_tiles[r] = (cnt + kTile - 1) / kTile;
@@ -437,7 +425,8 @@ __global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict_
   const int rows = g.ms[1] * g.ms[2];
   double smax = 0.0;
   if (r < rows) {
-    const int y = r % g.ms[1] + 1, z = r / g.ms[1] + 1;
+    int y, z;
+    row_of(g, r, &y, &z);
     const int a = cell_linear(g, 1, y, z), b = cell_linear(g, g.ms[0], y, z);
     const int first = c.scan_in[a], cnt = c.scan_in[b] + c.cnt_in[b] - first;
     for (int k = 0; k < row_tiles[r]; k++) {
@@ -480,14 +469,14 @@ __global__ void __launch_bounds__(kTile) k_build_tiles(
     return;
   }
   const TileDesc t = tiles[tile];
-  double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+  double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
   tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
   const int lane = threadIdx.x;
   int cnt = 0;
   if (lane < t.ni) {
     const int i = t.i0 + lane;
     const int self = sm.seg_base[8] + (i - sm.seg_start[8]);   // range (dy, dz) = (0, 0)
-    const double qx = sx[self], qy = sy[self], qz = sz[self];
+    const double qx = sx[3 * self], qy = sy[3 * self], qz = sz[3 * self];
     const int cx = interior_coord(qx, g.inv_w[0], g.ms[0]);
     unsigned short *out = list + (size_t)tile * kcap * kTile + lane;
     for (int r = 0; r < 9; r++) {
@@ -503,11 +492,11 @@ __global__ void __launch_bounds__(kTile) k_build_tiles(
           lo = sm.seg_base[2 * r] + (c.scan_in[cl] - sm.seg_start[2 * r]);
           len = c.cnt_in[cl];
         }
-        for (int o = lo; o < lo + len; o++) {
-          if (o == self) continue;
-          const double r2 = list_r2(sx[o], sy[o], sz[o], qx, qy, qz);
+        for (int e = 3 * lo; e < 3 * (lo + len); e += 3) {
+          if (e == 3 * self) continue;
+          const double r2 = list_r2(sx[e], sy[e], sz[e], qx, qy, qz);
           if (r2 <= g.SL2) {
-            if (cnt < kcap) out[(size_t)cnt * kTile] = (unsigned short)o;
+            if (cnt < kcap) out[(size_t)cnt * kTile] = (unsigned short)e;
             cnt++;
           }
         }
@@ -613,7 +602,7 @@ template <int MODE, bool SCALED = false>
 __global__ void __launch_bounds__(kTile) k_kick_tiles(
     const double4 *__restrict__ pos, double4 *__restrict__ mom, const TileDesc *__restrict__ tiles,
     int ntiles, Geom g, Cells c, int n, int kcap, const unsigned short *__restrict__ list,
-    const int *__restrict__ nn, double C2_8, double dt, double4 *__restrict__ out,
+    const int *__restrict__ nn, PairK pk, double dt, double4 *__restrict__ out,
     double *__restrict__ partial, LJ lj = LJ{}, const int *__restrict__ halt = nullptr) {
   extern __shared__ double stage_mem[];
   __shared__ TileSmem sm;
@@ -624,7 +613,7 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
     return;
   }
   const TileDesc t = tiles[tile];
-  double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+  double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
   tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
   const int lane = threadIdx.x;
   double v2 = 0.0;
@@ -641,14 +630,15 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
     const int mg = m - m % kILP;
     int nxt[kILP];
 #pragma unroll
-    for (int u = 0; u < kILP; u++) nxt[u] = u < mg ? lst[(size_t)u * kTile] : 0;
+    for (int u = 0; u < kILP; u++) nxt[u] = u < mg ? __builtin_nontemporal_load(lst + (size_t)u * kTile) : 0;
     for (int k = 0; k < mg; k += kILP) {
       int cur[kILP];
 #pragma unroll
       for (int u = 0; u < kILP; u++) cur[u] = nxt[u];
       if (k + kILP < mg) {
 #pragma unroll
-        for (int u = 0; u < kILP; u++) nxt[u] = lst[(size_t)(k + kILP + u) * kTile];
+        for (int u = 0; u < kILP; u++)
+          nxt[u] = __builtin_nontemporal_load(lst + (size_t)(k + kILP + u) * kTile);
       }
 #pragma unroll
       for (int u = 0; u < kILP; u++) {
@@ -660,7 +650,7 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
           ay[u] += df * dy;
           az[u] += df * dz;
         } else {
-          kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, C2_8, dt, g.CL2, ax[u], ay[u], az[u]);
+          kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, pk, g.CL2, ax[u], ay[u], az[u]);
         }
       }
     }
@@ -673,7 +663,7 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
         ay[0] += df * dy;
         az[0] += df * dz;
       } else {
-        kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, C2_8, dt, g.CL2, ax[0], ay[0], az[0]);
+        kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, pk, g.CL2, ax[0], ay[0], az[0]);
       }
     }
     double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -765,7 +755,7 @@ __global__ void __launch_bounds__(kTile) k_lj_observe_tiles(
     return;
   }
   const TileDesc t = tiles[tile];
-  double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+  double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
   tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
   const int lane = threadIdx.x;
   double acc[K];
@@ -860,8 +850,8 @@ __global__ void __launch_bounds__(kTile) k_adj_hvp_tiles(
     return;
   }
   const TileDesc t = tiles[tile];
-  double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
-  int *sidx = reinterpret_cast<int *>(stage_mem + 3 * t.stage);
+  double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
+  int *sidx = reinterpret_cast<int *>(stage_mem + 3 * t.stage);   // slot of staged particle e / 3
   tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
   for (int k = 0; k < kSeg; k++) {
     const int b = sm.seg_base[k], len = sm.seg_base[k + 1] - b, st = sm.seg_start[k];
@@ -882,7 +872,7 @@ __global__ void __launch_bounds__(kTile) k_adj_hvp_tiles(
       const double s2 = dx * dx + dy * dy + dz * dz;
       double phi, dphi, V;
       lj_terms(s2, lj, &phi, &dphi, &V);
-      const double4 aj = ext[sidx[o]];
+      const double4 aj = ext[sidx[o / 3]];
       const double wx = aj.x - ai.x, wy = aj.y - ai.y, wz = aj.z - ai.z;
       const double dw = 2.0 * dphi * (dx * wx + dy * wy + dz * wz);
       jx += phi * wx + dw * dx;
@@ -941,7 +931,7 @@ __global__ void __launch_bounds__(kTile) k_observe_tiles(
     return;
   }
   const TileDesc t = tiles[tile];
-  double *sx = stage_mem, *sy = stage_mem + t.stage, *sz = stage_mem + 2 * t.stage;
+  double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
   tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
   const int lane = threadIdx.x;
   double ke = 0.0, pe = 0.0, vir = 0.0;
@@ -1078,7 +1068,7 @@ __global__ void __launch_bounds__(kTile) k_canonical_tiles(
   int cnum = 0;
   int w = WRITE ? offset[i] : 0;
   for (int k = 0; k < nn[i]; k++) {
-    const int o = lst[(size_t)k * kTile];
+    const int o = lst[(size_t)k * kTile] / 3;
     int sgi = 0;
     while (o >= sm.seg_base[sgi + 1]) sgi++;
     const int j = sm.seg_start[sgi] + (o - sm.seg_base[sgi]);
@@ -1177,6 +1167,7 @@ struct mdx_system {
   double t_build = 0, t_force = 0, t_other = 0;
   long long n_build = 0, n_steps = 0, n_kick = 0;
   double t_kick_only = 0;
+  long long n_kick_timed = 0;
   std::string err;
 };
 
@@ -1368,8 +1359,8 @@ int build_pairlist(mdx_system *s) {
   if (g > 0)
     k_scatter<<<nblk(g), kBlock, 0, s->stream>>>(s->cellid + n, s->slot + n, s->cell_start, g, n,
                                                   s->order);
-  k_cell_sort<<<nblk(s->ncell), kBlock, 0, s->stream>>>(s->order, s->cell_start, s->cell_count,
-                                                         s->ncell);
+  k_cell_sort<<<std::min(4096, (s->ncell + 3) / 4), kBlock, 0, s->stream>>>(
+      s->order, s->cell_start, s->cell_count, s->ncell);
   // 4. gather into cell order (pos_tmp/mom_tmp/tag_tmp hold the pre-sort copy)
   HIPCHK(s, hipMemcpyAsync(s->pos_tmp, s->pos, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream));
   HIPCHK(s, hipMemcpyAsync(s->mom_tmp, s->mom, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream));
@@ -1433,7 +1424,7 @@ int kick(mdx_system *s) {
   ev_record(s, 2);
   k_kick_tiles<0><<<grid, kTile, stage_bytes(s), s->stream>>>(
       s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), s->n, s->kcap, s->list16, s->nn,
-      8.0 * s->C2, s->prm.dt, nullptr, s->pmax);
+      make_pairk(s->C2, s->prm.dt), s->prm.dt, nullptr, s->pmax);
   ev_record(s, 3);
   s->vmax_valid = false;   // recomputed from p when the expiry check needs it
   s->n_kick++;
@@ -1467,7 +1458,7 @@ int run_batch(mdx_system *s, int want, int *ran) {
     if (s->timing) (void)hipEventRecord(s->evk[2 * b], s->stream);
     k_kick_tiles<2><<<grid, kTile, stage_bytes(s), s->stream>>>(
         s->pos_alt, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16,
-        s->nn, 8.0 * s->C2, s->prm.dt, s->pos, s->pmax, LJ{}, &s->dctl->halt);
+        s->nn, make_pairk(s->C2, s->prm.dt), s->prm.dt, s->pos, s->pmax, LJ{}, &s->dctl->halt);
     if (s->timing) (void)hipEventRecord(s->evk[2 * b + 1], s->stream);
     k_step_guard<<<1, kBlock, 0, s->stream>>>(s->pmax, grid, s->dctl);
   }
@@ -1488,6 +1479,7 @@ int run_batch(mdx_system *s, int want, int *ran) {
     const float all = ev_ms(s, 4, 5);
     s->t_force += k;
     s->t_kick_only += k;
+    s->n_kick_timed += *ran;
     s->t_other += all - k;
   }
   s->n_kick += *ran;
@@ -1619,7 +1611,7 @@ int lj_step(mdx_system *s, const LJ &lj, double4 *mid) {
                                                           s->geom.L[1], s->geom.L[2], mid, nullptr);
   k_kick_tiles<2, true><<<grid, kTile, stage_bytes(s), s->stream>>>(
       mid, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn,
-      8.0 * s->C2, s->prm.dt, s->pos, s->pmax, lj);
+      make_pairk(s->C2, s->prm.dt), s->prm.dt, s->pos, s->pmax, lj);
   return launch_check(s, "lj step");
 }
 
@@ -1997,6 +1989,10 @@ int mdx_calculate_force(mdx_system *s) {
   int rc = kick(s);
   if (rc) return rc;
   HIPCHK(s, hipStreamSynchronize(s->stream));
+  if (s->timing) {
+    s->t_kick_only += ev_ms(s, 2, 3);
+    s->n_kick_timed++;
+  }
   return MDX_OK;
 }
 
@@ -2054,7 +2050,7 @@ int mdx_forces(mdx_system *s, double *f3) {
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
   k_kick_tiles<1><<<tile_grid(s->ntiles), kTile, stage_bytes(s), s->stream>>>(
       s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), s->n, s->kcap, s->list16, s->nn,
-      8.0 * s->C2, 1.0, s->vec_out, nullptr);
+      make_pairk(s->C2, 1.0), 1.0, s->vec_out, nullptr);
   k_scatter_by_tag<<<nblk(s->n), kBlock, 0, s->stream>>>(s->vec_out, s->tag, s->n, s->io_buf);
   HIPCHK(s, hipMemcpyAsync(f3, s->io_buf, sizeof(double) * 3 * s->n, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(s, hipStreamSynchronize(s->stream));
@@ -2125,13 +2121,13 @@ int mdx_timing_reset(mdx_system *s, int enable) {
   if (!s) return MDX_ERR_ARG;
   s->timing = enable != 0;
   s->t_build = s->t_force = s->t_other = s->t_kick_only = 0;
-  s->n_build = s->n_steps = s->n_kick = 0;
+  s->n_build = s->n_steps = s->n_kick = s->n_kick_timed = 0;
   return MDX_OK;
 }
 
 int mdx_kick_profile(mdx_system *s, double *avg_ms, double *bytes) {
   if (!s) return MDX_ERR_ARG;
-  if (avg_ms) *avg_ms = s->n_steps > 0 && s->timing ? s->t_kick_only / (double)s->n_steps : 0.0;
+  if (avg_ms) *avg_ms = s->n_kick_timed > 0 ? s->t_kick_only / (double)s->n_kick_timed : 0.0;
   if (bytes) {
     // Algorithmic bytes of one fused kick + half-drift launch, per real:
     // read q and p, write p and q (24 B each), the partner count (4 B), and

@@ -71,12 +71,78 @@ struct TileSmem {
   int seg_base[kSeg + 1];
 };
 
+// Copy a tile's 18 segments (seg_base prefix sums filled) into LDS as
+// interleaved (x, y, z): staged particle u of segment k comes from
+// load(k, seg_start[k] + u - seg_base[k]).  The segments are walked as one
+// index space, eight items per thread at a time: the segment of each item is
+// found first (an unrolled binary search over the LDS prefix sums), then all
+// eight global loads are issued together, so their latencies overlap.  Every
+// thread calls it; it ends with a barrier.
+template <class Load>
+__device__ __forceinline__ void stage_segments(const TileSmem &sm, double *s3, Load load) {
+  constexpr int kBatch = 8;
+  const int total = sm.seg_base[kSeg];
+  for (int base = 0; base < total; base += kBatch * (int)blockDim.x) {
+    int seg[kBatch], slot[kBatch];
+#pragma unroll
+    for (int j = 0; j < kBatch; j++) {
+      const int u = base + j * (int)blockDim.x + (int)threadIdx.x;
+      int k = 0;
+#pragma unroll
+      for (int step = 16; step > 0; step >>= 1)
+        if (k + step < kSeg && sm.seg_base[k + step] <= u) k += step;
+      // items past the end load the first real of the tile's own row
+      // (segment 8, never empty) and are not stored: no branch between loads
+      const bool in = u < total;
+      seg[j] = in ? k : 8;
+      slot[j] = in ? sm.seg_start[k] + (u - sm.seg_base[k]) : sm.seg_start[8];
+    }
+    double4 q[kBatch];
+#pragma unroll
+    for (int j = 0; j < kBatch; j++) q[j] = load(seg[j], slot[j]);
+#pragma unroll
+    for (int j = 0; j < kBatch; j++) {
+      const int u = base + j * (int)blockDim.x + (int)threadIdx.x;
+      if (u < total) {
+        s3[3 * u] = q[j].x;
+        s3[3 * u + 1] = q[j].y;
+        s3[3 * u + 2] = q[j].z;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Order of the interior cell rows (y, z) in the tile array: strips of h
+// consecutive z, walked y-major with z fastest inside a strip, h = ceil(ms_z
+// / 8) so that each XCD's eighth of the tiles is about one strip.  The rows
+// a tile stages (y +- 1, z +- 1) then belong to tiles running on the same
+// XCD at about the same time, and come from its L2.  Row p -> (y, z), both
+// 1-based.
+__host__ __device__ inline int row_strip(const Geom &g) { return (g.ms[2] + 7) / 8; }
+__host__ __device__ inline void row_of(const Geom &g, int p, int *y, int *z) {
+  const int h = row_strip(g), ny = g.ms[1];
+  const int st = p / (h * ny), z0 = st * h;
+  const int hs = g.ms[2] - z0 < h ? g.ms[2] - z0 : h;
+  const int q = p - st * h * ny;
+  *y = q / hs + 1;
+  *z = z0 + q % hs + 1;
+}
+
+// Constants of the kick's pair term with the time step folded in.
+struct PairK {
+  double a, b, c;   // 24 dt, -48 dt, 8 C2 dt
+};
+inline PairK make_pairk(double C2, double dt) { return PairK{24.0 * dt, -48.0 * dt, 8.0 * C2 * dt}; }
+
 // Pair term of ForceCalculator::CalculateForceSorted (fcalculator.cc:203-217):
-// df = ((24 r^6 - 48) / (r^6 r^6 r^2) + 8 C2) dt, zero beyond the cutoff;
-// the quotient uses recip().
-__device__ __forceinline__ double pair_df(double r2, double C2_8, double dt, double CL2) {
-  const double r6 = r2 * r2 * r2;
-  const double df = ((24.0 * r6 - 48.0) * recip(r6 * r6 * r2) + C2_8) * dt;
+// df = ((24 r^6 - 48) / (r^6 r^6 r^2) + 8 C2) dt, zero beyond the cutoff,
+// evaluated as r^-8 (24 dt - 48 dt r^-6) + 8 C2 dt from one reciprocal of
+// r^2 (recip()): the same quantity in fewer dependent operations.
+__device__ __forceinline__ double pair_df(double r2, const PairK &k, double CL2) {
+  const double ir2 = recip(r2);
+  const double ir6 = ir2 * ir2 * ir2;
+  const double df = __fma_rn(ir6 * ir2, __fma_rn(k.b, ir6, k.a), k.c);
   return r2 > CL2 ? 0.0 : df;
 }
 
@@ -97,11 +163,11 @@ __device__ __forceinline__ double list_r2(double xj, double yj, double zj, doubl
 
 // one partner of the kick: a += df (q_j - q_i)   (fcalculator.cc:203-217)
 __device__ __forceinline__ void kick_term(double xj, double yj, double zj, double xi, double yi,
-                                          double zi, double C2_8, double dt, double CL2,
-                                          double &ax, double &ay, double &az) {
+                                          double zi, const PairK &k, double CL2, double &ax,
+                                          double &ay, double &az) {
   const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
   const double s2 = dx * dx + dy * dy + dz * dz;
-  const double df = pair_df(s2, C2_8, dt, CL2);
+  const double df = pair_df(s2, k, CL2);
   ax += df * dx;
   ay += df * dy;
   az += df * dz;
@@ -139,6 +205,44 @@ __device__ __forceinline__ void observe_term(double r2, double C0, double C2, do
   const double r6 = r2 * r2 * r2;
   pe += 4.0 * (1.0 / (r2 * r2 * r2 * r2 * r2 * r2) - 1.0 / r6 + C2 * r2 + C0);
   vir += ((24.0 * r6 - 48.0) / (r6 * r6 * r2) + C2 * 8.0) * r2;
+}
+
+// Sort one cell's distinct keys ascending in place, one wavefront per cell:
+// the keys go to LDS, and each key is written to its rank (the number of
+// smaller keys).  The wave's lanes run in lockstep, so the LDS copy is
+// complete before any lane reads it back; cells larger than the LDS buffer
+// (not met in practice) fall back to an insertion sort by one lane.
+constexpr int kSortBuf = 256;
+__device__ __forceinline__ void wave_rank_sort(int *seg, int len, int *buf) {
+  const int lane = threadIdx.x & 63;
+  if (len <= 1) return;
+  if (len > kSortBuf) {
+    if (lane == 0) {
+      for (int a = 1; a < len; a++) {
+        const int v = seg[a];
+        int b = a - 1;
+        while (b >= 0 && seg[b] > v) {
+          seg[b + 1] = seg[b];
+          b--;
+        }
+        seg[b + 1] = v;
+      }
+    }
+    return;
+  }
+  for (int a = lane; a < len; a += 64) buf[a] = seg[a];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int a = lane; a < len; a += 64) {
+    const int v = buf[a];
+    int rank = 0;
+    for (int b = 0; b < len; b++) rank += buf[b] < v ? 1 : 0;
+    seg[rank] = v;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 }  // namespace mdxk
