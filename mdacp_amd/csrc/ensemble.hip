@@ -234,11 +234,18 @@ __device__ void e_stage(const EnsDev &d, const Rep &rp, const TileDesc &t, const
     for (int k = 0; k < kSeg; k++) sm.seg_base[k + 1] += sm.seg_base[k];
   }
   __syncthreads();
-  stage_segments(sm, sx, [&](int k, int slot) {
-    if ((k & 1) == 0) return reals[slot];
-    if (mid) return image_of(mid[d.gsrc[slot - d.NR]], d.gcode[slot - d.NR], rp.g.L);
-    return d.qe[slot];
-  });
+  if (mid) {
+    // branch-free two-level gather (see the single-system kick)
+    stage_segments(sm, sx, [&](int k, int slot) {
+      const int idx = max(slot - d.NR, rp.ibase);
+      const int gs = d.gsrc[idx], gc = d.gcode[idx];
+      const bool img = (k & 1) != 0;
+      const double4 *src = img ? mid : reals;   // one load at a valid address
+      return image_of(src[img ? gs : slot], img ? gc : kCodeNone, rp.g.L);
+    });
+  } else {
+    stage_segments(sm, sx, [&](int k, int slot) { return (k & 1) ? d.qe[slot] : reals[slot]; });
+  }
   (void)sy;
   (void)sz;
 }

@@ -355,8 +355,12 @@ __global__ void k_gather_images(const int *__restrict__ order, int n, int g,
 // triples.  Neighbour lists hold 16-bit entries e = 3 x (staged index), so a
 // partner is three LDS loads at fixed offsets from one address.
 
+// per tile, written at the build: seg_start[18], seg_base[19] (prefix sums)
+constexpr int kTSeg = 40;
+
 struct Cells {
   const int *scan_in, *cnt_in, *scan_out, *cnt_out;
+  const int *tseg;   // [tile][kTSeg] segments of every tile
 };
 
 
@@ -384,26 +388,21 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles) {
 __host__ __device__ inline int tile_grid(int ntiles) { return ((ntiles + 7) >> 3) << 3; }
 
 
+// the tile's segments (precomputed at the build) into sm; every thread of
+// the block calls it
+__device__ __forceinline__ void tile_load_segments(const Cells &c, int tile, TileSmem &sm) {
+  const int *ts = c.tseg + (size_t)tile * kTSeg;
+  const int t = threadIdx.x;
+  if (t < kSeg) sm.seg_start[t] = ts[t];
+  else if (t < 2 * kSeg + 1) sm.seg_base[t - kSeg] = ts[t];
+  __syncthreads();
+}
+
 // segments and staging; every thread of the block calls it
-__device__ void tile_stage(const TileDesc &t, const Geom &g, const Cells &c, int n,
-                           const double4 *__restrict__ pos, TileSmem &sm, double *sx, double *sy,
-                           double *sz, int stride) {
-  if (threadIdx.x < kSeg) {
-    int st, ln;
-    tile_segment(g, c, n, t.y, t.z, t.xf, t.xl, threadIdx.x, &st, &ln);
-    sm.seg_start[threadIdx.x] = st;
-    sm.seg_base[threadIdx.x + 1] = ln;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    sm.seg_base[0] = 0;
-    for (int k = 0; k < kSeg; k++) sm.seg_base[k + 1] += sm.seg_base[k];
-  }
-  __syncthreads();
-  stage_segments(sm, sx, [&](int, int slot) { return pos[slot]; });
-  (void)sy;
-  (void)sz;
-  (void)stride;
+__device__ void tile_stage(int tile, const Cells &c, const double4 *__restrict__ pos, TileSmem &sm,
+                           double *s3) {
+  tile_load_segments(c, tile, sm);
+  stage_segments(sm, s3, [&](int, int slot) { return pos[slot]; });
 }
 
 // rows (in row_of order) -> number of tiles
@@ -420,7 +419,8 @@ __global__ void k_row_tiles(Geom g, Cells c, int *__restrict__ row_tiles) {
 
 __global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict__ pos,
                               const int *__restrict__ row_tiles, const int *__restrict__ row_off,
-                              TileDesc *__restrict__ tiles, double *__restrict__ partial) {
+                              TileDesc *__restrict__ tiles, int *__restrict__ tseg,
+                              double *__restrict__ partial) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int rows = g.ms[1] * g.ms[2];
   double smax = 0.0;
@@ -437,11 +437,15 @@ __global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict_
       t.z = z;
       t.xf = interior_coord(pos[t.i0].x, g.inv_w[0], g.ms[0]);
       t.xl = interior_coord(pos[t.i0 + t.ni - 1].x, g.inv_w[0], g.ms[0]);
+      int *ts = tseg + (size_t)(row_off[r] + k) * kTSeg;
       int stage = 0;
+      ts[kSeg] = 0;
       for (int sgi = 0; sgi < kSeg; sgi++) {
         int st, ln;
         tile_segment(g, c, n, y, z, t.xf, t.xl, sgi, &st, &ln);
+        ts[sgi] = st;
         stage += ln;
+        ts[kSeg + 1 + sgi] = stage;
       }
       t.stage = stage;
       t.pad = 0;
@@ -470,7 +474,7 @@ __global__ void __launch_bounds__(kTile) k_build_tiles(
   }
   const TileDesc t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-  tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
+  tile_stage(tile, c, pos, sm, sx);
   const int lane = threadIdx.x;
   int cnt = 0;
   if (lane < t.ni) {
@@ -614,13 +618,21 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
   }
   const TileDesc t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-  tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
   const int lane = threadIdx.x;
+  // the lane's own loads go out before the staging, which hides them
+  double4 pm = make_double4(0.0, 0.0, 0.0, 0.0);
+  int m = 0;
+  if (lane < t.ni) {
+    m = nn[t.i0 + lane];
+    if (MODE != 1) pm = mom[t.i0 + lane];
+  }
+  tile_stage(tile, c, pos, sm, sx);
   double v2 = 0.0;
   if (lane < t.ni) {
     const int i = t.i0 + lane;
-    const double4 qi = pos[i];
-    const int m = nn[i];
+    // own position from the staging (segment 8 holds the tile's row)
+    const int self = 3 * (sm.seg_base[8] + (i - sm.seg_start[8]));
+    const double4 qi = make_double4(sx[self], sy[self], sz[self], 0.0);
     const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
     // kILP independent partner chains per lane, summed in a fixed order;
     // the offsets of the next group are loaded while this group computes
@@ -676,7 +688,7 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
     if (MODE == 1) {
       out[i] = make_double4(fx, fy, fz, 0.0);
     } else {
-      double4 p = mom[i];
+      double4 p = pm;
       p.x += fx;
       p.y += fy;
       p.z += fz;
@@ -756,7 +768,7 @@ __global__ void __launch_bounds__(kTile) k_lj_observe_tiles(
   }
   const TileDesc t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-  tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
+  tile_stage(tile, c, pos, sm, sx);
   const int lane = threadIdx.x;
   double acc[K];
   for (int k = 0; k < K; k++) acc[k] = 0.0;
@@ -852,7 +864,7 @@ __global__ void __launch_bounds__(kTile) k_adj_hvp_tiles(
   const TileDesc t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
   int *sidx = reinterpret_cast<int *>(stage_mem + 3 * t.stage);   // slot of staged particle e / 3
-  tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
+  tile_stage(tile, c, pos, sm, sx);
   for (int k = 0; k < kSeg; k++) {
     const int b = sm.seg_base[k], len = sm.seg_base[k + 1] - b, st = sm.seg_start[k];
     for (int u = threadIdx.x; u < len; u += blockDim.x) sidx[b + u] = st + u;
@@ -932,7 +944,7 @@ __global__ void __launch_bounds__(kTile) k_observe_tiles(
   }
   const TileDesc t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-  tile_stage(t, g, c, n, pos, sm, sx, sy, sz, t.stage);
+  tile_stage(tile, c, pos, sm, sx);
   const int lane = threadIdx.x;
   double ke = 0.0, pe = 0.0, vir = 0.0;
   if (lane < t.ni) {
@@ -1048,18 +1060,7 @@ __global__ void __launch_bounds__(kTile) k_canonical_tiles(
   const int tile = blockIdx.x;
   if (tile >= ntiles) return;
   const TileDesc t = tiles[tile];
-  if (threadIdx.x < kSeg) {
-    int st, ln;
-    tile_segment(g, c, n, t.y, t.z, t.xf, t.xl, threadIdx.x, &st, &ln);
-    sm.seg_start[threadIdx.x] = st;
-    sm.seg_base[threadIdx.x + 1] = ln;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    sm.seg_base[0] = 0;
-    for (int k = 0; k < kSeg; k++) sm.seg_base[k + 1] += sm.seg_base[k];
-  }
-  __syncthreads();
+  tile_load_segments(c, tile, sm);
   const int lane = threadIdx.x;
   if (lane >= t.ni) return;
   const int i = t.i0 + lane;
@@ -1142,6 +1143,7 @@ struct mdx_system {
   int *nn = nullptr;
   // tiles and their 16-bit lists
   TileDesc *tiles = nullptr;
+  int *tseg = nullptr;        // [tiles_cap][kTSeg]
   int ntiles = 0, tiles_cap = 0, stage_max = 0;
   int *row_tiles = nullptr, *row_off = nullptr;
   unsigned short *list16 = nullptr;
@@ -1286,7 +1288,9 @@ int ensure_list_capacity(mdx_system *s, int kcap) {
 
 size_t stage_bytes(const mdx_system *s) { return (size_t)s->stage_max * 3 * sizeof(double); }
 
-Cells cells_of(const mdx_system *s) { return Cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out}; }
+Cells cells_of(const mdx_system *s) {
+  return Cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out, s->tseg};
+}
 
 // per-block partial buffers for every grid the library launches
 int ensure_partials(mdx_system *s, int blocks) {
@@ -1373,19 +1377,21 @@ int build_pairlist(mdx_system *s) {
                                                         s->gcode);
   if ((rc = launch_check(s, "sort"))) return rc;
   // 5. tiles over the interior cell rows
-  const Cells cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out};
+  Cells cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out, s->tseg};
   const int rows = s->geom.ms[1] * s->geom.ms[2];
   k_row_tiles<<<nblk(rows), kBlock, 0, s->stream>>>(s->geom, cells, s->row_tiles);
   if ((rc = scan(s, s->row_tiles, s->row_off, rows, &s->dstat->ntiles))) return rc;
   if ((rc = sync_status(s))) return rc;
   s->ntiles = s->hstat->ntiles;
   if (s->ntiles > s->tiles_cap) {
-    if ((rc = dev_alloc(s, &s->tiles, (size_t)s->ntiles + s->ntiles / 8 + 64))) return rc;
-    s->tiles_cap = s->ntiles + s->ntiles / 8 + 64;
+    const size_t cap = (size_t)s->ntiles + s->ntiles / 8 + 64;
+    if ((rc = dev_alloc(s, &s->tiles, cap)) || (rc = dev_alloc(s, &s->tseg, cap * kTSeg))) return rc;
+    s->tiles_cap = (int)cap;
   }
+  cells.tseg = s->tseg;
   if ((rc = ensure_partials(s, std::max(tile_grid(s->ntiles), nblk(rows))))) return rc;
   k_write_tiles<<<nblk(rows), kBlock, 0, s->stream>>>(s->geom, cells, n, s->pos, s->row_tiles,
-                                                       s->row_off, s->tiles, s->pmax);
+                                                       s->row_off, s->tiles, s->tseg, s->pmax);
   k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(rows), &s->dstat->stage_max);
   if ((rc = sync_status(s))) return rc;
   s->stage_max = (int)s->hstat->stage_max;
@@ -1891,7 +1897,7 @@ int mdx_create(const mdx_params *prm, mdx_system **out) {
 int mdx_destroy(mdx_system *s) {
   if (!s) return MDX_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  void *bufs[] = {s->pos, s->pos_alt, s->pmax, s->tiles, s->row_tiles, s->row_off, s->list16,
+  void *bufs[] = {s->pos, s->pos_alt, s->tseg, s->pmax, s->tiles, s->row_tiles, s->row_off, s->list16,
                   s->pos_tmp, s->ref_pos, s->mom, s->mom_tmp, s->gpos_tmp, s->vec_out,
                   s->tag, s->tag_tmp, s->gsrc, s->gcode, s->gsrc_tmp, s->gcode_tmp, s->cellid,
                   s->slot, s->order, s->inv, s->nimg, s->goff, s->cell_count, s->cell_start,

@@ -50,8 +50,10 @@ __device__ __forceinline__ int interior_coord(double x, double inv_w, int ms) {
 }
 
 __device__ __forceinline__ double shifted(double x, int s, double L) {
-  // identical rounding to the reference's image q + diff (mdunit.cc:183)
-  return s == 0 ? x : (s > 0 ? x + L : x + (-L));
+  // the reference's image q + diff (mdunit.cc:183), one rounding: s L is
+  // exactly 0 or +-L, so x + s L is x, x + L or x - L bit for bit (for any
+  // x but -0.0, which positions never are), without a branch
+  return __fma_rn((double)s, L, x);
 }
 
 constexpr int kTile = 128;
@@ -89,8 +91,11 @@ __device__ __forceinline__ void stage_segments(const TileSmem &sm, double *s3, L
       const int u = base + j * (int)blockDim.x + (int)threadIdx.x;
       int k = 0;
 #pragma unroll
-      for (int step = 16; step > 0; step >>= 1)
-        if (k + step < kSeg && sm.seg_base[k + step] <= u) k += step;
+      for (int step = 16; step > 0; step >>= 1) {
+        const int kk = k + step;
+        const bool up = kk < kSeg && sm.seg_base[kk < kSeg ? kk : kSeg] <= u;
+        k = up ? kk : k;
+      }
       // items past the end load the first real of the tile's own row
       // (segment 8, never empty) and are not stored: no branch between loads
       const bool in = u < total;
