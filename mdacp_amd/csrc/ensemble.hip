@@ -691,30 +691,30 @@ __global__ void __launch_bounds__(T) k_e_step(EnsDev d) {
       const int m = d.nn[i];
       const unsigned short *lst = d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * T + lane;
       const double CL2 = rp.g.CL2;
+      // entry k feeds chain k % kILP; the last group's entries past m add
+      // exactly +0.0: the sums of the single-system kick, bit for bit
       double ax[kILP], ay[kILP], az[kILP];
 #pragma unroll
       for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
-      const int mg = m - m % kILP;
       int nxt[kILP];
 #pragma unroll
-      for (int u = 0; u < kILP; u++) nxt[u] = u < mg ? lst[(size_t)u * T] : 0;
-      for (int k = 0; k < mg; k += kILP) {
+      for (int u = 0; u < kILP; u++)
+        nxt[u] = m > 0 ? __builtin_nontemporal_load(lst + (size_t)min(u, m - 1) * T) : 0;
+      for (int k = 0; k < m; k += kILP) {
         int cur[kILP];
 #pragma unroll
         for (int u = 0; u < kILP; u++) cur[u] = nxt[u];
-        if (k + kILP < mg) {
+        if (k + kILP < m) {
 #pragma unroll
-          for (int u = 0; u < kILP; u++) nxt[u] = lst[(size_t)(k + kILP + u) * T];
+          for (int u = 0; u < kILP; u++)
+            nxt[u] = __builtin_nontemporal_load(lst + (size_t)min(k + kILP + u, m - 1) * T);
         }
 #pragma unroll
         for (int u = 0; u < kILP; u++) {
-          const int o = cur[u];
-          kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, d.pk, CL2, ax[u], ay[u], az[u]);
+          const bool v = k + u < m;
+          const int o = v ? cur[u] : 0;
+          kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, d.pk, v ? CL2 : -1.0, ax[u], ay[u], az[u]);
         }
-      }
-      for (int k = mg; k < m; k++) {
-        const int o = lst[(size_t)k * T];
-        kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, d.pk, CL2, ax[0], ay[0], az[0]);
       }
       double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll

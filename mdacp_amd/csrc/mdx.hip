@@ -59,6 +59,7 @@ struct DevStatus {
   double dmax2;                    // max displacement^2 since the last build
   double kmax_d;                   // longest neighbour list (as a double)
   double stage_max;                // largest tile staging (particles)
+  double plane_max;                // largest staging of one plane of a tile
   long long canon_count;
   double sums[4];                  // KE, PE, virial, spare
 };
@@ -423,7 +424,7 @@ __global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict_
                               double *__restrict__ partial) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int rows = g.ms[1] * g.ms[2];
-  double smax = 0.0;
+  double smax = 0.0, pmax = 0.0;
   if (r < rows) {
     int y, z;
     row_of(g, r, &y, &z);
@@ -451,20 +452,26 @@ __global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict_
       t.pad = 0;
       tiles[row_off[r] + k] = t;
       smax = fmax(smax, (double)stage);
+      for (int p = 0; p < kPlanes; p++)
+        pmax = fmax(pmax, (double)(ts[kSeg + 6 * p + 6] - ts[kSeg + 6 * p]));
     }
   }
   block_max_to(smax, partial);
+  block_max_to(pmax, partial + gridDim.x);
 }
 
 // Full neighbour list per tile: lane = real i, every partner j != i (real
 // or image) of its 27 cells with r^2 <= SL^2, r^2 formed as the reference's
 // AVX2 search, fma(dx,dx, fma(dy,dy, dz*dz)) with dq = q_j - q_i
 // (meshlist.cc:354-362).  Entry k of lane l sits at
-// list[tile * kcap * kTile + k * kTile + l] and is the partner's LDS offset.
+// list[tile * kcap * kTile + k * kTile + l] and is 3 x the partner's index
+// in the tile's staging.  The cells are visited plane by plane (dz = -1, 0,
+// +1), each plane staged alone; nb[i] holds where planes 1 and 2 start in
+// the lane's list (low and high 16 bits).
 __global__ void __launch_bounds__(kTile) k_build_tiles(
     const double4 *__restrict__ pos, const TileDesc *__restrict__ tiles, int ntiles, Geom g,
     Cells c, int n, int kcap, unsigned short *__restrict__ list, int *__restrict__ nn,
-    double *__restrict__ partial, int *__restrict__ flags) {
+    int *__restrict__ nb, double *__restrict__ partial, int *__restrict__ flags) {
   extern __shared__ double stage_mem[];
   __shared__ TileSmem sm;
   const int tile = tile_of_block(blockIdx.x, ntiles);
@@ -474,39 +481,51 @@ __global__ void __launch_bounds__(kTile) k_build_tiles(
   }
   const TileDesc t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-  tile_stage(tile, c, pos, sm, sx);
   const int lane = threadIdx.x;
-  int cnt = 0;
-  if (lane < t.ni) {
-    const int i = t.i0 + lane;
-    const int self = sm.seg_base[8] + (i - sm.seg_start[8]);   // range (dy, dz) = (0, 0)
-    const double qx = sx[3 * self], qy = sy[3 * self], qz = sz[3 * self];
-    const int cx = interior_coord(qx, g.inv_w[0], g.ms[0]);
-    unsigned short *out = list + (size_t)tile * kcap * kTile + lane;
-    for (int r = 0; r < 9; r++) {
-      const int y = t.y + r % 3 - 1, z = t.z + r / 3 - 1;
-      for (int dx = -1; dx <= 1; dx++) {
-        const int x = cx + dx;
-        const int cl = cell_linear(g, x, y, z);
-        int lo, len;
-        if (is_outer(g, x, y, z)) {
-          lo = sm.seg_base[2 * r + 1] + (n + c.scan_out[cl] - sm.seg_start[2 * r + 1]);
-          len = c.cnt_out[cl];
-        } else {
-          lo = sm.seg_base[2 * r] + (c.scan_in[cl] - sm.seg_start[2 * r]);
-          len = c.cnt_in[cl];
-        }
-        for (int e = 3 * lo; e < 3 * (lo + len); e += 3) {
-          if (e == 3 * self) continue;
-          const double r2 = list_r2(sx[e], sy[e], sz[e], qx, qy, qz);
-          if (r2 <= g.SL2) {
-            if (cnt < kcap) out[(size_t)cnt * kTile] = (unsigned short)e;
-            cnt++;
+  const bool act = lane < t.ni;
+  const int i = t.i0 + lane;
+  const double4 qi = act ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+  tile_load_segments(c, tile, sm);
+  const int self = act ? sm.seg_base[8] + (i - sm.seg_start[8]) : -1;   // in row (0, 0)
+  const int cx = interior_coord(qi.x, g.inv_w[0], g.ms[0]);
+  unsigned short *out = list + (size_t)tile * kcap * kTile + lane;
+  int cnt = 0, b1 = 0, b2 = 0;
+  for (int p = 0; p < kPlanes; p++) {
+    if (p > 0) __syncthreads();   // the previous plane is read
+    stage_range(sm, 6 * p, 6 * p + 6, sx, [&](int, int slot) { return pos[slot]; });
+    const int pb = sm.seg_base[6 * p];
+    if (act) {
+      for (int r = 3 * p; r < 3 * p + 3; r++) {
+        const int y = t.y + r % 3 - 1, z = t.z + r / 3 - 1;
+        for (int dx = -1; dx <= 1; dx++) {
+          const int x = cx + dx;
+          const int cl = cell_linear(g, x, y, z);
+          int lo, len;
+          if (is_outer(g, x, y, z)) {
+            lo = sm.seg_base[2 * r + 1] + (n + c.scan_out[cl] - sm.seg_start[2 * r + 1]);
+            len = c.cnt_out[cl];
+          } else {
+            lo = sm.seg_base[2 * r] + (c.scan_in[cl] - sm.seg_start[2 * r]);
+            len = c.cnt_in[cl];
+          }
+          for (int o = lo; o < lo + len; o++) {
+            if (o == self) continue;
+            const int e = 3 * (o - pb);
+            const double r2 = list_r2(sx[e], sy[e], sz[e], qi.x, qi.y, qi.z);
+            if (r2 <= g.SL2) {
+              if (cnt < kcap) out[(size_t)cnt * kTile] = (unsigned short)(3 * o);
+              cnt++;
+            }
           }
         }
       }
     }
+    if (p == 0) b1 = cnt;
+    if (p == 1) b2 = cnt;
+  }
+  if (act) {
     nn[i] = cnt < kcap ? cnt : kcap;
+    nb[i] = min(b1, kcap) | (min(b2, kcap) << 16);
     if (cnt > kcap) atomicOr(flags, ST_LIST_OVERFLOW);
   }
   block_max_to((double)cnt, partial);
@@ -606,8 +625,9 @@ template <int MODE, bool SCALED = false>
 __global__ void __launch_bounds__(kTile) k_kick_tiles(
     const double4 *__restrict__ pos, double4 *__restrict__ mom, const TileDesc *__restrict__ tiles,
     int ntiles, Geom g, Cells c, int n, int kcap, const unsigned short *__restrict__ list,
-    const int *__restrict__ nn, PairK pk, double dt, double4 *__restrict__ out,
-    double *__restrict__ partial, LJ lj = LJ{}, const int *__restrict__ halt = nullptr) {
+    const int *__restrict__ nn, const int *__restrict__ nb, PairK pk, double dt,
+    double4 *__restrict__ out, double *__restrict__ partial, LJ lj = LJ{},
+    const int *__restrict__ halt = nullptr) {
   extern __shared__ double stage_mem[];
   __shared__ TileSmem sm;
   if (halt && *halt) return;
@@ -619,65 +639,67 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
   const TileDesc t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
   const int lane = threadIdx.x;
+  const bool act = lane < t.ni;
+  const int i = t.i0 + lane;
   // the lane's own loads go out before the staging, which hides them
-  double4 pm = make_double4(0.0, 0.0, 0.0, 0.0);
-  int m = 0;
-  if (lane < t.ni) {
-    m = nn[t.i0 + lane];
-    if (MODE != 1) pm = mom[t.i0 + lane];
+  double4 qi = make_double4(0.0, 0.0, 0.0, 0.0), pm = qi;
+  int m = 0, bnd = 0;
+  if (act) {
+    qi = pos[i];
+    m = nn[i];
+    bnd = nb[i];
+    if (MODE != 1) pm = mom[i];
   }
-  tile_stage(tile, c, pos, sm, sx);
-  double v2 = 0.0;
-  if (lane < t.ni) {
-    const int i = t.i0 + lane;
-    // own position from the staging (segment 8 holds the tile's row)
-    const int self = 3 * (sm.seg_base[8] + (i - sm.seg_start[8]));
-    const double4 qi = make_double4(sx[self], sy[self], sz[self], 0.0);
-    const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
-    // kILP independent partner chains per lane, summed in a fixed order;
-    // the offsets of the next group are loaded while this group computes
-    double ax[kILP], ay[kILP], az[kILP];
+  tile_load_segments(c, tile, sm);
+  const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
+  // kILP independent partner chains per lane: entry k of the list feeds
+  // chain k % kILP, and the chains are summed in a fixed order at the end
+  double ax[kILP], ay[kILP], az[kILP];
 #pragma unroll
-    for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
-    const int mg = m - m % kILP;
+  for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
+  for (int p = 0; p < kPlanes; p++) {
+    if (p > 0) __syncthreads();   // the previous plane is read
+    stage_range(sm, 6 * p, 6 * p + 6, sx, [&](int, int slot) { return pos[slot]; });
+    const int base3 = 3 * sm.seg_base[6 * p];
+    const int kb = p == 0 ? 0 : (p == 1 ? (bnd & 0xffff) : (bnd >> 16));
+    const int ke = p == 0 ? (bnd & 0xffff) : (p == 1 ? (bnd >> 16) : m);
+    if (!act || kb >= ke) continue;
+    // groups of kILP entries aligned on the whole list; entries of a group
+    // outside [kb, ke) add exactly +0.0 (the accumulators are never -0.0),
+    // so the sums are those of one pass over the list.  The entries of the
+    // next group are loaded while this group computes.
+    int g0 = kb - kb % kILP;
     int nxt[kILP];
 #pragma unroll
-    for (int u = 0; u < kILP; u++) nxt[u] = u < mg ? __builtin_nontemporal_load(lst + (size_t)u * kTile) : 0;
-    for (int k = 0; k < mg; k += kILP) {
+    for (int u = 0; u < kILP; u++)
+      nxt[u] = __builtin_nontemporal_load(lst + (size_t)min(g0 + u, m - 1) * kTile);
+    for (; g0 < ke; g0 += kILP) {
       int cur[kILP];
 #pragma unroll
       for (int u = 0; u < kILP; u++) cur[u] = nxt[u];
-      if (k + kILP < mg) {
+      if (g0 + kILP < ke) {
 #pragma unroll
         for (int u = 0; u < kILP; u++)
-          nxt[u] = __builtin_nontemporal_load(lst + (size_t)(k + kILP + u) * kTile);
+          nxt[u] = __builtin_nontemporal_load(lst + (size_t)min(g0 + kILP + u, m - 1) * kTile);
       }
 #pragma unroll
       for (int u = 0; u < kILP; u++) {
-        const int o = cur[u];
+        const bool v = g0 + u >= kb && g0 + u < ke;
+        const int o = v ? cur[u] - base3 : 0;
         if (SCALED) {
           const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
           const double df = pair_df_lj(dx * dx + dy * dy + dz * dz, lj, dt);
-          ax[u] += df * dx;
-          ay[u] += df * dy;
-          az[u] += df * dz;
+          ax[u] += (v ? df : 0.0) * dx;
+          ay[u] += (v ? df : 0.0) * dy;
+          az[u] += (v ? df : 0.0) * dz;
         } else {
-          kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, pk, g.CL2, ax[u], ay[u], az[u]);
+          kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, pk, v ? g.CL2 : -1.0, ax[u], ay[u], az[u]);
         }
       }
     }
-    for (int k = mg; k < m; k++) {
-      const int o = lst[(size_t)k * kTile];
-      if (SCALED) {
-        const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
-        const double df = pair_df_lj(dx * dx + dy * dy + dz * dz, lj, dt);
-        ax[0] += df * dx;
-        ay[0] += df * dy;
-        az[0] += df * dz;
-      } else {
-        kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, pk, g.CL2, ax[0], ay[0], az[0]);
-      }
-    }
+  }
+  double v2 = 0.0;
+  if (act) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll
     for (int u = 0; u < kILP; u++) {
@@ -1140,11 +1162,11 @@ struct mdx_system {
       *goff = nullptr;
   int *cell_count = nullptr, *cell_start = nullptr, *cnt_in = nullptr, *cnt_out = nullptr,
       *scan_in = nullptr, *scan_out = nullptr, *scan_aux = nullptr;
-  int *nn = nullptr;
+  int *nn = nullptr, *nb = nullptr;   // list length; plane starts in it
   // tiles and their 16-bit lists
   TileDesc *tiles = nullptr;
   int *tseg = nullptr;        // [tiles_cap][kTSeg]
-  int ntiles = 0, tiles_cap = 0, stage_max = 0;
+  int ntiles = 0, tiles_cap = 0, stage_max = 0, plane_max = 0;
   int *row_tiles = nullptr, *row_off = nullptr;
   unsigned short *list16 = nullptr;
   size_t list16_elems = 0;
@@ -1238,6 +1260,7 @@ int ensure_capacity(mdx_system *s, int n) {
       (rc = dev_alloc(s, &s->tag, cap)) || (rc = dev_alloc(s, &s->tag_tmp, cap)) ||
       (rc = dev_alloc(s, &s->inv, cap)) || (rc = dev_alloc(s, &s->nimg, cap)) ||
       (rc = dev_alloc(s, &s->goff, cap)) || (rc = dev_alloc(s, &s->nn, cap)) ||
+      (rc = dev_alloc(s, &s->nb, cap)) ||
       (rc = dev_alloc(s, &s->io_buf, 3 * (size_t)cap)) || (rc = dev_alloc(s, &s->i_buf, cap)))
     return rc;
   s->cap = cap;
@@ -1287,6 +1310,7 @@ int ensure_list_capacity(mdx_system *s, int kcap) {
 }
 
 size_t stage_bytes(const mdx_system *s) { return (size_t)s->stage_max * 3 * sizeof(double); }
+size_t plane_bytes(const mdx_system *s) { return (size_t)s->plane_max * 3 * sizeof(double); }
 
 Cells cells_of(const mdx_system *s) {
   return Cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out, s->tseg};
@@ -1389,12 +1413,14 @@ int build_pairlist(mdx_system *s) {
     s->tiles_cap = (int)cap;
   }
   cells.tseg = s->tseg;
-  if ((rc = ensure_partials(s, std::max(tile_grid(s->ntiles), nblk(rows))))) return rc;
+  if ((rc = ensure_partials(s, std::max(tile_grid(s->ntiles), 2 * nblk(rows))))) return rc;
   k_write_tiles<<<nblk(rows), kBlock, 0, s->stream>>>(s->geom, cells, n, s->pos, s->row_tiles,
                                                        s->row_off, s->tiles, s->tseg, s->pmax);
   k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(rows), &s->dstat->stage_max);
+  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax + nblk(rows), nblk(rows), &s->dstat->plane_max);
   if ((rc = sync_status(s))) return rc;
   s->stage_max = (int)s->hstat->stage_max;
+  s->plane_max = (int)s->hstat->plane_max;
   if ((size_t)s->stage_max * 3 * sizeof(double) + 4096 > s->lds_limit)
     return fail(s, MDX_ERR_CAPACITY, "tile staging of %d particles exceeds LDS (%zu B)",
                 s->stage_max, s->lds_limit);
@@ -1402,8 +1428,8 @@ int build_pairlist(mdx_system *s) {
   // 6. neighbour lists, one pass at the current width, again if it overflows
   for (int attempt = 0; attempt < 3; attempt++) {
     HIPCHK(s, hipMemsetAsync(&s->dstat->flags, 0, sizeof(int), s->stream));
-    k_build_tiles<<<tile_grid(s->ntiles), kTile, stage_bytes(s), s->stream>>>(
-        s->pos, s->tiles, s->ntiles, s->geom, cells, n, s->kcap, s->list16, s->nn, s->pmax,
+    k_build_tiles<<<tile_grid(s->ntiles), kTile, plane_bytes(s), s->stream>>>(
+        s->pos, s->tiles, s->ntiles, s->geom, cells, n, s->kcap, s->list16, s->nn, s->nb, s->pmax,
         &s->dstat->flags);
     k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, tile_grid(s->ntiles), &s->dstat->kmax_d);
     if ((rc = launch_check(s, "list"))) return rc;
@@ -1428,9 +1454,9 @@ int build_pairlist(mdx_system *s) {
 int kick(mdx_system *s) {
   const int grid = tile_grid(s->ntiles);
   ev_record(s, 2);
-  k_kick_tiles<0><<<grid, kTile, stage_bytes(s), s->stream>>>(
+  k_kick_tiles<0><<<grid, kTile, plane_bytes(s), s->stream>>>(
       s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), s->n, s->kcap, s->list16, s->nn,
-      make_pairk(s->C2, s->prm.dt), s->prm.dt, nullptr, s->pmax);
+      s->nb, make_pairk(s->C2, s->prm.dt), s->prm.dt, nullptr, s->pmax);
   ev_record(s, 3);
   s->vmax_valid = false;   // recomputed from p when the expiry check needs it
   s->n_kick++;
@@ -1462,9 +1488,10 @@ int run_batch(mdx_system *s, int want, int *ran) {
                                                             s->geom.L[1], s->geom.L[2], s->pos_alt,
                                                             &s->dctl->halt);
     if (s->timing) (void)hipEventRecord(s->evk[2 * b], s->stream);
-    k_kick_tiles<2><<<grid, kTile, stage_bytes(s), s->stream>>>(
+    k_kick_tiles<2><<<grid, kTile, plane_bytes(s), s->stream>>>(
         s->pos_alt, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16,
-        s->nn, make_pairk(s->C2, s->prm.dt), s->prm.dt, s->pos, s->pmax, LJ{}, &s->dctl->halt);
+        s->nn, s->nb, make_pairk(s->C2, s->prm.dt), s->prm.dt, s->pos, s->pmax, LJ{},
+        &s->dctl->halt);
     if (s->timing) (void)hipEventRecord(s->evk[2 * b + 1], s->stream);
     k_step_guard<<<1, kBlock, 0, s->stream>>>(s->pmax, grid, s->dctl);
   }
@@ -1615,8 +1642,8 @@ int lj_step(mdx_system *s, const LJ &lj, double4 *mid) {
   k_drift_refresh<<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->gsrc, s->gcode, n, g,
                                                           s->prm.dt * 0.5, s->geom.L[0],
                                                           s->geom.L[1], s->geom.L[2], mid, nullptr);
-  k_kick_tiles<2, true><<<grid, kTile, stage_bytes(s), s->stream>>>(
-      mid, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn,
+  k_kick_tiles<2, true><<<grid, kTile, plane_bytes(s), s->stream>>>(
+      mid, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb,
       make_pairk(s->C2, s->prm.dt), s->prm.dt, s->pos, s->pmax, lj);
   return launch_check(s, "lj step");
 }
@@ -2054,9 +2081,9 @@ int mdx_observe(mdx_system *s, double *ke, double *pe, double *vir) {
 int mdx_forces(mdx_system *s, double *f3) {
   if (!s || !f3) return fail(s, MDX_ERR_ARG, "null argument");
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
-  k_kick_tiles<1><<<tile_grid(s->ntiles), kTile, stage_bytes(s), s->stream>>>(
+  k_kick_tiles<1><<<tile_grid(s->ntiles), kTile, plane_bytes(s), s->stream>>>(
       s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), s->n, s->kcap, s->list16, s->nn,
-      make_pairk(s->C2, 1.0), 1.0, s->vec_out, nullptr);
+      s->nb, make_pairk(s->C2, 1.0), 1.0, s->vec_out, nullptr);
   k_scatter_by_tag<<<nblk(s->n), kBlock, 0, s->stream>>>(s->vec_out, s->tag, s->n, s->io_buf);
   HIPCHK(s, hipMemcpyAsync(f3, s->io_buf, sizeof(double) * 3 * s->n, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(s, hipStreamSynchronize(s->stream));

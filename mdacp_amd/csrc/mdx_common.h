@@ -73,34 +73,36 @@ struct TileSmem {
   int seg_base[kSeg + 1];
 };
 
-// Copy a tile's 18 segments (seg_base prefix sums filled) into LDS as
-// interleaved (x, y, z): staged particle u of segment k comes from
-// load(k, seg_start[k] + u - seg_base[k]).  The segments are walked as one
-// index space, eight items per thread at a time: the segment of each item is
-// found first (an unrolled binary search over the LDS prefix sums), then all
-// eight global loads are issued together, so their latencies overlap.  Every
-// thread calls it; it ends with a barrier.
+// Copy segments [k0, k1) of a tile (seg_base prefix sums filled) into LDS
+// as interleaved (x, y, z), positions relative to seg_base[k0]: staged
+// particle a of segment k comes from load(k, seg_start[k] + a - seg_base[k]).
+// The segments are walked as one index space, eight items per thread at a
+// time: the segment of each item is found first (an unrolled binary search
+// over the LDS prefix sums), then all eight global loads are issued
+// together, so their latencies overlap.  Items past the end load the first
+// real of the tile's own row (segment 8, never empty) and are not stored,
+// so no branch separates the loads.  Every thread calls it; it ends with a
+// barrier.
 template <class Load>
-__device__ __forceinline__ void stage_segments(const TileSmem &sm, double *s3, Load load) {
+__device__ __forceinline__ void stage_range(const TileSmem &sm, int k0, int k1, double *s3, Load load) {
   constexpr int kBatch = 8;
-  const int total = sm.seg_base[kSeg];
+  const int b0 = sm.seg_base[k0], total = sm.seg_base[k1] - b0;
   for (int base = 0; base < total; base += kBatch * (int)blockDim.x) {
     int seg[kBatch], slot[kBatch];
 #pragma unroll
     for (int j = 0; j < kBatch; j++) {
       const int u = base + j * (int)blockDim.x + (int)threadIdx.x;
-      int k = 0;
+      const int a = b0 + u;
+      int k = k0;
 #pragma unroll
       for (int step = 16; step > 0; step >>= 1) {
         const int kk = k + step;
-        const bool up = kk < kSeg && sm.seg_base[kk < kSeg ? kk : kSeg] <= u;
+        const bool up = kk < k1 && sm.seg_base[kk < kSeg ? kk : kSeg] <= a;
         k = up ? kk : k;
       }
-      // items past the end load the first real of the tile's own row
-      // (segment 8, never empty) and are not stored: no branch between loads
       const bool in = u < total;
       seg[j] = in ? k : 8;
-      slot[j] = in ? sm.seg_start[k] + (u - sm.seg_base[k]) : sm.seg_start[8];
+      slot[j] = in ? sm.seg_start[k] + (a - sm.seg_base[k]) : sm.seg_start[8];
     }
     double4 q[kBatch];
 #pragma unroll
@@ -117,6 +119,16 @@ __device__ __forceinline__ void stage_segments(const TileSmem &sm, double *s3, L
   }
   __syncthreads();
 }
+
+template <class Load>
+__device__ __forceinline__ void stage_segments(const TileSmem &sm, double *s3, Load load) {
+  stage_range(sm, 0, kSeg, s3, load);
+}
+
+// A tile's partners lie in three z-planes of cell rows (dz = -1, 0, +1):
+// segments 6p .. 6p+5, list rows 3p .. 3p+2.  The kick and the list build
+// stage one plane at a time, a third of the LDS of the whole tile.
+constexpr int kPlanes = 3;
 
 // Order of the interior cell rows (y, z) in the tile array: strips of h
 // consecutive z, walked y-major with z fastest inside a strip, h = ceil(ms_z
