@@ -76,16 +76,16 @@ struct TileSmem {
 // Copy segments [k0, k1) of a tile (seg_base prefix sums filled) into LDS
 // as interleaved (x, y, z), positions relative to seg_base[k0]: staged
 // particle a of segment k comes from load(k, seg_start[k] + a - seg_base[k]).
-// The segments are walked as one index space, eight items per thread at a
+// The segments are walked as one index space, four items per thread at a
 // time: the segment of each item is found first (an unrolled binary search
-// over the LDS prefix sums), then all eight global loads are issued
+// over the LDS prefix sums), then all four global loads are issued
 // together, so their latencies overlap.  Items past the end load the first
 // real of the tile's own row (segment 8, never empty) and are not stored,
 // so no branch separates the loads.  Every thread calls it; it ends with a
 // barrier.
 template <class Load>
 __device__ __forceinline__ void stage_range(const TileSmem &sm, int k0, int k1, double *s3, Load load) {
-  constexpr int kBatch = 8;
+  constexpr int kBatch = 4;
   const int b0 = sm.seg_base[k0], total = sm.seg_base[k1] - b0;
   for (int base = 0; base < total; base += kBatch * (int)blockDim.x) {
     int seg[kBatch], slot[kBatch];
