@@ -620,6 +620,8 @@ __device__ __forceinline__ void lj_terms(double s, const LJ &p, double *phi, dou
 //   MODE 1  out[i] = F (dt = 1); p untouched
 //   MODE 2  MODE 0, plus the trailing half drift out of place:
 //           out[i] = q_i + p_i dt/2 (fcalculator.cc:26-28)
+//   MODE 3  MODE 2, plus the next step's leading half drift of the reals:
+//           qn[i] = out[i] + p_i dt/2
 //   SCALED  the pair term of the differentiable form (pair_df_lj)
 template <int MODE, bool SCALED = false>
 __global__ void __launch_bounds__(kTile) k_kick_tiles(
@@ -627,7 +629,7 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
     int ntiles, Geom g, Cells c, int n, int kcap, const unsigned short *__restrict__ list,
     const int *__restrict__ nn, const int *__restrict__ nb, PairK pk, double dt,
     double4 *__restrict__ out, double *__restrict__ partial, LJ lj = LJ{},
-    const int *__restrict__ halt = nullptr) {
+    const int *__restrict__ halt = nullptr, double4 *__restrict__ qn = nullptr) {
   extern __shared__ double stage_mem[];
   __shared__ TileSmem sm;
   if (halt && *halt) return;
@@ -717,9 +719,37 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
       mom[i] = p;
       v2 = speed2(p);
       if (MODE == 2) out[i] = drift_half(qi, p, dt * 0.5);
+      if (MODE == 3) {
+        const double4 q = drift_half(qi, p, dt * 0.5);
+        out[i] = q;
+        qn[i] = drift_half(q, p, dt * 0.5);
+      }
     }
   }
   if (MODE != 1) block_max_to(v2, partial);
+}
+
+// images of a mid-step buffer from its reals (the refresh of
+// mdunit.cc:181-191 at the next step's mid-step positions)
+__global__ void k_refresh_mid(double4 *__restrict__ mid, const int *__restrict__ gsrc,
+                              const int *__restrict__ gcode, int n, int g, Geom geom,
+                              const int *__restrict__ halt) {
+  if (*halt) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g) return;
+  mid[n + k] = image_of(mid[gsrc[k]], gcode[k], geom.L);
+}
+
+// After a batch: the images of pos at the mid-step refresh of its last
+// step, which read mid[(steps_done - 1) & 1] (the reference's ghosts keep
+// their last refresh until the next one).
+__global__ void k_images_from_mid(double4 *__restrict__ pos, const double4 *__restrict__ mid0,
+                                  const double4 *__restrict__ mid1, int n, int g,
+                                  const StepCtl *__restrict__ ctl) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g) return;
+  const double4 *mid = ((ctl->steps_done - 1) & 1) ? mid1 : mid0;
+  pos[n + k] = mid[n + k];
 }
 
 __global__ void k_vmax(const double4 *__restrict__ mom, int n, double *__restrict__ partial) {
@@ -1154,7 +1184,7 @@ struct mdx_system {
   int n = 0, g = 0, cap = 0, gcap = 0, kcap = 0;
   size_t tot_elems = 0;     // length of pos / ref_pos / cellid / ... arrays
   // device buffers
-  double4 *pos = nullptr, *pos_alt = nullptr, *pos_tmp = nullptr, *ref_pos = nullptr, *mom = nullptr,
+  double4 *pos = nullptr, *pos_alt = nullptr, *pos_mid = nullptr, *pos_tmp = nullptr, *ref_pos = nullptr, *mom = nullptr,
           *mom_tmp = nullptr, *gpos_tmp = nullptr, *vec_out = nullptr;
   int *tag = nullptr, *tag_tmp = nullptr, *gsrc = nullptr, *gcode = nullptr, *gsrc_tmp = nullptr,
       *gcode_tmp = nullptr;
@@ -1287,6 +1317,7 @@ int ensure_image_capacity(mdx_system *s, int gcap) {
   int rc;
   if ((rc = ensure_partials(s, nblk((long long)tot))) ||
       (rc = dev_alloc(s, &s->ref_pos, tot)) || (rc = dev_alloc(s, &s->pos_alt, tot)) ||
+      (rc = dev_alloc(s, &s->pos_mid, tot)) ||
       (rc = dev_alloc(s, &s->gpos_tmp, gcap)) || (rc = dev_alloc(s, &s->gsrc, gcap)) ||
       (rc = dev_alloc(s, &s->gcode, gcap)) || (rc = dev_alloc(s, &s->gsrc_tmp, gcap)) ||
       (rc = dev_alloc(s, &s->gcode_tmp, gcap)) || (rc = dev_alloc(s, &s->cellid, tot)) ||
@@ -1464,37 +1495,48 @@ int kick(mdx_system *s) {
 }
 
 // Up to `want` NVE steps after the expiry check of the first, queued
-// without a host round trip: per step, drift + image refresh into pos_alt
-// (images also written back to pos), kick + trailing drift from pos_alt into
-// the reals of pos, and the budget guard.  pos ends as the reference's state
-// after Calculate(): reals at the end of the step, images at their mid-step
-// refresh.  Same arithmetic as drift(); refresh(); kick(); drift().
+// without a host round trip.  The first step's drift + image refresh goes
+// into pos_alt (images also written back to pos); then per step the kick
+// stages the current mid-step buffer (pos_alt / pos_mid, reals and images),
+// writes the reals' end-of-step positions into pos and the next step's
+// mid-step reals into the other buffer, whose images are then refreshed,
+// and the guard checks the budget.  After the batch the images of pos get
+// the mid-step refresh of its last step, so pos ends as the reference's
+// state after Calculate(): reals at the end of the step, images at their
+// mid-step refresh.  Same arithmetic as drift(); refresh(); kick(); drift().
 // *ran receives the number of steps the device completed (>= 1).
 int run_batch(mdx_system *s, int want, int *ran) {
   const int n = s->n, g = s->g, grid = tile_grid(s->ntiles);
   const int B = std::max(1, std::min(want, kRing));
+  const double dt = s->prm.dt;
   StepCtl *h = s->hctl;
   h->budget = s->buffer_length;
-  h->dt = s->prm.dt;
+  h->dt = dt;
   h->buffer = s->prm.buffer_length;
   h->halt = 0;
   h->steps_done = 0;
   h->too_fast = 0;
   HIPCHK(s, hipMemcpyAsync(s->dctl, h, offsetof(StepCtl, ring), hipMemcpyHostToDevice, s->stream));
   ev_record(s, 4);
+  k_drift_refresh<<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->gsrc, s->gcode, n, g,
+                                                          dt * 0.5, s->geom.L[0], s->geom.L[1],
+                                                          s->geom.L[2], s->pos_alt, &s->dctl->halt);
   for (int b = 0; b < B; b++) {
-    k_drift_refresh<<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->gsrc, s->gcode, n, g,
-                                                            s->prm.dt * 0.5, s->geom.L[0],
-                                                            s->geom.L[1], s->geom.L[2], s->pos_alt,
-                                                            &s->dctl->halt);
+    const double4 *cur = (b & 1) ? s->pos_mid : s->pos_alt;
+    double4 *nxt = (b & 1) ? s->pos_alt : s->pos_mid;
     if (s->timing) (void)hipEventRecord(s->evk[2 * b], s->stream);
-    k_kick_tiles<2><<<grid, kTile, plane_bytes(s), s->stream>>>(
-        s->pos_alt, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16,
-        s->nn, s->nb, make_pairk(s->C2, s->prm.dt), s->prm.dt, s->pos, s->pmax, LJ{},
-        &s->dctl->halt);
+    k_kick_tiles<3><<<grid, kTile, plane_bytes(s), s->stream>>>(
+        cur, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn,
+        s->nb, make_pairk(s->C2, dt), dt, s->pos, s->pmax, LJ{}, &s->dctl->halt, nxt);
     if (s->timing) (void)hipEventRecord(s->evk[2 * b + 1], s->stream);
     k_step_guard<<<1, kBlock, 0, s->stream>>>(s->pmax, grid, s->dctl);
+    if (g > 0 && b + 1 < B)
+      k_refresh_mid<<<nblk(g), kBlock, 0, s->stream>>>(nxt, s->gsrc, s->gcode, n, g, s->geom,
+                                                        &s->dctl->halt);
   }
+  if (g > 0)
+    k_images_from_mid<<<nblk(g), kBlock, 0, s->stream>>>(s->pos, s->pos_alt, s->pos_mid, n, g,
+                                                          s->dctl);
   ev_record(s, 5);
   int rc = launch_check(s, "step");
   if (rc) return rc;
@@ -1924,7 +1966,7 @@ int mdx_create(const mdx_params *prm, mdx_system **out) {
 int mdx_destroy(mdx_system *s) {
   if (!s) return MDX_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  void *bufs[] = {s->pos, s->pos_alt, s->tseg, s->pmax, s->tiles, s->row_tiles, s->row_off, s->list16,
+  void *bufs[] = {s->pos, s->pos_alt, s->pos_mid, s->tseg, s->pmax, s->tiles, s->row_tiles, s->row_off, s->list16,
                   s->pos_tmp, s->ref_pos, s->mom, s->mom_tmp, s->gpos_tmp, s->vec_out,
                   s->tag, s->tag_tmp, s->gsrc, s->gcode, s->gsrc_tmp, s->gcode_tmp, s->cellid,
                   s->slot, s->order, s->inv, s->nimg, s->goff, s->cell_count, s->cell_start,
