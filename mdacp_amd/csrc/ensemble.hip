@@ -699,7 +699,7 @@ __global__ void __launch_bounds__(T) k_e_step(EnsDev d) {
       int nxt[kILP];
 #pragma unroll
       for (int u = 0; u < kILP; u++)
-        nxt[u] = m > 0 ? __builtin_nontemporal_load(lst + (size_t)min(u, m - 1) * T) : 0;
+        nxt[u] = m > 0 ? lst[(size_t)min(u, m - 1) * T] : 0;
       for (int k = 0; k < m; k += kILP) {
         int cur[kILP];
 #pragma unroll
@@ -707,7 +707,7 @@ __global__ void __launch_bounds__(T) k_e_step(EnsDev d) {
         if (k + kILP < m) {
 #pragma unroll
           for (int u = 0; u < kILP; u++)
-            nxt[u] = __builtin_nontemporal_load(lst + (size_t)min(k + kILP + u, m - 1) * T);
+            nxt[u] = lst[(size_t)min(k + kILP + u, m - 1) * T];
         }
 #pragma unroll
         for (int u = 0; u < kILP; u++) {

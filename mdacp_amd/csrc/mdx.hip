@@ -674,7 +674,7 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
     int nxt[kILP];
 #pragma unroll
     for (int u = 0; u < kILP; u++)
-      nxt[u] = __builtin_nontemporal_load(lst + (size_t)min(g0 + u, m - 1) * kTile);
+      nxt[u] = lst[(size_t)min(g0 + u, m - 1) * kTile];
     for (; g0 < ke; g0 += kILP) {
       int cur[kILP];
 #pragma unroll
@@ -682,7 +682,7 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
       if (g0 + kILP < ke) {
 #pragma unroll
         for (int u = 0; u < kILP; u++)
-          nxt[u] = __builtin_nontemporal_load(lst + (size_t)min(g0 + kILP + u, m - 1) * kTile);
+          nxt[u] = lst[(size_t)min(g0 + kILP + u, m - 1) * kTile];
       }
 #pragma unroll
       for (int u = 0; u < kILP; u++) {
