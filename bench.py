@@ -189,6 +189,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--only", choices=("batched", "adjoint"),
+                    help="run one form alone and print its object (for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES between two device buffers (counter calibration)")
     args = ap.parse_args()
@@ -199,6 +201,12 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the multi-GPU sharding is a design, not code)", file=sys.stderr)
         sys.exit(2)
+    if args.only == "batched":
+        print(json.dumps({"batched": batched(args.batched_steps, args.batched_steps)}), flush=True)
+        return
+    if args.only == "adjoint":
+        print(json.dumps({"adjoint": adjoint()}), flush=True)
+        return
 
     import torch
     from mdacp_amd import workloads as W
@@ -240,7 +248,7 @@ def main():
                      "other_step_kernels_total": tm["other_ms"], "builds": tm["builds"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_kick_tiles<2>", "avg_launch_ms": kick_ms,
+                     "kernel": "k_kick_tiles<3>", "avg_launch_ms": kick_ms,
                      "algorithmic_bytes_per_launch": kick_bytes},
     }
     sys_.close()

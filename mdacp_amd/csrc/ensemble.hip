@@ -312,7 +312,7 @@ __global__ void k_e_wrap(EnsDev d) {
       d.tag_tmp[i] = d.tag[i];
       const int cl = rp.cbase + cell_linear(g, c[0], c[1], c[2]);
       d.cellid[i] = cl;
-      d.slot[i] = atomicAdd(&d.cell_count[cl], 1);
+      d.slot[i] = wave_grouped_add(d.cell_count, cl);
       d.nimg[i] = cnt - 1;
     }
   }
@@ -380,7 +380,7 @@ __global__ void k_e_images(EnsDev d) {
             const int gz = s[2] > 0 ? g.m[2] - 1 : (s[2] < 0 ? 0 : cz);
             const int gcl = rp.cbase + cell_linear(g, gx, gy, gz);
             d.cellid[d.NR + idx] = gcl;
-            d.slot[d.NR + idx] = atomicAdd(&d.cell_count[gcl], 1);
+            d.slot[d.NR + idx] = wave_grouped_add(d.cell_count, gcl);
             idx++;
           }
     }

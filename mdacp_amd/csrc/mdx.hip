@@ -108,14 +108,20 @@ struct StepCtl {
   double ring[kRing];
 };
 
-__global__ void k_step_guard(const double *__restrict__ partial, int nb, StepCtl *__restrict__ ctl) {
-  __shared__ double lds[kBlock];
+constexpr int kGuard = 1024;   // threads of the guard: four independent loads each per round
+__global__ void __launch_bounds__(kGuard) k_step_guard(const double *__restrict__ partial, int nb,
+                                                       StepCtl *__restrict__ ctl) {
+  __shared__ double lds[kGuard];
   if (ctl->halt) return;
-  double m = 0.0;
-  for (int b = threadIdx.x; b < nb; b += kBlock) m = fmax(m, partial[b]);
-  lds[threadIdx.x] = m;
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += 4 * kGuard) {
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (b + u * kGuard < nb) m[u] = fmax(m[u], partial[b + u * kGuard]);
+  }
+  lds[threadIdx.x] = fmax(fmax(m[0], m[1]), fmax(m[2], m[3]));
   __syncthreads();
-  for (int off = kBlock / 2; off > 0; off >>= 1) {
+  for (int off = kGuard / 2; off > 0; off >>= 1) {
     if (threadIdx.x < off) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + off]);
     __syncthreads();
   }
@@ -227,7 +233,7 @@ __global__ void k_wrap_bin_count(double4 *__restrict__ pos, int n, Geom g, doubl
   pos[i] = q;
   const int cl = cell_linear(g, c[0], c[1], c[2]);
   cellid[i] = cl;
-  slot[i] = atomicAdd(&cell_count[cl], 1);
+  slot[i] = wave_grouped_add(cell_count, cl);
   nimg[i] = cnt - 1;
 }
 
@@ -275,7 +281,7 @@ __global__ void k_make_images(const double4 *__restrict__ pos, const int *__rest
         const int gz = s[2] > 0 ? g.m[2] - 1 : (s[2] < 0 ? 0 : cz);
         const int gcl = cell_linear(g, gx, gy, gz);
         gcell[k] = gcl;
-        gslot[k] = atomicAdd(&cell_count[gcl], 1);
+        gslot[k] = wave_grouped_add(cell_count, gcl);
         k++;
       }
 }
@@ -1529,7 +1535,7 @@ int run_batch(mdx_system *s, int want, int *ran) {
         cur, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn,
         s->nb, make_pairk(s->C2, dt), dt, s->pos, s->pmax, LJ{}, &s->dctl->halt, nxt);
     if (s->timing) (void)hipEventRecord(s->evk[2 * b + 1], s->stream);
-    k_step_guard<<<1, kBlock, 0, s->stream>>>(s->pmax, grid, s->dctl);
+    k_step_guard<<<1, kGuard, 0, s->stream>>>(s->pmax, grid, s->dctl);
     if (g > 0 && b + 1 < B)
       k_refresh_mid<<<nblk(g), kBlock, 0, s->stream>>>(nxt, s->gsrc, s->gcode, n, g, s->geom,
                                                         &s->dctl->halt);

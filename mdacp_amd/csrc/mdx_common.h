@@ -262,6 +262,27 @@ __device__ __forceinline__ void wave_rank_sort(int *seg, int len, int *buf) {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Slot of this lane in counter[key], with one atomicAdd per distinct key of
+// the wavefront: lanes that share a key take consecutive slots in lane
+// order.  Every active lane of the wavefront calls it (it is safe under
+// divergence: the ballots see the active lanes only).
+__device__ __forceinline__ int wave_grouped_add(int *counter, int key) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(1);
+  int slot = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lkey = __shfl(key, leader);
+    const unsigned long long same = __ballot(key == lkey) & todo;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter + lkey, __popcll(same));
+    base = __shfl(base, leader);
+    if (key == lkey) slot = base + __popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
+  return slot;
+}
+
 }  // namespace mdxk
 
 #endif
