@@ -94,7 +94,7 @@ struct EnsDev {
   int *checklist; // replicas that run a displacement check after this step
   int R, NR, NI, NT;
   int T;          // tile size (reals per tile)
-  int stage_cap;  // particles a tile may stage (dynamic LDS / 24 B)
+  int stage_cap;  // particles one plane of a tile may stage (dynamic LDS / 24 B)
   int check_list_length;
   double dt, buffer, SL;
   PairK pk;       // pair-term constants with dt folded in
@@ -110,6 +110,7 @@ struct EnsDev {
   TileDesc *tiles;
   unsigned short *list;
   int *nn;
+  int *nb;              // [NR] where planes 1 and 2 start in each list (16 + 16 bits)
   double *partial;      // [NT] max |p|^2 per tile
   double *disp;         // [NR + NI] squared displacement since the build
   double *dpart;        // [R][kRebuildBlocks] block maxima of disp
@@ -214,14 +215,9 @@ __device__ __forceinline__ void e_segment(const EnsDev &d, const Rep &rp, int y,
   }
 }
 
-// Stage the 18 segments of tile t into LDS as interleaved (x, y, z) triples
-// (list entries are 3 x the staged index).  Reals come
-// from `reals`; images from the positions stored at the build (mid ==
-// nullptr) or as their source's mid-step position plus the shift, which is
-// the image refresh of MDUnit::MakeBufferForBorderParticles
-// (mdunit.cc:181-191) done while staging.  Every thread calls it.
-__device__ void e_stage(const EnsDev &d, const Rep &rp, const TileDesc &t, const double4 *reals,
-                        const double4 *mid, TileSmem &sm, double *sx, double *sy, double *sz) {
+// The tile's 18 segments into sm (starts and prefix sums); every thread
+// of the block calls it.
+__device__ void e_segments(const EnsDev &d, const Rep &rp, const TileDesc &t, TileSmem &sm) {
   if (threadIdx.x < kSeg) {
     int st, ln;
     e_segment(d, rp, t.y, t.z, t.xf, t.xl, threadIdx.x, &st, &ln);
@@ -234,20 +230,30 @@ __device__ void e_stage(const EnsDev &d, const Rep &rp, const TileDesc &t, const
     for (int k = 0; k < kSeg; k++) sm.seg_base[k + 1] += sm.seg_base[k];
   }
   __syncthreads();
+}
+
+// Stage plane p of a tile (segments 6p .. 6p+5) into LDS as interleaved
+// (x, y, z) (list entries are 3 x the index in the whole tile's staging).
+// Reals come from `reals`; images from the positions stored at the build
+// (mid == nullptr) or as their source's mid-step position plus the shift,
+// the image refresh of MDUnit::MakeBufferForBorderParticles
+// (mdunit.cc:181-191) done while staging.  Every thread calls it.
+__device__ void e_stage_plane(const EnsDev &d, const Rep &rp, int p, const double4 *reals,
+                              const double4 *mid, const TileSmem &sm, double *s3) {
   if (mid) {
-    // branch-free two-level gather (see the single-system kick)
-    stage_segments(sm, sx, [&](int k, int slot) {
+    // branch-free two-level gather: every item reads an image entry (a
+    // clamped index for reals) and selects one load at a valid address
+    stage_range(sm, 6 * p, 6 * p + 6, s3, [&](int k, int slot) {
       const int idx = max(slot - d.NR, rp.ibase);
       const int gs = d.gsrc[idx], gc = d.gcode[idx];
       const bool img = (k & 1) != 0;
-      const double4 *src = img ? mid : reals;   // one load at a valid address
+      const double4 *src = img ? mid : reals;
       return image_of(src[img ? gs : slot], img ? gc : kCodeNone, rp.g.L);
     });
   } else {
-    stage_segments(sm, sx, [&](int k, int slot) { return (k & 1) ? d.qe[slot] : reals[slot]; });
+    stage_range(sm, 6 * p, 6 * p + 6, s3,
+                [&](int k, int slot) { return (k & 1) ? d.qe[slot] : reals[slot]; });
   }
-  (void)sy;
-  (void)sz;
 }
 
 // Persistent tile loop: workgroup b runs on XCD b % 8; the workgroups of one
@@ -459,11 +465,16 @@ __global__ void __launch_bounds__(kReplicaThreads) k_e_cells(EnsDev d) {
       }
       t.xf = xf;
       t.xl = xl;
+      // largest staging of one z-plane of partners (six segments)
       int stage = 0;
-      for (int sgi = 0; sgi < kSeg; sgi++) {
-        int st, ln;
-        e_segment(d, rp, y, z, xf, xl, sgi, &st, &ln);
-        stage += ln;
+      for (int pl = 0; pl < kPlanes; pl++) {
+        int ps = 0;
+        for (int sgi = 6 * pl; sgi < 6 * pl + 6; sgi++) {
+          int st, ln;
+          e_segment(d, rp, y, z, xf, xl, sgi, &st, &ln);
+          ps += ln;
+        }
+        stage = max(stage, ps);
       }
       t.stage = stage;
       t.pad = r;
@@ -591,38 +602,51 @@ __global__ void __launch_bounds__(T) k_e_build(EnsDev d) {
     {
       const TileDesc t = d.tiles[rp.tbase + s];
       double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-      e_stage(d, rp, t, d.qe, nullptr, sm, sx, sy, sz);
       const int lane = threadIdx.x;
-      int cnt = 0;
-      if (lane < t.ni) {
-        const int i = t.i0 + lane;
-        const int self = sm.seg_base[8] + (i - sm.seg_start[8]);
-        const double qx = sx[3 * self], qy = sy[3 * self], qz = sz[3 * self];
-        const int cx = interior_coord(qx, g.inv_w[0], g.ms[0]);
-        unsigned short *out = d.list + rp.lbase + (size_t)s * rp.kcap * T + lane;
-        for (int rr = 0; rr < 9; rr++) {
-          const int y = t.y + rr % 3 - 1, z = t.z + rr / 3 - 1;
-          for (int dx = -1; dx <= 1; dx++) {
-            const int x = cx + dx;
-            const int cl = rp.cbase + cell_linear(g, x, y, z);
-            int lo, len;
-            if (is_outer(g, x, y, z)) {
-              lo = sm.seg_base[2 * rr + 1] + (d.scan_out[cl] - sm.seg_start[2 * rr + 1]);
-              len = d.cnt_out[cl];
-            } else {
-              lo = sm.seg_base[2 * rr] + (d.scan_in[cl] - sm.seg_start[2 * rr]);
-              len = d.cnt_in[cl];
-            }
-            for (int e = 3 * lo; e < 3 * (lo + len); e += 3) {
-              if (e == 3 * self) continue;
-              if (list_r2(sx[e], sy[e], sz[e], qx, qy, qz) <= g.SL2) {
-                if (cnt < rp.kcap) out[(size_t)cnt * T] = (unsigned short)e;
-                cnt++;
+      const bool act = lane < t.ni;
+      const int i = t.i0 + lane;
+      const double4 qi = act ? d.qe[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+      e_segments(d, rp, t, sm);
+      const int self = act ? sm.seg_base[8] + (i - sm.seg_start[8]) : -1;
+      const int cx = interior_coord(qi.x, g.inv_w[0], g.ms[0]);
+      unsigned short *out = d.list + rp.lbase + (size_t)s * rp.kcap * T + lane;
+      int cnt = 0, b1 = 0, b2 = 0;
+      // the 27 cells plane by plane (dz = -1, 0, +1), each plane staged alone
+      for (int pl = 0; pl < kPlanes; pl++) {
+        if (pl > 0) __syncthreads();
+        e_stage_plane(d, rp, pl, d.qe, nullptr, sm, sx);
+        const int pb = sm.seg_base[6 * pl];
+        if (act) {
+          for (int rr = 3 * pl; rr < 3 * pl + 3; rr++) {
+            const int y = t.y + rr % 3 - 1, z = t.z + rr / 3 - 1;
+            for (int dx = -1; dx <= 1; dx++) {
+              const int x = cx + dx;
+              const int cl = rp.cbase + cell_linear(g, x, y, z);
+              int lo, len;
+              if (is_outer(g, x, y, z)) {
+                lo = sm.seg_base[2 * rr + 1] + (d.scan_out[cl] - sm.seg_start[2 * rr + 1]);
+                len = d.cnt_out[cl];
+              } else {
+                lo = sm.seg_base[2 * rr] + (d.scan_in[cl] - sm.seg_start[2 * rr]);
+                len = d.cnt_in[cl];
+              }
+              for (int o = lo; o < lo + len; o++) {
+                if (o == self) continue;
+                const int e = 3 * (o - pb);
+                if (list_r2(sx[e], sy[e], sz[e], qi.x, qi.y, qi.z) <= g.SL2) {
+                  if (cnt < rp.kcap) out[(size_t)cnt * T] = (unsigned short)(3 * o);
+                  cnt++;
+                }
               }
             }
           }
         }
+        if (pl == 0) b1 = cnt;
+        if (pl == 1) b2 = cnt;
+      }
+      if (act) {
         d.nn[i] = cnt < rp.kcap ? cnt : rp.kcap;
+        d.nb[i] = min(b1, rp.kcap) | (min(b2, rp.kcap) << 16);
       }
       const int km = block_max_int(cnt);
       if (threadIdx.x == 0) d.tile_kmax[rp.tbase + s] = km;
@@ -682,40 +706,55 @@ __global__ void __launch_bounds__(T) k_e_step(EnsDev d) {
     const double4 *qh = c->par ? d.qh1 : d.qh0;
     double4 *qn = c->par ? d.qh0 : d.qh1;
     double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-    e_stage(d, rp, t, qh, qh, sm, sx, sy, sz);
     const int lane = threadIdx.x;
-    double v2 = 0.0;
-    if (lane < t.ni) {
-      const int i = t.i0 + lane;
-      const double4 qi = qh[i];
-      const int m = d.nn[i];
-      const unsigned short *lst = d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * T + lane;
-      const double CL2 = rp.g.CL2;
-      // entry k feeds chain k % kILP; the last group's entries past m add
-      // exactly +0.0: the sums of the single-system kick, bit for bit
-      double ax[kILP], ay[kILP], az[kILP];
+    const bool act = lane < t.ni;
+    const int i = t.i0 + lane;
+    double4 qi = make_double4(0.0, 0.0, 0.0, 0.0), pm = qi;
+    int m = 0, bnd = 0;
+    if (act) {
+      qi = qh[i];
+      m = d.nn[i];
+      bnd = d.nb[i];
+      pm = d.mom[i];
+    }
+    e_segments(d, rp, t, sm);
+    const unsigned short *lst = d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * T + lane;
+    const double CL2 = rp.g.CL2;
+    // the single-system kick's sums, bit for bit: entry k feeds chain
+    // k % kILP; partners are staged one z-plane at a time and the entries of
+    // a group outside the plane add exactly +0.0
+    double ax[kILP], ay[kILP], az[kILP];
 #pragma unroll
-      for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
+    for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
+    for (int pl = 0; pl < kPlanes; pl++) {
+      if (pl > 0) __syncthreads();
+      e_stage_plane(d, rp, pl, qh, qh, sm, sx);
+      const int base3 = 3 * sm.seg_base[6 * pl];
+      const int kb = pl == 0 ? 0 : (pl == 1 ? (bnd & 0xffff) : (bnd >> 16));
+      const int ke = pl == 0 ? (bnd & 0xffff) : (pl == 1 ? (bnd >> 16) : m);
+      if (!act || kb >= ke) continue;
+      int g0 = kb - kb % kILP;
       int nxt[kILP];
 #pragma unroll
-      for (int u = 0; u < kILP; u++)
-        nxt[u] = m > 0 ? lst[(size_t)min(u, m - 1) * T] : 0;
-      for (int k = 0; k < m; k += kILP) {
+      for (int u = 0; u < kILP; u++) nxt[u] = lst[(size_t)min(g0 + u, m - 1) * T];
+      for (; g0 < ke; g0 += kILP) {
         int cur[kILP];
 #pragma unroll
         for (int u = 0; u < kILP; u++) cur[u] = nxt[u];
-        if (k + kILP < m) {
+        if (g0 + kILP < ke) {
 #pragma unroll
-          for (int u = 0; u < kILP; u++)
-            nxt[u] = lst[(size_t)min(k + kILP + u, m - 1) * T];
+          for (int u = 0; u < kILP; u++) nxt[u] = lst[(size_t)min(g0 + kILP + u, m - 1) * T];
         }
 #pragma unroll
         for (int u = 0; u < kILP; u++) {
-          const bool v = k + u < m;
-          const int o = v ? cur[u] : 0;
+          const bool v = g0 + u >= kb && g0 + u < ke;
+          const int o = v ? cur[u] - base3 : 0;
           kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, d.pk, v ? CL2 : -1.0, ax[u], ay[u], az[u]);
         }
       }
+    }
+    double v2 = 0.0;
+    if (act) {
       double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll
       for (int u = 0; u < kILP; u++) {
@@ -723,7 +762,7 @@ __global__ void __launch_bounds__(T) k_e_step(EnsDev d) {
         fy += ay[u];
         fz += az[u];
       }
-      double4 p = d.mom[i];
+      double4 p = pm;
       p.x += fx;
       p.y += fy;
       p.z += fz;
@@ -911,21 +950,30 @@ __global__ void __launch_bounds__(T) k_e_observe(EnsDev d, double C0, double C2)
   const Rep &rp = d.rep[r];
   const double4 *mid = c->stepped ? (c->par ? d.qh0 : d.qh1) : nullptr;
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-  e_stage(d, rp, t, d.qe, mid, sm, sx, sy, sz);
+  const bool act = lane < t.ni;
+  const int i = t.i0 + lane;
+  e_segments(d, rp, t, sm);
   double ke = 0.0, pe = 0.0, vir = 0.0;
-  if (lane < t.ni) {
-    const int i = t.i0 + lane;
-    const double4 p = d.mom[i];
-    ke = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z;
-    const double4 qi = d.qe[i];
-    const unsigned short *lst = d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * T + lane;
-    for (int k = 0; k < d.nn[i]; k++) {
-      const int o = lst[(size_t)k * T];
+  const int m = act ? d.nn[i] : 0, bnd = act ? d.nb[i] : 0;
+  const double4 qi = act ? d.qe[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+  const unsigned short *lst = d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * T + lane;
+  for (int pl = 0; pl < kPlanes; pl++) {
+    if (pl > 0) __syncthreads();
+    e_stage_plane(d, rp, pl, d.qe, mid, sm, sx);
+    const int base3 = 3 * sm.seg_base[6 * pl];
+    const int kb = pl == 0 ? 0 : (pl == 1 ? (bnd & 0xffff) : (bnd >> 16));
+    const int kend = pl == 0 ? (bnd & 0xffff) : (pl == 1 ? (bnd >> 16) : m);
+    for (int k = kb; k < kend; k++) {
+      const int o = lst[(size_t)k * T] - base3;
       const double dx = qi.x - sx[o], dy = qi.y - sy[o], dz = qi.z - sz[o];
       const double r2 = dx * dx + dy * dy + dz * dz;
       if (r2 > rp.g.CL2) continue;
       observe_term(r2, C0, C2, pe, vir);
     }
+  }
+  if (act) {
+    const double4 p = d.mom[i];
+    ke = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z;
     pe *= 0.5;
     vir *= 0.5;
   }
@@ -1030,7 +1078,7 @@ struct mdx_ensemble {
   TileDesc *tiles = nullptr;
   unsigned short *list = nullptr;
   size_t list_elems = 0;
-  int *nn = nullptr, *tile_kmax = nullptr;
+  int *nn = nullptr, *nb = nullptr, *tile_kmax = nullptr;
   double *partial = nullptr, *obs_partial = nullptr, *obs_out = nullptr, *io = nullptr;
   bool have_state = false;
   hipGraph_t graph = nullptr;
@@ -1123,6 +1171,7 @@ EnsDev dev_view(const mdx_ensemble *e) {
   d.tiles = e->tiles;
   d.list = e->list;
   d.nn = e->nn;
+  d.nb = e->nb;
   d.partial = e->partial;
   d.obs_partial = e->obs_partial;
   d.tile_kmax = e->tile_kmax;
@@ -1472,7 +1521,8 @@ int mdx_ensemble_create(const mdx_params *prm, int nrep, const int *n, const dou
               (rc = ealloc(e, &e->cellid, tot)) || (rc = ealloc(e, &e->slot, tot)) ||
               (rc = ealloc(e, &e->order, tot)) || (rc = ealloc(e, &e->nimg, NR)) ||
               (rc = ealloc(e, &e->goff, NR)) || (rc = ealloc(e, &e->inv, NR)) ||
-              (rc = ealloc(e, &e->nn, NR)) || (rc = ealloc(e, &e->cell_count, cb)) ||
+              (rc = ealloc(e, &e->nn, NR)) || (rc = ealloc(e, &e->nb, NR)) ||
+              (rc = ealloc(e, &e->cell_count, cb)) ||
               (rc = ealloc(e, &e->cnt_in, cb)) || (rc = ealloc(e, &e->cnt_out, cb)) ||
               (rc = ealloc(e, &e->scan_in, cb)) || (rc = ealloc(e, &e->scan_out, cb)) ||
               (rc = ealloc(e, &e->cell_start, cb)) || (rc = ealloc(e, &e->obs_out, 3 * (size_t)nrep)) ||
@@ -1488,7 +1538,7 @@ int mdx_ensemble_create(const mdx_params *prm, int nrep, const int *n, const dou
       const double cellv = (rp.g.L[0] / rp.g.ms[0]) * (rp.g.L[1] / rp.g.ms[1]) * (rp.g.L[2] / rp.g.ms[2]);
       dmax = std::max(dmax, rp.n / (rp.g.L[0] * rp.g.L[1] * rp.g.L[2]) * cellv);
     }
-    const int want = (int)(dmax * 36.0 * 1.25) + 256;
+    const int want = (int)(dmax * 12.0 * 1.25) + 256;   // one plane: 3 rows of 4 cells
     const int lim = (int)((e->lds_limit - 1024) / 24);
     rc = set_stage_cap(e, std::min(want, lim));
   }
@@ -1510,7 +1560,7 @@ int mdx_ensemble_destroy(mdx_ensemble *e) {
                   e->pos_tmp, e->mom_tmp, e->gpos_tmp, e->init_q, e->init_p, e->tag, e->tag_tmp,
                   e->gsrc, e->gcode, e->gsrc_tmp, e->gcode_tmp, e->cellid, e->slot, e->nimg, e->goff,
                   e->order, e->inv, e->cell_count, e->cnt_in, e->cnt_out, e->scan_in, e->scan_out,
-                  e->cell_start, e->tiles, e->list, e->nn, e->tile_kmax, e->partial, e->obs_partial,
+                  e->cell_start, e->tiles, e->list, e->nn, e->nb, e->tile_kmax, e->partial, e->obs_partial,
                   e->obs_out, e->io};
   for (void *b : bufs)
     if (b) (void)hipFree(b);

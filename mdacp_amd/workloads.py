@@ -116,49 +116,51 @@ def slab_state(seed: int = 3, Lx: float = 200.0, Ly: float = 100.0, Lz: float = 
                temperature: float = 0.9, min_sep: float = 0.8,
                dt: float = 0.001) -> State:
     """Inhomogeneous box: liquid slab x in [Lx/4, 3Lx/4) at liquid_density,
-    vapour elsewhere; random sequential addition with overlaps < min_sep
-    rejected (grid-hashed), momenta Maxwell at ``temperature``."""
+    vapour elsewhere; positions uniformly random in each region, a
+    candidate rejected when it comes closer than min_sep (periodic) to an
+    accepted point; momenta Maxwell at ``temperature``.
+
+    Random sequential addition in batches: each batch of candidates is
+    tested against the accepted points (k-d tree), and of two candidates of
+    one batch closer than min_sep the later one is dropped."""
+    from scipy.spatial import cKDTree
     rng = np.random.Generator(np.random.PCG64(seed))
     L = np.array([Lx, Ly, Lz])
     x0, x1 = Lx * 0.25, Lx * 0.75
     n_liq = int(round(liquid_density * (x1 - x0) * Ly * Lz))
     n_vap = int(round(vapour_density * (Lx - (x1 - x0)) * Ly * Lz))
-    cw = min_sep
-    ncell = np.floor(L / cw).astype(int)
-    cw = L / ncell
-    grid: dict = {}
-    pts = []
+    pts = np.empty((0, 3))
 
-    def try_add(x):
-        c = np.floor(x / cw).astype(int) % ncell
-        for dz in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dx in (-1, 0, 1):
-                    key = ((c[0] + dx) % ncell[0], (c[1] + dy) % ncell[1], (c[2] + dz) % ncell[2])
-                    for y in grid.get(key, ()):
-                        d = x - y
-                        d -= L * np.round(d / L)
-                        if d @ d < min_sep * min_sep:
-                            return False
-        grid.setdefault((c[0], c[1], c[2]), []).append(x)
-        pts.append(x)
-        return True
-
-    def fill(count, lo, hi):
+    def fill(count, draw):
+        nonlocal pts
         added = 0
         while added < count:
-            x = np.array([rng.uniform(lo, hi), rng.uniform(0, Ly), rng.uniform(0, Lz)])
-            if try_add(x):
-                added += 1
+            m = int(min(max(4096, 1.5 * (count - added)), 250000))
+            cand = draw(m)
+            if len(pts):
+                d, _ = cKDTree(pts, boxsize=L).query(cand, k=1, distance_upper_bound=min_sep)
+                cand = cand[~(d < min_sep)]
+            if len(cand) > 1:
+                pairs = cKDTree(cand, boxsize=L).query_pairs(min_sep, output_type="ndarray")
+                if len(pairs):
+                    keep = np.ones(len(cand), dtype=bool)
+                    keep[pairs.max(axis=1)] = False
+                    cand = cand[keep]
+            cand = cand[:count - added]
+            pts = np.vstack([pts, cand])
+            added += len(cand)
 
-    fill(n_liq, x0, x1)
-    # vapour: x uniformly in the complement [x1, Lx + x0) mapped back into the box
-    added = 0
-    while added < n_vap:
-        x = np.array([rng.uniform(x1, Lx + x0) % Lx, rng.uniform(0, Ly), rng.uniform(0, Lz)])
-        if try_add(x):
-            added += 1
-    q = np.array(pts)
+    def liquid(m):
+        return np.column_stack([rng.uniform(x0, x1, m), rng.uniform(0, Ly, m), rng.uniform(0, Lz, m)])
+
+    def vapour(m):
+        # x uniformly in the complement [x1, Lx + x0), mapped back into the box
+        return np.column_stack([rng.uniform(x1, Lx + x0, m) % Lx, rng.uniform(0, Ly, m),
+                                rng.uniform(0, Lz, m)])
+
+    fill(n_liq, liquid)
+    fill(n_vap, vapour)
+    q = pts
     p = maxwell_momenta(q.shape[0], temperature, seed + 1000)
     return State(q, p, L, dt, name="slab")
 

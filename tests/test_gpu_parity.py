@@ -166,6 +166,30 @@ def test_full_size_pairs_vs_oracle():
         assert force_err(g.forces(), of) < FORCE_TOL
 
 
+def test_full_size_inhomogeneous_slab():
+    """The inhomogeneous workload at full size: 200 x 100 x 100 box, liquid
+    slab (rho 0.8, T 0.9) in vapour (rho 0.05), 850,000 particles, skewed
+    cell occupancy (about 30 vs 2 per cell).  Pair set (digest), forces
+    and observables against the oracle; then 50 steps keep the momentum."""
+    s = W.slab_state()
+    print(f"slab N={s.n}", flush=True)
+    o = O.Oracle(s)
+    o.make_pairlist()
+    op, of, oo = o.pairs(), o.forces(), o.observe()
+    with System.from_state(s) as g:
+        g.make_pairlist()
+        gp = g.pairs()
+        assert len(gp) == len(op)
+        assert digest(gp) == digest(op)
+        assert force_err(g.forces(), of) < FORCE_TOL
+        np.testing.assert_allclose(g.observe(), oo, rtol=OBS_TOL)
+        p0 = s.p.sum(axis=0)
+        g.calculate(50)
+        q, p = g.state()
+        assert np.isfinite(q).all() and np.isfinite(p).all()
+        np.testing.assert_allclose(p.sum(axis=0), p0, atol=1e-8)
+
+
 def test_error_paths():
     s = W.plain_small()
     s.q[9, 2] = np.nan
