@@ -1178,7 +1178,8 @@ EnsDev dev_view(const mdx_ensemble *e) {
   return d;
 }
 
-size_t stage_bytes(const mdx_ensemble *e) { return (size_t)e->stage_cap * 3 * sizeof(double); }
+// dynamic LDS of the tile kernels: one plane of a tile plus the sentinel
+size_t stage_bytes(const mdx_ensemble *e) { return (size_t)(e->stage_cap + 1) * 3 * sizeof(double); }
 
 int launch_check(mdx_ensemble *e, const char *what) {
   hipError_t er = hipGetLastError();
@@ -1244,7 +1245,7 @@ int regrow_lists(mdx_ensemble *e, const std::vector<int> &grow) {
 }
 
 int set_stage_cap(mdx_ensemble *e, int cap) {
-  const size_t need = (size_t)cap * 3 * sizeof(double) + 1024;
+  const size_t need = (size_t)(cap + 1) * 3 * sizeof(double) + 1024;
   if (need > e->lds_limit)
     return efail(e, MDX_ERR_CAPACITY, "tile staging of %d particles exceeds LDS (%zu B)", cap,
                  e->lds_limit);

@@ -60,6 +60,7 @@ struct DevStatus {
   double kmax_d;                   // longest neighbour list (as a double)
   double stage_max;                // largest tile staging (particles)
   double plane_max;                // largest staging of one plane of a tile
+  double pass_max;                 // largest staging of one pass of the kick
   long long canon_count;
   double sums[4];                  // KE, PE, virial, spare
 };
@@ -430,7 +431,7 @@ __global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict_
                               double *__restrict__ partial) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int rows = g.ms[1] * g.ms[2];
-  double smax = 0.0, pmax = 0.0;
+  double smax = 0.0, pmax = 0.0, qmax = 0.0;
   if (r < rows) {
     int y, z;
     row_of(g, r, &y, &z);
@@ -460,10 +461,13 @@ __global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict_
       smax = fmax(smax, (double)stage);
       for (int p = 0; p < kPlanes; p++)
         pmax = fmax(pmax, (double)(ts[kSeg + 6 * p + 6] - ts[kSeg + 6 * p]));
+      for (int p = 0; p < kPasses; p++)
+        qmax = fmax(qmax, (double)(ts[kSeg + 6 * pass_plane(p + 1)] - ts[kSeg + 6 * pass_plane(p)]));
     }
   }
   block_max_to(smax, partial);
   block_max_to(pmax, partial + gridDim.x);
+  block_max_to(qmax, partial + 2 * gridDim.x);
 }
 
 // Full neighbour list per tile: lane = real i, every partner j != i (real
@@ -665,43 +669,45 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
   double ax[kILP], ay[kILP], az[kILP];
 #pragma unroll
   for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
-  for (int p = 0; p < kPlanes; p++) {
-    if (p > 0) __syncthreads();   // the previous plane is read
-    stage_range(sm, 6 * p, 6 * p + 6, sx, [&](int, int slot) { return pos[slot]; });
-    const int base3 = 3 * sm.seg_base[6 * p];
-    const int kb = p == 0 ? 0 : (p == 1 ? (bnd & 0xffff) : (bnd >> 16));
-    const int ke = p == 0 ? (bnd & 0xffff) : (p == 1 ? (bnd >> 16) : m);
+  for (int p = 0; p < kPasses; p++) {
+    if (p > 0) __syncthreads();   // the previous pass is read
+    const int pl0 = pass_plane(p), pl1 = pass_plane(p + 1);
+    stage_range(sm, 6 * pl0, 6 * pl1, sx, [&](int, int slot) { return pos[slot]; });
+    const int base3 = 3 * sm.seg_base[6 * pl0];
+    const int far3 = 3 * (sm.seg_base[6 * pl1] - sm.seg_base[6 * pl0]);   // the sentinel
+    const int kb = pl0 == 0 ? 0 : (pl0 == 1 ? (bnd & 0xffff) : (bnd >> 16));
+    const int ke = pl1 == 3 ? m : (pl1 == 1 ? (bnd & 0xffff) : (bnd >> 16));
     if (!act || kb >= ke) continue;
-    // groups of kILP entries aligned on the whole list; entries of a group
-    // outside [kb, ke) add exactly +0.0 (the accumulators are never -0.0),
-    // so the sums are those of one pass over the list.  The entries of the
-    // next group are loaded while this group computes.
+    // groups of kILP entries aligned on the whole list; an entry of a group
+    // outside [kb, ke) points at the sentinel and adds exactly +0.0 (the
+    // accumulators are never -0.0), so the sums are those of one pass over
+    // the list.  The list is padded by a group, so no load is clamped; the
+    // entries of the next group are loaded while this group computes.
     int g0 = kb - kb % kILP;
+    const unsigned short *lp = lst + (size_t)g0 * kTile;
     int nxt[kILP];
 #pragma unroll
-    for (int u = 0; u < kILP; u++)
-      nxt[u] = lst[(size_t)min(g0 + u, m - 1) * kTile];
-    for (; g0 < ke; g0 += kILP) {
+    for (int u = 0; u < kILP; u++) nxt[u] = lp[u * kTile];
+    for (; g0 < ke; g0 += kILP, lp += kILP * kTile) {
       int cur[kILP];
 #pragma unroll
       for (int u = 0; u < kILP; u++) cur[u] = nxt[u];
       if (g0 + kILP < ke) {
 #pragma unroll
-        for (int u = 0; u < kILP; u++)
-          nxt[u] = lst[(size_t)min(g0 + kILP + u, m - 1) * kTile];
+        for (int u = 0; u < kILP; u++) nxt[u] = lp[(kILP + u) * kTile];
       }
 #pragma unroll
       for (int u = 0; u < kILP; u++) {
-        const bool v = g0 + u >= kb && g0 + u < ke;
-        const int o = v ? cur[u] - base3 : 0;
+        const bool v = (unsigned)(g0 + u - kb) < (unsigned)(ke - kb);
+        const int o = v ? cur[u] - base3 : far3;
         if (SCALED) {
           const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
           const double df = pair_df_lj(dx * dx + dy * dy + dz * dz, lj, dt);
-          ax[u] += (v ? df : 0.0) * dx;
-          ay[u] += (v ? df : 0.0) * dy;
-          az[u] += (v ? df : 0.0) * dz;
+          ax[u] += df * dx;
+          ay[u] += df * dy;
+          az[u] += df * dz;
         } else {
-          kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, pk, v ? g.CL2 : -1.0, ax[u], ay[u], az[u]);
+          kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, pk, g.CL2, ax[u], ay[u], az[u]);
         }
       }
     }
@@ -1202,7 +1208,7 @@ struct mdx_system {
   // tiles and their 16-bit lists
   TileDesc *tiles = nullptr;
   int *tseg = nullptr;        // [tiles_cap][kTSeg]
-  int ntiles = 0, tiles_cap = 0, stage_max = 0, plane_max = 0;
+  int ntiles = 0, tiles_cap = 0, stage_max = 0, plane_max = 0, pass_max = 0;
   int *row_tiles = nullptr, *row_off = nullptr;
   unsigned short *list16 = nullptr;
   size_t list16_elems = 0;
@@ -1337,7 +1343,8 @@ int ensure_image_capacity(mdx_system *s, int gcap) {
 }
 
 int ensure_list_capacity(mdx_system *s, int kcap) {
-  const size_t need = (size_t)std::max(s->ntiles, 1) * kcap * kTile;
+  // padded by one group of entries, which the kick reads past a list's end
+  const size_t need = (size_t)std::max(s->ntiles, 1) * kcap * kTile + (size_t)kILP * kTile;
   if (need <= s->list16_elems && kcap <= s->kcap) return MDX_OK;
   int rc = dev_alloc(s, &s->list16, need);
   if (rc) return rc;
@@ -1346,8 +1353,10 @@ int ensure_list_capacity(mdx_system *s, int kcap) {
   return MDX_OK;
 }
 
-size_t stage_bytes(const mdx_system *s) { return (size_t)s->stage_max * 3 * sizeof(double); }
-size_t plane_bytes(const mdx_system *s) { return (size_t)s->plane_max * 3 * sizeof(double); }
+// dynamic LDS of the tile kernels: the staged particles plus the sentinel
+size_t stage_bytes(const mdx_system *s) { return (size_t)(s->stage_max + 1) * 3 * sizeof(double); }
+size_t plane_bytes(const mdx_system *s) { return (size_t)(s->plane_max + 1) * 3 * sizeof(double); }
+size_t pass_bytes(const mdx_system *s) { return (size_t)(s->pass_max + 1) * 3 * sizeof(double); }
 
 Cells cells_of(const mdx_system *s) {
   return Cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out, s->tseg};
@@ -1450,15 +1459,17 @@ int build_pairlist(mdx_system *s) {
     s->tiles_cap = (int)cap;
   }
   cells.tseg = s->tseg;
-  if ((rc = ensure_partials(s, std::max(tile_grid(s->ntiles), 2 * nblk(rows))))) return rc;
+  if ((rc = ensure_partials(s, std::max(tile_grid(s->ntiles), 3 * nblk(rows))))) return rc;
   k_write_tiles<<<nblk(rows), kBlock, 0, s->stream>>>(s->geom, cells, n, s->pos, s->row_tiles,
                                                        s->row_off, s->tiles, s->tseg, s->pmax);
   k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(rows), &s->dstat->stage_max);
   k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax + nblk(rows), nblk(rows), &s->dstat->plane_max);
+  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax + 2 * nblk(rows), nblk(rows), &s->dstat->pass_max);
   if ((rc = sync_status(s))) return rc;
   s->stage_max = (int)s->hstat->stage_max;
   s->plane_max = (int)s->hstat->plane_max;
-  if ((size_t)s->stage_max * 3 * sizeof(double) + 4096 > s->lds_limit)
+  s->pass_max = (int)s->hstat->pass_max;
+  if (stage_bytes(s) + 4096 > s->lds_limit)
     return fail(s, MDX_ERR_CAPACITY, "tile staging of %d particles exceeds LDS (%zu B)",
                 s->stage_max, s->lds_limit);
   if ((rc = ensure_list_capacity(s, s->kcap))) return rc;
@@ -1491,7 +1502,7 @@ int build_pairlist(mdx_system *s) {
 int kick(mdx_system *s) {
   const int grid = tile_grid(s->ntiles);
   ev_record(s, 2);
-  k_kick_tiles<0><<<grid, kTile, plane_bytes(s), s->stream>>>(
+  k_kick_tiles<0><<<grid, kTile, pass_bytes(s), s->stream>>>(
       s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), s->n, s->kcap, s->list16, s->nn,
       s->nb, make_pairk(s->C2, s->prm.dt), s->prm.dt, nullptr, s->pmax);
   ev_record(s, 3);
@@ -1531,7 +1542,7 @@ int run_batch(mdx_system *s, int want, int *ran) {
     const double4 *cur = (b & 1) ? s->pos_mid : s->pos_alt;
     double4 *nxt = (b & 1) ? s->pos_alt : s->pos_mid;
     if (s->timing) (void)hipEventRecord(s->evk[2 * b], s->stream);
-    k_kick_tiles<3><<<grid, kTile, plane_bytes(s), s->stream>>>(
+    k_kick_tiles<3><<<grid, kTile, pass_bytes(s), s->stream>>>(
         cur, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn,
         s->nb, make_pairk(s->C2, dt), dt, s->pos, s->pmax, LJ{}, &s->dctl->halt, nxt);
     if (s->timing) (void)hipEventRecord(s->evk[2 * b + 1], s->stream);
@@ -1690,7 +1701,7 @@ int lj_step(mdx_system *s, const LJ &lj, double4 *mid) {
   k_drift_refresh<<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->gsrc, s->gcode, n, g,
                                                           s->prm.dt * 0.5, s->geom.L[0],
                                                           s->geom.L[1], s->geom.L[2], mid, nullptr);
-  k_kick_tiles<2, true><<<grid, kTile, plane_bytes(s), s->stream>>>(
+  k_kick_tiles<2, true><<<grid, kTile, pass_bytes(s), s->stream>>>(
       mid, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb,
       make_pairk(s->C2, s->prm.dt), s->prm.dt, s->pos, s->pmax, lj);
   return launch_check(s, "lj step");
@@ -2129,7 +2140,7 @@ int mdx_observe(mdx_system *s, double *ke, double *pe, double *vir) {
 int mdx_forces(mdx_system *s, double *f3) {
   if (!s || !f3) return fail(s, MDX_ERR_ARG, "null argument");
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
-  k_kick_tiles<1><<<tile_grid(s->ntiles), kTile, plane_bytes(s), s->stream>>>(
+  k_kick_tiles<1><<<tile_grid(s->ntiles), kTile, pass_bytes(s), s->stream>>>(
       s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), s->n, s->kcap, s->list16, s->nn,
       s->nb, make_pairk(s->C2, 1.0), 1.0, s->vec_out, nullptr);
   k_scatter_by_tag<<<nblk(s->n), kBlock, 0, s->stream>>>(s->vec_out, s->tag, s->n, s->io_buf);

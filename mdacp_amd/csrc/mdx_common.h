@@ -83,6 +83,8 @@ struct TileSmem {
 // real of the tile's own row (segment 8, never empty) and are not stored,
 // so no branch separates the loads.  Every thread calls it; it ends with a
 // barrier.
+constexpr double kFar = 1.0e100;
+
 template <class Load>
 __device__ __forceinline__ void stage_range(const TileSmem &sm, int k0, int k1, double *s3, Load load) {
   constexpr int kBatch = 4;
@@ -117,6 +119,9 @@ __device__ __forceinline__ void stage_range(const TileSmem &sm, int k0, int k1, 
       }
     }
   }
+  // sentinel after the staged particles: a partner that far away is past
+  // any cutoff, so a masked list entry can point here
+  if (threadIdx.x == 0) s3[3 * total] = s3[3 * total + 1] = s3[3 * total + 2] = kFar;
   __syncthreads();
 }
 
@@ -126,9 +131,18 @@ __device__ __forceinline__ void stage_segments(const TileSmem &sm, double *s3, L
 }
 
 // A tile's partners lie in three z-planes of cell rows (dz = -1, 0, +1):
-// segments 6p .. 6p+5, list rows 3p .. 3p+2.  The kick and the list build
-// stage one plane at a time, a third of the LDS of the whole tile.
+// segments 6p .. 6p+5, list rows 3p .. 3p+2.  The list build stages one
+// plane at a time, a third of the LDS of the whole tile.
 constexpr int kPlanes = 3;
+
+// The single-system kick walks its partners in kPasses passes of whole
+// planes, pass p covering planes [pass_plane(p), pass_plane(p + 1)): fewer
+// passes stage more at once (more LDS per tile) but lose less to the lanes
+// of a wavefront reaching a plane boundary at different list entries.
+constexpr int kPasses = 3;
+__host__ __device__ constexpr int pass_plane(int p) {
+  return kPasses == 3 ? p : (kPasses == 2 ? (p == 0 ? 0 : (p == 1 ? 1 : 3)) : (p == 0 ? 0 : 3));
+}
 
 // Order of the interior cell rows (y, z) in the tile array: strips of h
 // consecutive z, walked y-major with z fastest inside a strip, h = ceil(ms_z
