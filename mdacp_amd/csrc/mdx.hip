@@ -369,6 +369,7 @@ constexpr int kTSeg = 40;
 struct Cells {
   const int *scan_in, *cnt_in, *scan_out, *cnt_out;
   const int *tseg;   // [tile][kTSeg] segments of every tile
+  const unsigned char *tperm;   // [tile][kTile] real (offset in the tile) of each list column
 };
 
 
@@ -481,7 +482,8 @@ __global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict_
 __global__ void __launch_bounds__(kTile) k_build_tiles(
     const double4 *__restrict__ pos, const TileDesc *__restrict__ tiles, int ntiles, Geom g,
     Cells c, int n, int kcap, unsigned short *__restrict__ list, int *__restrict__ nn,
-    int *__restrict__ nb, double *__restrict__ partial, int *__restrict__ flags) {
+    int *__restrict__ nb, unsigned char *__restrict__ tperm, double *__restrict__ partial,
+    int *__restrict__ flags) {
   extern __shared__ double stage_mem[];
   __shared__ TileSmem sm;
   const int tile = tile_of_block(blockIdx.x, ntiles);
@@ -495,10 +497,21 @@ __global__ void __launch_bounds__(kTile) k_build_tiles(
   const bool act = lane < t.ni;
   const int i = t.i0 + lane;
   const double4 qi = act ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+  // List columns in the order of z (ties by position in the tile): reals
+  // near the bottom of their cells have many partners in the plane below,
+  // so a wavefront's lanes reach each plane boundary at similar entries.
+  __shared__ double zs[kTile];
+  zs[lane] = act ? qi.z : kFar;
   tile_load_segments(c, tile, sm);
+  int col = lane;
+  if (act) {
+    col = 0;
+    for (int j = 0; j < t.ni; j++) col += (zs[j] < qi.z || (zs[j] == qi.z && j < lane)) ? 1 : 0;
+    tperm[(size_t)tile * kTile + col] = (unsigned char)lane;
+  }
   const int self = act ? sm.seg_base[8] + (i - sm.seg_start[8]) : -1;   // in row (0, 0)
   const int cx = interior_coord(qi.x, g.inv_w[0], g.ms[0]);
-  unsigned short *out = list + (size_t)tile * kcap * kTile + lane;
+  unsigned short *out = list + (size_t)tile * kcap * kTile + col;
   int cnt = 0, b1 = 0, b2 = 0;
   for (int p = 0; p < kPlanes; p++) {
     if (p > 0) __syncthreads();   // the previous plane is read
@@ -652,7 +665,7 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
   const int lane = threadIdx.x;
   const bool act = lane < t.ni;
-  const int i = t.i0 + lane;
+  const int i = t.i0 + (act ? c.tperm[(size_t)tile * kTile + lane] : 0);   // list column -> real
   // the lane's own loads go out before the staging, which hides them
   double4 qi = make_double4(0.0, 0.0, 0.0, 0.0), pm = qi;
   int m = 0, bnd = 0;
@@ -837,7 +850,7 @@ __global__ void __launch_bounds__(kTile) k_lj_observe_tiles(
   double acc[K];
   for (int k = 0; k < K; k++) acc[k] = 0.0;
   if (lane < t.ni) {
-    const int i = t.i0 + lane;
+    const int i = t.i0 + c.tperm[(size_t)tile * kTile + lane];
     const double4 qi = pos[i];
     const double4 p = mom[i];
     const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
@@ -937,7 +950,7 @@ __global__ void __launch_bounds__(kTile) k_adj_hvp_tiles(
   const int lane = threadIdx.x;
   double acc[2] = {0.0, 0.0};
   if (lane < t.ni) {
-    const int i = t.i0 + lane;
+    const int i = t.i0 + c.tperm[(size_t)tile * kTile + lane];
     const double4 qi = pos[i];
     const double4 ai = ext[i];
     const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
@@ -1012,7 +1025,7 @@ __global__ void __launch_bounds__(kTile) k_observe_tiles(
   const int lane = threadIdx.x;
   double ke = 0.0, pe = 0.0, vir = 0.0;
   if (lane < t.ni) {
-    const int i = t.i0 + lane;
+    const int i = t.i0 + c.tperm[(size_t)tile * kTile + lane];
     const double4 p = mom[i];
     ke = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z;
     const double4 qi = pos[i];
@@ -1127,7 +1140,7 @@ __global__ void __launch_bounds__(kTile) k_canonical_tiles(
   tile_load_segments(c, tile, sm);
   const int lane = threadIdx.x;
   if (lane >= t.ni) return;
-  const int i = t.i0 + lane;
+  const int i = t.i0 + c.tperm[(size_t)tile * kTile + lane];
   const int ti = tag[i];
   const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
   int cnum = 0;
@@ -1208,6 +1221,7 @@ struct mdx_system {
   // tiles and their 16-bit lists
   TileDesc *tiles = nullptr;
   int *tseg = nullptr;        // [tiles_cap][kTSeg]
+  unsigned char *tperm = nullptr;   // [tiles_cap][kTile] real of each list column
   int ntiles = 0, tiles_cap = 0, stage_max = 0, plane_max = 0, pass_max = 0;
   int *row_tiles = nullptr, *row_off = nullptr;
   unsigned short *list16 = nullptr;
@@ -1359,7 +1373,7 @@ size_t plane_bytes(const mdx_system *s) { return (size_t)(s->plane_max + 1) * 3 
 size_t pass_bytes(const mdx_system *s) { return (size_t)(s->pass_max + 1) * 3 * sizeof(double); }
 
 Cells cells_of(const mdx_system *s) {
-  return Cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out, s->tseg};
+  return Cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out, s->tseg, s->tperm};
 }
 
 // per-block partial buffers for every grid the library launches
@@ -1447,7 +1461,7 @@ int build_pairlist(mdx_system *s) {
                                                         s->gcode);
   if ((rc = launch_check(s, "sort"))) return rc;
   // 5. tiles over the interior cell rows
-  Cells cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out, s->tseg};
+  Cells cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out, s->tseg, s->tperm};
   const int rows = s->geom.ms[1] * s->geom.ms[2];
   k_row_tiles<<<nblk(rows), kBlock, 0, s->stream>>>(s->geom, cells, s->row_tiles);
   if ((rc = scan(s, s->row_tiles, s->row_off, rows, &s->dstat->ntiles))) return rc;
@@ -1455,10 +1469,13 @@ int build_pairlist(mdx_system *s) {
   s->ntiles = s->hstat->ntiles;
   if (s->ntiles > s->tiles_cap) {
     const size_t cap = (size_t)s->ntiles + s->ntiles / 8 + 64;
-    if ((rc = dev_alloc(s, &s->tiles, cap)) || (rc = dev_alloc(s, &s->tseg, cap * kTSeg))) return rc;
+    if ((rc = dev_alloc(s, &s->tiles, cap)) || (rc = dev_alloc(s, &s->tseg, cap * kTSeg)) ||
+        (rc = dev_alloc(s, &s->tperm, cap * kTile)))
+      return rc;
     s->tiles_cap = (int)cap;
   }
   cells.tseg = s->tseg;
+  cells.tperm = s->tperm;
   if ((rc = ensure_partials(s, std::max(tile_grid(s->ntiles), 3 * nblk(rows))))) return rc;
   k_write_tiles<<<nblk(rows), kBlock, 0, s->stream>>>(s->geom, cells, n, s->pos, s->row_tiles,
                                                        s->row_off, s->tiles, s->tseg, s->pmax);
@@ -1477,8 +1494,8 @@ int build_pairlist(mdx_system *s) {
   for (int attempt = 0; attempt < 3; attempt++) {
     HIPCHK(s, hipMemsetAsync(&s->dstat->flags, 0, sizeof(int), s->stream));
     k_build_tiles<<<tile_grid(s->ntiles), kTile, plane_bytes(s), s->stream>>>(
-        s->pos, s->tiles, s->ntiles, s->geom, cells, n, s->kcap, s->list16, s->nn, s->nb, s->pmax,
-        &s->dstat->flags);
+        s->pos, s->tiles, s->ntiles, s->geom, cells, n, s->kcap, s->list16, s->nn, s->nb, s->tperm,
+        s->pmax, &s->dstat->flags);
     k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, tile_grid(s->ntiles), &s->dstat->kmax_d);
     if ((rc = launch_check(s, "list"))) return rc;
     if ((rc = sync_status(s))) return rc;
@@ -1983,7 +2000,7 @@ int mdx_create(const mdx_params *prm, mdx_system **out) {
 int mdx_destroy(mdx_system *s) {
   if (!s) return MDX_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  void *bufs[] = {s->pos, s->pos_alt, s->pos_mid, s->tseg, s->pmax, s->tiles, s->row_tiles, s->row_off, s->list16,
+  void *bufs[] = {s->pos, s->pos_alt, s->pos_mid, s->tseg, s->tperm, s->pmax, s->tiles, s->row_tiles, s->row_off, s->list16,
                   s->pos_tmp, s->ref_pos, s->mom, s->mom_tmp, s->gpos_tmp, s->vec_out,
                   s->tag, s->tag_tmp, s->gsrc, s->gcode, s->gsrc_tmp, s->gcode_tmp, s->cellid,
                   s->slot, s->order, s->inv, s->nimg, s->goff, s->cell_count, s->cell_start,
