@@ -576,7 +576,7 @@ __global__ void k_e_gather_images(EnsDev d) {
 
 // Full neighbour list per tile (as the single-system build): lane = real,
 // every partner in its 27 cells with r^2 <= SL^2 (meshlist.cc:354-362),
-// stored as LDS offsets at list[lbase + ((tile * kcap) + k) * T + lane].
+// in the plane-list form at list[lbase + ((tile * kcap) + k) * T + lane].
 template <int T>
 __global__ void __launch_bounds__(T) k_e_build(EnsDev d) {
   extern __shared__ double stage_mem[];
@@ -634,12 +634,13 @@ __global__ void __launch_bounds__(T) k_e_build(EnsDev d) {
                 if (o == self) continue;
                 const int e = 3 * (o - pb);
                 if (list_r2(sx[e], sy[e], sz[e], qi.x, qi.y, qi.z) <= g.SL2) {
-                  if (cnt < rp.kcap) out[(size_t)cnt * T] = (unsigned short)(3 * o);
+                  if (cnt < rp.kcap) out[(size_t)cnt * T] = plane_entry(o - pb);
                   cnt++;
                 }
               }
             }
           }
+          pad_plane<T>(out, cnt, rp.kcap, sm.seg_base[6 * pl + 6] - pb);
         }
         if (pl == 0) b1 = cnt;
         if (pl == 1) b2 = cnt;
@@ -719,39 +720,20 @@ __global__ void __launch_bounds__(T) k_e_step(EnsDev d) {
     }
     e_segments(d, rp, t, sm);
     const unsigned short *lst = d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * T + lane;
-    const double CL2 = rp.g.CL2;
+    const double CL2 = rp.g.CL2, xi = qi.x, yi = qi.y, zi = qi.z;
     // the single-system kick's sums, bit for bit: entry k feeds chain
-    // k % kILP; partners are staged one z-plane at a time and the entries of
-    // a group outside the plane add exactly +0.0
+    // k % kILP; partners are staged one z-plane at a time
     double ax[kILP], ay[kILP], az[kILP];
 #pragma unroll
     for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
     for (int pl = 0; pl < kPlanes; pl++) {
       if (pl > 0) __syncthreads();
       e_stage_plane(d, rp, pl, qh, qh, sm, sx);
-      const int base3 = 3 * sm.seg_base[6 * pl];
-      const int kb = pl == 0 ? 0 : (pl == 1 ? (bnd & 0xffff) : (bnd >> 16));
-      const int ke = pl == 0 ? (bnd & 0xffff) : (pl == 1 ? (bnd >> 16) : m);
-      if (!act || kb >= ke) continue;
-      int g0 = kb - kb % kILP;
-      int nxt[kILP];
-#pragma unroll
-      for (int u = 0; u < kILP; u++) nxt[u] = lst[(size_t)min(g0 + u, m - 1) * T];
-      for (; g0 < ke; g0 += kILP) {
-        int cur[kILP];
-#pragma unroll
-        for (int u = 0; u < kILP; u++) cur[u] = nxt[u];
-        if (g0 + kILP < ke) {
-#pragma unroll
-          for (int u = 0; u < kILP; u++) nxt[u] = lst[(size_t)min(g0 + kILP + u, m - 1) * T];
-        }
-#pragma unroll
-        for (int u = 0; u < kILP; u++) {
-          const bool v = g0 + u >= kb && g0 + u < ke;
-          const int o = v ? cur[u] - base3 : 0;
-          kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, d.pk, v ? CL2 : -1.0, ax[u], ay[u], az[u]);
-        }
-      }
+      if (!act) continue;
+      plane_groups<T>(lst, plane_begin(pl, bnd), plane_end(pl, bnd, m), stage_mem,
+                      [&](int u, double xj, double yj, double zj) {
+        kick_term(xj, yj, zj, xi, yi, zi, d.pk, CL2, ax[u], ay[u], az[u]);
+      });
     }
     double v2 = 0.0;
     if (act) {
@@ -960,12 +942,11 @@ __global__ void __launch_bounds__(T) k_e_observe(EnsDev d, double C0, double C2)
   for (int pl = 0; pl < kPlanes; pl++) {
     if (pl > 0) __syncthreads();
     e_stage_plane(d, rp, pl, d.qe, mid, sm, sx);
-    const int base3 = 3 * sm.seg_base[6 * pl];
-    const int kb = pl == 0 ? 0 : (pl == 1 ? (bnd & 0xffff) : (bnd >> 16));
-    const int kend = pl == 0 ? (bnd & 0xffff) : (pl == 1 ? (bnd >> 16) : m);
-    for (int k = kb; k < kend; k++) {
-      const int o = lst[(size_t)k * T] - base3;
-      const double dx = qi.x - sx[o], dy = qi.y - sy[o], dz = qi.z - sz[o];
+    // a sentinel entry lies past the cutoff
+    const int kend = plane_end(pl, bnd, m);
+    for (int k = plane_begin(pl, bnd); k < kend; k++) {
+      const double *q = staged_at(stage_mem, lst[(size_t)k * T]);
+      const double dx = qi.x - q[0], dy = qi.y - q[1], dz = qi.z - q[2];
       const double r2 = dx * dx + dy * dy + dz * dz;
       if (r2 > rp.g.CL2) continue;
       observe_term(r2, C0, C2, pe, vir);
@@ -1249,6 +1230,10 @@ int set_stage_cap(mdx_ensemble *e, int cap) {
   if (need > e->lds_limit)
     return efail(e, MDX_ERR_CAPACITY, "tile staging of %d particles exceeds LDS (%zu B)", cap,
                  e->lds_limit);
+  if (cap > kMaxPlane)
+    return efail(e, MDX_ERR_CAPACITY,
+                 "a tile plane of %d particles exceeds the 16-bit list offsets (%d particles)", cap,
+                 kMaxPlane);
   e->stage_cap = cap;
   e->graph_dirty = true;
   return MDX_OK;
@@ -1333,7 +1318,7 @@ int fix_capacity(mdx_ensemble *e, bool *fixed) {
   }
   int rc;
   if (need_stage > 0) {
-    if ((rc = set_stage_cap(e, need_stage + need_stage / 8 + 64))) return rc;
+    if ((rc = set_stage_cap(e, std::max(need_stage, std::min(need_stage + need_stage / 8 + 64, kMaxPlane))))) return rc;
     if ((rc = size_step_grid(e))) return rc;
   }
   if (!grow.empty() && (rc = regrow_lists(e, grow))) return rc;
@@ -1411,7 +1396,7 @@ int reset_state(mdx_ensemble *e) {
   }
   if ((rc = write_ctl(e))) return rc;
   // LDS per tile sized to the tiles as built, with room to grow
-  const int cap = smax + smax / 8 + 64;
+  const int cap = std::max(smax, std::min(smax + smax / 8 + 64, kMaxPlane));
   if (cap < e->stage_cap) {
     if ((rc = set_stage_cap(e, cap)) || (rc = size_step_grid(e))) return rc;
   }
@@ -1488,7 +1473,7 @@ int mdx_ensemble_create(const mdx_params *prm, int nrep, const int *n, const dou
     cb += rp.ncell;
     // partners within SL at the mean density, with room for fluctuations
     const double mean = 4.18879 * SL * SL * SL * (n[r] / V);
-    e->kcap[r] = ((int)(mean * 1.4) + 24 + 7) & ~7;
+    e->kcap[r] = ((int)(mean * 1.4) + 24 + kPlanes * (kILP - 1) + 7) & ~7;
   }
   e->NC = cb;
   int rc = MDX_OK;
@@ -1541,7 +1526,7 @@ int mdx_ensemble_create(const mdx_params *prm, int nrep, const int *n, const dou
     }
     const int want = (int)(dmax * 12.0 * 1.25) + 256;   // one plane: 3 rows of 4 cells
     const int lim = (int)((e->lds_limit - 1024) / 24);
-    rc = set_stage_cap(e, std::min(want, lim));
+    rc = set_stage_cap(e, std::min(std::min(want, lim), kMaxPlane));
   }
   if (!rc) rc = size_step_grid(e);
   if (rc) {

@@ -60,7 +60,6 @@ struct DevStatus {
   double kmax_d;                   // longest neighbour list (as a double)
   double stage_max;                // largest tile staging (particles)
   double plane_max;                // largest staging of one plane of a tile
-  double pass_max;                 // largest staging of one pass of the kick
   long long canon_count;
   double sums[4];                  // KE, PE, virial, spare
 };
@@ -414,6 +413,23 @@ __device__ void tile_stage(int tile, const Cells &c, const double4 *__restrict__
   stage_segments(sm, s3, [&](int, int slot) { return pos[slot]; });
 }
 
+// Partners of one list column in list order, for kernels that stage the
+// whole tile: f(o) with o = 3 x the partner's index in that staging;
+// sentinel entries are skipped.
+template <class F>
+__device__ __forceinline__ void for_each_partner(const unsigned short *__restrict__ lst, int m,
+                                                 int nb, const TileSmem &sm, F f) {
+  for (int p = 0; p < kPlanes; p++) {
+    const int pb = sm.seg_base[6 * p];
+    const int lim = kEntryBytes * (sm.seg_base[6 * p + 6] - pb);
+    const int ke = plane_end(p, nb, m);
+    for (int k = plane_begin(p, nb); k < ke; k++) {
+      const int e = lst[(size_t)k * kTile];
+      if (e < lim) f(3 * (pb + e / kEntryBytes));
+    }
+  }
+}
+
 // rows (in row_of order) -> number of tiles
 __global__ void k_row_tiles(Geom g, Cells c, int *__restrict__ row_tiles) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -432,7 +448,7 @@ __global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict_
                               double *__restrict__ partial) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int rows = g.ms[1] * g.ms[2];
-  double smax = 0.0, pmax = 0.0, qmax = 0.0;
+  double smax = 0.0, pmax = 0.0;
   if (r < rows) {
     int y, z;
     row_of(g, r, &y, &z);
@@ -462,23 +478,21 @@ __global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict_
       smax = fmax(smax, (double)stage);
       for (int p = 0; p < kPlanes; p++)
         pmax = fmax(pmax, (double)(ts[kSeg + 6 * p + 6] - ts[kSeg + 6 * p]));
-      for (int p = 0; p < kPasses; p++)
-        qmax = fmax(qmax, (double)(ts[kSeg + 6 * pass_plane(p + 1)] - ts[kSeg + 6 * pass_plane(p)]));
     }
   }
   block_max_to(smax, partial);
   block_max_to(pmax, partial + gridDim.x);
-  block_max_to(qmax, partial + 2 * gridDim.x);
 }
 
-// Full neighbour list per tile: lane = real i, every partner j != i (real
-// or image) of its 27 cells with r^2 <= SL^2, r^2 formed as the reference's
-// AVX2 search, fma(dx,dx, fma(dy,dy, dz*dz)) with dq = q_j - q_i
-// (meshlist.cc:354-362).  Entry k of lane l sits at
-// list[tile * kcap * kTile + k * kTile + l] and is 3 x the partner's index
-// in the tile's staging.  The cells are visited plane by plane (dz = -1, 0,
-// +1), each plane staged alone; nb[i] holds where planes 1 and 2 start in
-// the lane's list (low and high 16 bits).
+// Full neighbour list per tile: one list column per real, every partner
+// j != i (real or image) of its 27 cells with r^2 <= SL^2, r^2 formed as
+// the reference's AVX2 search, fma(dx,dx, fma(dy,dy, dz*dz)) with dq = q_j
+// - q_i (meshlist.cc:354-362).  Entry k of column l sits at
+// list[tile * kcap * kTile + k * kTile + l] in the plane-list form: the
+// cells are visited plane by plane (dz = -1, 0, +1), each plane staged
+// alone, its entries byte offsets in that staging, padded with its
+// sentinel to a multiple of kILP; nb[i] holds where planes 1 and 2 start
+// (low and high 16 bits) and nn[i] the padded length.
 __global__ void __launch_bounds__(kTile) k_build_tiles(
     const double4 *__restrict__ pos, const TileDesc *__restrict__ tiles, int ntiles, Geom g,
     Cells c, int n, int kcap, unsigned short *__restrict__ list, int *__restrict__ nn,
@@ -536,12 +550,13 @@ __global__ void __launch_bounds__(kTile) k_build_tiles(
             const int e = 3 * (o - pb);
             const double r2 = list_r2(sx[e], sy[e], sz[e], qi.x, qi.y, qi.z);
             if (r2 <= g.SL2) {
-              if (cnt < kcap) out[(size_t)cnt * kTile] = (unsigned short)(3 * o);
+              if (cnt < kcap) out[(size_t)cnt * kTile] = plane_entry(o - pb);
               cnt++;
             }
           }
         }
       }
+      pad_plane<kTile>(out, cnt, kcap, sm.seg_base[6 * p + 6] - pb);
     }
     if (p == 0) b1 = cnt;
     if (p == 1) b2 = cnt;
@@ -678,52 +693,28 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
   tile_load_segments(c, tile, sm);
   const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
   // kILP independent partner chains per lane: entry k of the list feeds
-  // chain k % kILP, and the chains are summed in a fixed order at the end
+  // chain k % kILP, and the chains are summed in a fixed order at the end.
+  // A sentinel entry adds exactly +0.0 (the accumulators are never -0.0).
   double ax[kILP], ay[kILP], az[kILP];
 #pragma unroll
   for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
-  for (int p = 0; p < kPasses; p++) {
-    if (p > 0) __syncthreads();   // the previous pass is read
-    const int pl0 = pass_plane(p), pl1 = pass_plane(p + 1);
-    stage_range(sm, 6 * pl0, 6 * pl1, sx, [&](int, int slot) { return pos[slot]; });
-    const int base3 = 3 * sm.seg_base[6 * pl0];
-    const int far3 = 3 * (sm.seg_base[6 * pl1] - sm.seg_base[6 * pl0]);   // the sentinel
-    const int kb = pl0 == 0 ? 0 : (pl0 == 1 ? (bnd & 0xffff) : (bnd >> 16));
-    const int ke = pl1 == 3 ? m : (pl1 == 1 ? (bnd & 0xffff) : (bnd >> 16));
-    if (!act || kb >= ke) continue;
-    // groups of kILP entries aligned on the whole list; an entry of a group
-    // outside [kb, ke) points at the sentinel and adds exactly +0.0 (the
-    // accumulators are never -0.0), so the sums are those of one pass over
-    // the list.  The list is padded by a group, so no load is clamped; the
-    // entries of the next group are loaded while this group computes.
-    int g0 = kb - kb % kILP;
-    const unsigned short *lp = lst + (size_t)g0 * kTile;
-    int nxt[kILP];
-#pragma unroll
-    for (int u = 0; u < kILP; u++) nxt[u] = lp[u * kTile];
-    for (; g0 < ke; g0 += kILP, lp += kILP * kTile) {
-      int cur[kILP];
-#pragma unroll
-      for (int u = 0; u < kILP; u++) cur[u] = nxt[u];
-      if (g0 + kILP < ke) {
-#pragma unroll
-        for (int u = 0; u < kILP; u++) nxt[u] = lp[(kILP + u) * kTile];
+  const double xi = qi.x, yi = qi.y, zi = qi.z, CL2 = g.CL2;
+  for (int p = 0; p < kPlanes; p++) {
+    if (p > 0) __syncthreads();   // the previous plane is read
+    stage_range(sm, 6 * p, 6 * p + 6, sx, [&](int, int slot) { return pos[slot]; });
+    if (!act) continue;
+    plane_groups<kTile>(lst, plane_begin(p, bnd), plane_end(p, bnd, m), stage_mem,
+                        [&](int u, double xj, double yj, double zj) {
+      if (SCALED) {
+        const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+        const double df = pair_df_lj(dx * dx + dy * dy + dz * dz, lj, dt);
+        ax[u] += df * dx;
+        ay[u] += df * dy;
+        az[u] += df * dz;
+      } else {
+        kick_term(xj, yj, zj, xi, yi, zi, pk, CL2, ax[u], ay[u], az[u]);
       }
-#pragma unroll
-      for (int u = 0; u < kILP; u++) {
-        const bool v = (unsigned)(g0 + u - kb) < (unsigned)(ke - kb);
-        const int o = v ? cur[u] - base3 : far3;
-        if (SCALED) {
-          const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
-          const double df = pair_df_lj(dx * dx + dy * dy + dz * dz, lj, dt);
-          ax[u] += df * dx;
-          ay[u] += df * dy;
-          az[u] += df * dz;
-        } else {
-          kick_term(sx[o], sy[o], sz[o], qi.x, qi.y, qi.z, pk, g.CL2, ax[u], ay[u], az[u]);
-        }
-      }
-    }
+    });
   }
   double v2 = 0.0;
   if (act) {
@@ -832,8 +823,8 @@ template <bool GRAD>
 __global__ void __launch_bounds__(kTile) k_lj_observe_tiles(
     const double4 *__restrict__ pos, const double4 *__restrict__ mom,
     const TileDesc *__restrict__ tiles, int ntiles, Geom g, Cells c, int n, int kcap,
-    const unsigned short *__restrict__ list, const int *__restrict__ nn, LJ lj, double cw,
-    double ce, double ck, double4 *__restrict__ aq, double4 *__restrict__ ap,
+    const unsigned short *__restrict__ list, const int *__restrict__ nn, const int *__restrict__ nb,
+    LJ lj, double cw, double ce, double ck, double4 *__restrict__ aq, double4 *__restrict__ ap,
     double *__restrict__ partial) {
   extern __shared__ double stage_mem[];
   __shared__ TileSmem sm;
@@ -856,8 +847,7 @@ __global__ void __launch_bounds__(kTile) k_lj_observe_tiles(
     const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
     double pe = 0.0, w = 0.0, dpe_s = 0.0, dw_s = 0.0;
     double gwx = 0.0, gwy = 0.0, gwz = 0.0, gex = 0.0, gey = 0.0, gez = 0.0;
-    for (int k = 0; k < nn[i]; k++) {
-      const int o = lst[(size_t)k * kTile];
+    for_each_partner(lst, nn[i], nb[i], sm, [&](int o) {
       const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
       const double s2 = dx * dx + dy * dy + dz * dz;
       double phi, dphi, V;
@@ -876,7 +866,7 @@ __global__ void __launch_bounds__(kTile) k_lj_observe_tiles(
         gey += phi * dy;
         gez += phi * dz;
       }
-    }
+    });
     if (!GRAD) {
       acc[0] = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z;
       acc[1] = 0.5 * pe;
@@ -929,7 +919,8 @@ __global__ void k_adj_pre(const double4 *__restrict__ ap, const double4 *__restr
 __global__ void __launch_bounds__(kTile) k_adj_hvp_tiles(
     const double4 *__restrict__ pos, const TileDesc *__restrict__ tiles, int ntiles, Geom g,
     Cells c, int n, int kcap, const unsigned short *__restrict__ list, const int *__restrict__ nn,
-    LJ lj, double dt, const double4 *__restrict__ ext, double4 *__restrict__ aq,
+    const int *__restrict__ nb, LJ lj, double dt, const double4 *__restrict__ ext,
+    double4 *__restrict__ aq,
     double4 *__restrict__ ap, double *__restrict__ partial) {
   extern __shared__ double stage_mem[];
   __shared__ TileSmem sm;
@@ -955,8 +946,7 @@ __global__ void __launch_bounds__(kTile) k_adj_hvp_tiles(
     const double4 ai = ext[i];
     const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
     double jx = 0.0, jy = 0.0, jz = 0.0, te = 0.0, ts = 0.0;
-    for (int k = 0; k < nn[i]; k++) {
-      const int o = lst[(size_t)k * kTile];
+    for_each_partner(lst, nn[i], nb[i], sm, [&](int o) {
       const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
       const double s2 = dx * dx + dy * dy + dz * dz;
       double phi, dphi, V;
@@ -970,7 +960,7 @@ __global__ void __launch_bounds__(kTile) k_adj_hvp_tiles(
       const double ad = ai.x * dx + ai.y * dy + ai.z * dz;
       te += phi * ad;                         // eps dF/deps . a
       ts += (phi + s2 * dphi) * ad;           // -(sigma/2) dF/dsigma . a
-    }
+    });
     acc[0] = dt * te / lj.eps;
     acc[1] = dt * (-2.0 / lj.sigma) * ts;
     double4 a = aq[i];
@@ -1009,8 +999,8 @@ __global__ void k_gather4_by_tag(const double4 *__restrict__ by_id, const int *_
 __global__ void __launch_bounds__(kTile) k_observe_tiles(
     const double4 *__restrict__ pos, const double4 *__restrict__ mom,
     const TileDesc *__restrict__ tiles, int ntiles, Geom g, Cells c, int n, int kcap,
-    const unsigned short *__restrict__ list, const int *__restrict__ nn, double C0, double C2,
-    double *__restrict__ partial) {
+    const unsigned short *__restrict__ list, const int *__restrict__ nn, const int *__restrict__ nb,
+    double C0, double C2, double *__restrict__ partial) {
   extern __shared__ double stage_mem[];
   __shared__ TileSmem sm;
   __shared__ double red[3][kTile];
@@ -1030,13 +1020,11 @@ __global__ void __launch_bounds__(kTile) k_observe_tiles(
     ke = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z;
     const double4 qi = pos[i];
     const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
-    for (int k = 0; k < nn[i]; k++) {
-      const int o = lst[(size_t)k * kTile];
+    for_each_partner(lst, nn[i], nb[i], sm, [&](int o) {
       const double dx = qi.x - sx[o], dy = qi.y - sy[o], dz = qi.z - sz[o];
       const double r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 > g.CL2) continue;
-      observe_term(r2, C0, C2, pe, vir);
-    }
+      if (r2 <= g.CL2) observe_term(r2, C0, C2, pe, vir);
+    });
     pe *= 0.5;
     vir *= 0.5;
   }
@@ -1130,8 +1118,9 @@ __device__ __forceinline__ void partner_id(int j, int n, const int *tag, const i
 template <bool WRITE>
 __global__ void __launch_bounds__(kTile) k_canonical_tiles(
     const TileDesc *__restrict__ tiles, int ntiles, Geom g, Cells c, int n, int kcap,
-    const unsigned short *__restrict__ list, const int *__restrict__ nn, const int *__restrict__ tag,
-    const int *__restrict__ gsrc, const int *__restrict__ gcode, const int *__restrict__ offset,
+    const unsigned short *__restrict__ list, const int *__restrict__ nn, const int *__restrict__ nb,
+    const int *__restrict__ tag, const int *__restrict__ gsrc, const int *__restrict__ gcode,
+    const int *__restrict__ offset,
     int *__restrict__ count, int *__restrict__ out) {
   __shared__ TileSmem sm;
   const int tile = blockIdx.x;
@@ -1145,8 +1134,8 @@ __global__ void __launch_bounds__(kTile) k_canonical_tiles(
   const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
   int cnum = 0;
   int w = WRITE ? offset[i] : 0;
-  for (int k = 0; k < nn[i]; k++) {
-    const int o = lst[(size_t)k * kTile] / 3;
+  for_each_partner(lst, nn[i], nb[i], sm, [&](int o3) {
+    const int o = o3 / 3;
     int sgi = 0;
     while (o >= sm.seg_base[sgi + 1]) sgi++;
     const int j = sm.seg_start[sgi] + (o - sm.seg_base[sgi]);
@@ -1161,7 +1150,7 @@ __global__ void __launch_bounds__(kTile) k_canonical_tiles(
       }
       cnum++;
     }
-  }
+  });
   if (!WRITE) count[i] = cnum;
 }
 
@@ -1222,7 +1211,7 @@ struct mdx_system {
   TileDesc *tiles = nullptr;
   int *tseg = nullptr;        // [tiles_cap][kTSeg]
   unsigned char *tperm = nullptr;   // [tiles_cap][kTile] real of each list column
-  int ntiles = 0, tiles_cap = 0, stage_max = 0, plane_max = 0, pass_max = 0;
+  int ntiles = 0, tiles_cap = 0, stage_max = 0, plane_max = 0;
   int *row_tiles = nullptr, *row_off = nullptr;
   unsigned short *list16 = nullptr;
   size_t list16_elems = 0;
@@ -1370,7 +1359,6 @@ int ensure_list_capacity(mdx_system *s, int kcap) {
 // dynamic LDS of the tile kernels: the staged particles plus the sentinel
 size_t stage_bytes(const mdx_system *s) { return (size_t)(s->stage_max + 1) * 3 * sizeof(double); }
 size_t plane_bytes(const mdx_system *s) { return (size_t)(s->plane_max + 1) * 3 * sizeof(double); }
-size_t pass_bytes(const mdx_system *s) { return (size_t)(s->pass_max + 1) * 3 * sizeof(double); }
 
 Cells cells_of(const mdx_system *s) {
   return Cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out, s->tseg, s->tperm};
@@ -1476,19 +1464,21 @@ int build_pairlist(mdx_system *s) {
   }
   cells.tseg = s->tseg;
   cells.tperm = s->tperm;
-  if ((rc = ensure_partials(s, std::max(tile_grid(s->ntiles), 3 * nblk(rows))))) return rc;
+  if ((rc = ensure_partials(s, std::max(tile_grid(s->ntiles), 2 * nblk(rows))))) return rc;
   k_write_tiles<<<nblk(rows), kBlock, 0, s->stream>>>(s->geom, cells, n, s->pos, s->row_tiles,
                                                        s->row_off, s->tiles, s->tseg, s->pmax);
   k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(rows), &s->dstat->stage_max);
   k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax + nblk(rows), nblk(rows), &s->dstat->plane_max);
-  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax + 2 * nblk(rows), nblk(rows), &s->dstat->pass_max);
   if ((rc = sync_status(s))) return rc;
   s->stage_max = (int)s->hstat->stage_max;
   s->plane_max = (int)s->hstat->plane_max;
-  s->pass_max = (int)s->hstat->pass_max;
   if (stage_bytes(s) + 4096 > s->lds_limit)
     return fail(s, MDX_ERR_CAPACITY, "tile staging of %d particles exceeds LDS (%zu B)",
                 s->stage_max, s->lds_limit);
+  if (s->plane_max > kMaxPlane)
+    return fail(s, MDX_ERR_CAPACITY,
+                "a tile plane of %d particles exceeds the 16-bit list offsets (%d particles)",
+                s->plane_max, kMaxPlane);
   if ((rc = ensure_list_capacity(s, s->kcap))) return rc;
   // 6. neighbour lists, one pass at the current width, again if it overflows
   for (int attempt = 0; attempt < 3; attempt++) {
@@ -1519,7 +1509,7 @@ int build_pairlist(mdx_system *s) {
 int kick(mdx_system *s) {
   const int grid = tile_grid(s->ntiles);
   ev_record(s, 2);
-  k_kick_tiles<0><<<grid, kTile, pass_bytes(s), s->stream>>>(
+  k_kick_tiles<0><<<grid, kTile, plane_bytes(s), s->stream>>>(
       s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), s->n, s->kcap, s->list16, s->nn,
       s->nb, make_pairk(s->C2, s->prm.dt), s->prm.dt, nullptr, s->pmax);
   ev_record(s, 3);
@@ -1559,7 +1549,7 @@ int run_batch(mdx_system *s, int want, int *ran) {
     const double4 *cur = (b & 1) ? s->pos_mid : s->pos_alt;
     double4 *nxt = (b & 1) ? s->pos_alt : s->pos_mid;
     if (s->timing) (void)hipEventRecord(s->evk[2 * b], s->stream);
-    k_kick_tiles<3><<<grid, kTile, pass_bytes(s), s->stream>>>(
+    k_kick_tiles<3><<<grid, kTile, plane_bytes(s), s->stream>>>(
         cur, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn,
         s->nb, make_pairk(s->C2, dt), dt, s->pos, s->pmax, LJ{}, &s->dctl->halt, nxt);
     if (s->timing) (void)hipEventRecord(s->evk[2 * b + 1], s->stream);
@@ -1661,8 +1651,8 @@ int reduce_observe(mdx_system *s, double *ke, double *pe, double *vir) {
   const int nb = tile_grid(s->ntiles);
   k_observe_tiles<<<nb, kTile, stage_bytes(s), s->stream>>>(s->pos, s->mom, s->tiles, s->ntiles,
                                                             s->geom, cells_of(s), s->n, s->kcap,
-                                                            s->list16, s->nn, s->C0, s->C2,
-                                                            s->partial);
+                                                            s->list16, s->nn, s->nb, s->C0,
+                                                            s->C2, s->partial);
   k_sum_partials<<<1, kBlock, 0, s->stream>>>(s->partial, nb, s->dstat->sums);
   int rc = launch_check(s, "observe");
   if (rc) return rc;
@@ -1718,7 +1708,7 @@ int lj_step(mdx_system *s, const LJ &lj, double4 *mid) {
   k_drift_refresh<<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->gsrc, s->gcode, n, g,
                                                           s->prm.dt * 0.5, s->geom.L[0],
                                                           s->geom.L[1], s->geom.L[2], mid, nullptr);
-  k_kick_tiles<2, true><<<grid, kTile, pass_bytes(s), s->stream>>>(
+  k_kick_tiles<2, true><<<grid, kTile, plane_bytes(s), s->stream>>>(
       mid, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb,
       make_pairk(s->C2, s->prm.dt), s->prm.dt, s->pos, s->pmax, lj);
   return launch_check(s, "lj step");
@@ -1777,7 +1767,7 @@ int lj_trajectory(mdx_system *s, double eps, double sigma, int nsteps, int nobs,
         const int nb = tile_grid(s->ntiles);
         k_lj_observe_tiles<false><<<nb, kTile, stage_bytes(s), s->stream>>>(
             s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16,
-            s->nn, lj, 0.0, 0.0, 0.0, nullptr, nullptr, s->partial);
+            s->nn, s->nb, lj, 0.0, 0.0, 0.0, nullptr, nullptr, s->partial);
         k_sum_partials_k<3><<<1, kBlock, 0, s->stream>>>(s->partial, nb, obs + 3 * (size_t)k, 1.0);
       }
     }
@@ -1851,14 +1841,14 @@ int lj_trajectory(mdx_system *s, double eps, double sigma, int nsteps, int nobs,
           // sensitivity of the observation after step k
           k_lj_observe_tiles<true><<<nb, kTile, stage_bytes(s), s->stream>>>(
               st_end + (size_t)(k - k0) * totcap, st_mom + (size_t)(k - k0) * n, s->tiles,
-              s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, lj, cw, ce, ck, aq,
-              ap, s->partial);
+              s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, lj, cw, ce,
+              ck, aq, ap, s->partial);
           k_sum_partials_k<4><<<1, kBlock, 0, s->stream>>>(s->partial, nb, theta + 4 * (size_t)slot++, 1.0);
         }
         k_adj_pre<<<nblk(T), kBlock, 0, s->stream>>>(ap, aq, s->gsrc, n, s->g, s->prm.dt * 0.5, aext);
         k_adj_hvp_tiles<<<nb, kTile, adj_stage_bytes(s), s->stream>>>(
             st_mid + (size_t)(k - k0) * totcap, s->tiles, s->ntiles, s->geom, cells_of(s), n,
-            s->kcap, s->list16, s->nn, lj, s->prm.dt, aext, aq, ap, s->partial);
+            s->kcap, s->list16, s->nn, s->nb, lj, s->prm.dt, aext, aq, ap, s->partial);
         k_sum_partials_k<2><<<1, kBlock, 0, s->stream>>>(s->partial, nb, theta + 4 * (size_t)slot++, 1.0);
         if ((rc = launch_check(s, "adjoint step"))) return rc;
       }
@@ -2036,7 +2026,7 @@ int mdx_set_state(mdx_system *s, int n, const double *q3, const double *p3) {
   const int gguess = (int)std::min(8.0 * n, 2.0 * n * (Vx - V) / V + 4096);
   if ((rc = ensure_image_capacity(s, gguess))) return rc;
   const double rho = n / V;
-  const int kguess = (int)(4.18879 * s->SL * s->SL * s->SL * rho * 1.35) + 16;
+  const int kguess = (int)(4.18879 * s->SL * s->SL * s->SL * rho * 1.35) + 16 + kPlanes * (kILP - 1);
   s->kcap = std::max(kguess, s->kcap);
   // positions staged in io_buf, momenta in vec_out (double4[cap] >= 3n doubles)
   double *pbuf = reinterpret_cast<double *>(s->vec_out);
@@ -2157,7 +2147,7 @@ int mdx_observe(mdx_system *s, double *ke, double *pe, double *vir) {
 int mdx_forces(mdx_system *s, double *f3) {
   if (!s || !f3) return fail(s, MDX_ERR_ARG, "null argument");
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
-  k_kick_tiles<1><<<tile_grid(s->ntiles), kTile, pass_bytes(s), s->stream>>>(
+  k_kick_tiles<1><<<tile_grid(s->ntiles), kTile, plane_bytes(s), s->stream>>>(
       s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), s->n, s->kcap, s->list16, s->nn,
       s->nb, make_pairk(s->C2, 1.0), 1.0, s->vec_out, nullptr);
   k_scatter_by_tag<<<nblk(s->n), kBlock, 0, s->stream>>>(s->vec_out, s->tag, s->n, s->io_buf);
@@ -2171,8 +2161,8 @@ int mdx_pairs_canonical(mdx_system *s, int32_t *out, int64_t cap, int64_t *count
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
   const int n = s->n;
   k_canonical_tiles<false><<<s->ntiles, kTile, 0, s->stream>>>(
-      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->tag, s->gsrc,
-      s->gcode, nullptr, s->i_buf, nullptr);
+      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, s->tag,
+      s->gsrc, s->gcode, nullptr, s->i_buf, nullptr);
   int rc = scan(s, s->i_buf, s->goff, n, &s->dstat->ghost_total);
   if (rc) return rc;
   if ((rc = sync_status(s))) return rc;
@@ -2182,8 +2172,8 @@ int mdx_pairs_canonical(mdx_system *s, int32_t *out, int64_t cap, int64_t *count
   int32_t *dout = nullptr;
   HIPCHK(s, hipMalloc((void **)&dout, sizeof(int32_t) * 3 * (size_t)std::max(total, 1LL)));
   k_canonical_tiles<true><<<s->ntiles, kTile, 0, s->stream>>>(
-      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->tag, s->gsrc,
-      s->gcode, s->goff, nullptr, dout);
+      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, s->tag,
+      s->gsrc, s->gcode, s->goff, nullptr, dout);
   std::vector<int32_t> h(3 * (size_t)total);
   hipError_t e = hipMemcpyAsync(h.data(), dout, sizeof(int32_t) * 3 * (size_t)total,
                                 hipMemcpyDeviceToHost, s->stream);

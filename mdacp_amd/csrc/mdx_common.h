@@ -22,14 +22,14 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// 1/x from the hardware estimate and two Newton steps (each squares the
-// relative error); within an ulp or two of the IEEE quotient.
+// 1/x from the hardware estimate r and one third-order correction:
+// with e = 1 - x r, r (1 + e + e^2) = (1 - e^3) / x, so the relative error
+// of the estimate is cubed (two Newton steps raise it to the fourth power
+// at one more operation).  Within an ulp of the IEEE quotient.
 __device__ __forceinline__ double recip(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = __fma_rn(-x, r, 1.0);
-  r = __fma_rn(r, e, r);
-  e = __fma_rn(-x, r, 1.0);
-  return __fma_rn(r, e, r);
+  const double r = __builtin_amdgcn_rcp(x);
+  const double e = __fma_rn(-x, r, 1.0);
+  return __fma_rn(r, __fma_rn(e, e, e), r);
 }
 
 struct Geom {
@@ -131,17 +131,82 @@ __device__ __forceinline__ void stage_segments(const TileSmem &sm, double *s3, L
 }
 
 // A tile's partners lie in three z-planes of cell rows (dz = -1, 0, +1):
-// segments 6p .. 6p+5, list rows 3p .. 3p+2.  The list build stages one
-// plane at a time, a third of the LDS of the whole tile.
+// segments 6p .. 6p+5, list rows 3p .. 3p+2.  Every kernel that walks the
+// lists stages one plane at a time, a third of the LDS of the whole tile.
 constexpr int kPlanes = 3;
 
-// The single-system kick walks its partners in kPasses passes of whole
-// planes, pass p covering planes [pass_plane(p), pass_plane(p + 1)): fewer
-// passes stage more at once (more LDS per tile) but lose less to the lanes
-// of a wavefront reaching a plane boundary at different list entries.
-constexpr int kPasses = 3;
-__host__ __device__ constexpr int pass_plane(int p) {
-  return kPasses == 3 ? p : (kPasses == 2 ? (p == 0 ? 0 : (p == 1 ? 1 : 3)) : (p == 0 ? 0 : 3));
+// Plane lists.  A real's list holds its partners plane by plane; an entry
+// is the partner's byte offset in its plane's LDS staging (kEntryBytes x
+// its index there), so a kernel reads the partner at staging + entry with
+// no index arithmetic.  Each plane's entries are padded to a multiple of
+// kILP with the plane's sentinel (the slot after its last particle, which
+// holds kFar), so no group of kILP entries straddles two planes and none
+// needs a mask.  A plane may stage at most kMaxPlane particles: the
+// sentinel's offset must fit 16 bits.
+constexpr int kEntryBytes = 3 * (int)sizeof(double);
+constexpr int kMaxPlane = 65535 / kEntryBytes - 1;
+
+__device__ __forceinline__ unsigned short plane_entry(int index_in_plane) {
+  return (unsigned short)(kEntryBytes * index_in_plane);
+}
+
+// xyz of the partner at entry e of a plane staged at s3
+__device__ __forceinline__ const double *staged_at(const double *s3, unsigned e) {
+  return reinterpret_cast<const double *>(reinterpret_cast<const char *>(s3) + e);
+}
+
+// Where plane p's entries of a list start and end, from the list length m
+// and nb = (start of plane 1) | (start of plane 2) << 16.
+__device__ __forceinline__ int plane_begin(int p, int nb) {
+  return p == 0 ? 0 : (p == 1 ? (nb & 0xffff) : (nb >> 16));
+}
+__device__ __forceinline__ int plane_end(int p, int nb, int m) {
+  return p == 0 ? (nb & 0xffff) : (p == 1 ? (nb >> 16) : m);
+}
+
+// Close plane `len` particles long in a list being built (column stride S):
+// sentinel entries up to the next multiple of kILP, stored while below kcap.
+template <int S>
+__device__ __forceinline__ void pad_plane(unsigned short *out, int &cnt, int kcap, int len) {
+  const unsigned short pad = plane_entry(len);
+  for (; cnt % kILP; cnt++)
+    if (cnt < kcap) out[(size_t)cnt * S] = pad;
+}
+
+// Entries [kb, ke) of one lane's list, both multiples of kILP, column
+// stride S, partners staged at s3: entry k goes to term(k % kILP, xj, yj,
+// zj) in list order.  The groups go in pairs, each group's entries loading
+// while the previous group computes, into the other of two register sets;
+// a pair is one straight-line block, so the compiler keeps every load a
+// group ahead of its use.  The last load of a plane may reload its last
+// group (never used).
+template <int S, class Term>
+__device__ __forceinline__ void plane_groups(const unsigned short *__restrict__ lst, int kb, int ke,
+                                             const double *s3, Term term) {
+  if (kb >= ke) return;
+  unsigned ga[kILP], gb[kILP];
+  auto load = [&](unsigned(&g)[kILP], int k) {
+    const unsigned short *lp = lst + (size_t)k * S;
+#pragma unroll
+    for (int u = 0; u < kILP; u++) g[u] = lp[u * S];
+  };
+  auto run = [&](const unsigned(&g)[kILP]) {
+#pragma unroll
+    for (int u = 0; u < kILP; u++) {
+      const double *q = staged_at(s3, g[u]);
+      term(u, q[0], q[1], q[2]);
+    }
+  };
+  const int last = ke - kILP, ng = (ke - kb) / kILP;
+  load(ga, kb);
+  int k = kb;
+  for (int j = 0; j < ng / 2; j++, k += 2 * kILP) {
+    load(gb, k + kILP);
+    run(ga);
+    load(ga, min(k + 2 * kILP, last));
+    run(gb);
+  }
+  if (ng & 1) run(ga);
 }
 
 // Order of the interior cell rows (y, z) in the tile array: strips of h
