@@ -609,8 +609,8 @@ __global__ void __launch_bounds__(T) k_e_build(EnsDev d) {
       e_segments(d, rp, t, sm);
       const int self = act ? sm.seg_base[8] + (i - sm.seg_start[8]) : -1;
       const int cx = interior_coord(qi.x, g.inv_w[0], g.ms[0]);
-      unsigned short *out = d.list + rp.lbase + (size_t)s * rp.kcap * T + lane;
-      int cnt = 0, b1 = 0, b2 = 0;
+      GroupWriter<T> gw(d.list + rp.lbase + (size_t)s * rp.kcap * T + (size_t)lane * kILP, rp.kcap);
+      int b1 = 0, b2 = 0;
       // the 27 cells plane by plane (dz = -1, 0, +1), each plane staged alone
       for (int pl = 0; pl < kPlanes; pl++) {
         if (pl > 0) __syncthreads();
@@ -633,18 +633,17 @@ __global__ void __launch_bounds__(T) k_e_build(EnsDev d) {
               for (int o = lo; o < lo + len; o++) {
                 if (o == self) continue;
                 const int e = 3 * (o - pb);
-                if (list_r2(sx[e], sy[e], sz[e], qi.x, qi.y, qi.z) <= g.SL2) {
-                  if (cnt < rp.kcap) out[(size_t)cnt * T] = plane_entry(o - pb);
-                  cnt++;
-                }
+                if (list_r2(sx[e], sy[e], sz[e], qi.x, qi.y, qi.z) <= g.SL2)
+                  gw.put(plane_entry(o - pb));
               }
             }
           }
-          pad_plane<T>(out, cnt, rp.kcap, sm.seg_base[6 * pl + 6] - pb);
+          gw.pad_plane(sm.seg_base[6 * pl + 6] - pb);
         }
-        if (pl == 0) b1 = cnt;
-        if (pl == 1) b2 = cnt;
+        if (pl == 0) b1 = gw.cnt;
+        if (pl == 1) b2 = gw.cnt;
       }
+      const int cnt = gw.cnt;
       if (act) {
         d.nn[i] = cnt < rp.kcap ? cnt : rp.kcap;
         d.nb[i] = min(b1, rp.kcap) | (min(b2, rp.kcap) << 16);
@@ -719,7 +718,7 @@ __global__ void __launch_bounds__(T) k_e_step(EnsDev d) {
       pm = d.mom[i];
     }
     e_segments(d, rp, t, sm);
-    const unsigned short *lst = d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * T + lane;
+    const unsigned short *lst = d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * T + (size_t)lane * kILP;
     const double CL2 = rp.g.CL2, xi = qi.x, yi = qi.y, zi = qi.z;
     // the single-system kick's sums, bit for bit: entry k feeds chain
     // k % kILP; partners are staged one z-plane at a time
@@ -938,14 +937,14 @@ __global__ void __launch_bounds__(T) k_e_observe(EnsDev d, double C0, double C2)
   double ke = 0.0, pe = 0.0, vir = 0.0;
   const int m = act ? d.nn[i] : 0, bnd = act ? d.nb[i] : 0;
   const double4 qi = act ? d.qe[i] : make_double4(0.0, 0.0, 0.0, 0.0);
-  const unsigned short *lst = d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * T + lane;
+  const unsigned short *lst = d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * T + (size_t)lane * kILP;
   for (int pl = 0; pl < kPlanes; pl++) {
     if (pl > 0) __syncthreads();
     e_stage_plane(d, rp, pl, d.qe, mid, sm, sx);
     // a sentinel entry lies past the cutoff
     const int kend = plane_end(pl, bnd, m);
     for (int k = plane_begin(pl, bnd); k < kend; k++) {
-      const double *q = staged_at(stage_mem, lst[(size_t)k * T]);
+      const double *q = staged_at(stage_mem, lst[list_index<T>(k)]);
       const double dx = qi.x - q[0], dy = qi.y - q[1], dz = qi.z - q[2];
       const double r2 = dx * dx + dy * dy + dz * dz;
       if (r2 > rp.g.CL2) continue;

@@ -424,7 +424,7 @@ __device__ __forceinline__ void for_each_partner(const unsigned short *__restric
     const int lim = kEntryBytes * (sm.seg_base[6 * p + 6] - pb);
     const int ke = plane_end(p, nb, m);
     for (int k = plane_begin(p, nb); k < ke; k++) {
-      const int e = lst[(size_t)k * kTile];
+      const int e = lst[list_index<kTile>(k)];
       if (e < lim) f(3 * (pb + e / kEntryBytes));
     }
   }
@@ -488,7 +488,7 @@ __global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict_
 // j != i (real or image) of its 27 cells with r^2 <= SL^2, r^2 formed as
 // the reference's AVX2 search, fma(dx,dx, fma(dy,dy, dz*dz)) with dq = q_j
 // - q_i (meshlist.cc:354-362).  Entry k of column l sits at
-// list[tile * kcap * kTile + k * kTile + l] in the plane-list form: the
+// list[tile * kcap * kTile + list_index<kTile>(k) + l kILP] in the plane-list form: the
 // cells are visited plane by plane (dz = -1, 0, +1), each plane staged
 // alone, its entries byte offsets in that staging, padded with its
 // sentinel to a multiple of kILP; nb[i] holds where planes 1 and 2 start
@@ -525,8 +525,8 @@ __global__ void __launch_bounds__(kTile) k_build_tiles(
   }
   const int self = act ? sm.seg_base[8] + (i - sm.seg_start[8]) : -1;   // in row (0, 0)
   const int cx = interior_coord(qi.x, g.inv_w[0], g.ms[0]);
-  unsigned short *out = list + (size_t)tile * kcap * kTile + col;
-  int cnt = 0, b1 = 0, b2 = 0;
+  GroupWriter<kTile> gw(list + (size_t)tile * kcap * kTile + (size_t)col * kILP, kcap);
+  int b1 = 0, b2 = 0;
   for (int p = 0; p < kPlanes; p++) {
     if (p > 0) __syncthreads();   // the previous plane is read
     stage_range(sm, 6 * p, 6 * p + 6, sx, [&](int, int slot) { return pos[slot]; });
@@ -549,18 +549,16 @@ __global__ void __launch_bounds__(kTile) k_build_tiles(
             if (o == self) continue;
             const int e = 3 * (o - pb);
             const double r2 = list_r2(sx[e], sy[e], sz[e], qi.x, qi.y, qi.z);
-            if (r2 <= g.SL2) {
-              if (cnt < kcap) out[(size_t)cnt * kTile] = plane_entry(o - pb);
-              cnt++;
-            }
+            if (r2 <= g.SL2) gw.put(plane_entry(o - pb));
           }
         }
       }
-      pad_plane<kTile>(out, cnt, kcap, sm.seg_base[6 * p + 6] - pb);
+      gw.pad_plane(sm.seg_base[6 * p + 6] - pb);
     }
-    if (p == 0) b1 = cnt;
-    if (p == 1) b2 = cnt;
+    if (p == 0) b1 = gw.cnt;
+    if (p == 1) b2 = gw.cnt;
   }
+  const int cnt = gw.cnt;
   if (act) {
     nn[i] = cnt < kcap ? cnt : kcap;
     nb[i] = min(b1, kcap) | (min(b2, kcap) << 16);
@@ -691,7 +689,7 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
     if (MODE != 1) pm = mom[i];
   }
   tile_load_segments(c, tile, sm);
-  const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
+  const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
   // kILP independent partner chains per lane: entry k of the list feeds
   // chain k % kILP, and the chains are summed in a fixed order at the end.
   // A sentinel entry adds exactly +0.0 (the accumulators are never -0.0).
@@ -844,7 +842,7 @@ __global__ void __launch_bounds__(kTile) k_lj_observe_tiles(
     const int i = t.i0 + c.tperm[(size_t)tile * kTile + lane];
     const double4 qi = pos[i];
     const double4 p = mom[i];
-    const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
+    const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
     double pe = 0.0, w = 0.0, dpe_s = 0.0, dw_s = 0.0;
     double gwx = 0.0, gwy = 0.0, gwz = 0.0, gex = 0.0, gey = 0.0, gez = 0.0;
     for_each_partner(lst, nn[i], nb[i], sm, [&](int o) {
@@ -944,7 +942,7 @@ __global__ void __launch_bounds__(kTile) k_adj_hvp_tiles(
     const int i = t.i0 + c.tperm[(size_t)tile * kTile + lane];
     const double4 qi = pos[i];
     const double4 ai = ext[i];
-    const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
+    const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
     double jx = 0.0, jy = 0.0, jz = 0.0, te = 0.0, ts = 0.0;
     for_each_partner(lst, nn[i], nb[i], sm, [&](int o) {
       const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
@@ -1019,7 +1017,7 @@ __global__ void __launch_bounds__(kTile) k_observe_tiles(
     const double4 p = mom[i];
     ke = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z;
     const double4 qi = pos[i];
-    const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
+    const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
     for_each_partner(lst, nn[i], nb[i], sm, [&](int o) {
       const double dx = qi.x - sx[o], dy = qi.y - sy[o], dz = qi.z - sz[o];
       const double r2 = dx * dx + dy * dy + dz * dz;
@@ -1131,7 +1129,7 @@ __global__ void __launch_bounds__(kTile) k_canonical_tiles(
   if (lane >= t.ni) return;
   const int i = t.i0 + c.tperm[(size_t)tile * kTile + lane];
   const int ti = tag[i];
-  const unsigned short *lst = list + (size_t)tile * kcap * kTile + lane;
+  const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
   int cnum = 0;
   int w = WRITE ? offset[i] : 0;
   for_each_partner(lst, nn[i], nb[i], sm, [&](int o3) {
@@ -1347,6 +1345,7 @@ int ensure_image_capacity(mdx_system *s, int gcap) {
 
 int ensure_list_capacity(mdx_system *s, int kcap) {
   // padded by one group of entries, which the kick reads past a list's end
+  kcap = (kcap + kILP - 1) / kILP * kILP;   // whole groups per column
   const size_t need = (size_t)std::max(s->ntiles, 1) * kcap * kTile + (size_t)kILP * kTile;
   if (need <= s->list16_elems && kcap <= s->kcap) return MDX_OK;
   int rc = dev_alloc(s, &s->list16, need);
@@ -2027,7 +2026,7 @@ int mdx_set_state(mdx_system *s, int n, const double *q3, const double *p3) {
   if ((rc = ensure_image_capacity(s, gguess))) return rc;
   const double rho = n / V;
   const int kguess = (int)(4.18879 * s->SL * s->SL * s->SL * rho * 1.35) + 16 + kPlanes * (kILP - 1);
-  s->kcap = std::max(kguess, s->kcap);
+  s->kcap = std::max((kguess + kILP - 1) / kILP * kILP, s->kcap);
   // positions staged in io_buf, momenta in vec_out (double4[cap] >= 3n doubles)
   double *pbuf = reinterpret_cast<double *>(s->vec_out);
   HIPCHK(s, hipMemcpyAsync(s->io_buf, q3, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s->stream));

@@ -164,49 +164,78 @@ __device__ __forceinline__ int plane_end(int p, int nb, int m) {
   return p == 0 ? (nb & 0xffff) : (p == 1 ? (nb >> 16) : m);
 }
 
-// Close plane `len` particles long in a list being built (column stride S):
-// sentinel entries up to the next multiple of kILP, stored while below kcap.
+// Grouped list layout: a tile's list entries sit in groups of kILP per
+// column, group g of column l at [(g S + l) kILP, (g S + l + 1) kILP) of
+// the tile's region (S columns), so one lane's group is one 16-byte load
+// and a wavefront's is 1 KiB in a row.  A column's base is the region plus
+// l kILP; its entry k sits at base[list_index<S>(k)].
+static_assert(kILP == 8, "a group of entries is one 16-byte load");
 template <int S>
-__device__ __forceinline__ void pad_plane(unsigned short *out, int &cnt, int kcap, int len) {
-  const unsigned short pad = plane_entry(len);
-  for (; cnt % kILP; cnt++)
-    if (cnt < kcap) out[(size_t)cnt * S] = pad;
+__host__ __device__ constexpr size_t list_index(int k) {
+  return (size_t)(k / kILP) * S * kILP + k % kILP;
 }
 
-// Entries [kb, ke) of one lane's list, both multiples of kILP, column
-// stride S, partners staged at s3: entry k goes to term(k % kILP, xj, yj,
-// zj) in list order.  The groups go in pairs, each group's entries loading
-// while the previous group computes, into the other of two register sets;
-// a pair is one straight-line block, so the compiler keeps every load a
-// group ahead of its use.  The last load of a plane may reload its last
-// group (never used).
+// Writes one column of a list under construction a group at a time: a
+// group's entries gather in registers (entry k of the group in bits 16k ..
+// 16k+15) and leave as one 16-byte store once the group is full.  Entries
+// past kcap (a multiple of kILP) are counted, not stored.
+template <int S>
+struct GroupWriter {
+  unsigned short *col;   // column base
+  int kcap, cnt = 0;
+  unsigned long long lo = 0, hi = 0;
+  __device__ GroupWriter(unsigned short *c, int kc) : col(c), kcap(kc) {}
+  __device__ __forceinline__ void put(unsigned e) {
+    lo = (lo >> 16) | (hi << 48);
+    hi = (hi >> 16) | ((unsigned long long)e << 48);
+    if (++cnt % kILP == 0 && cnt <= kcap)
+      *reinterpret_cast<uint4 *>(col + list_index<S>(cnt - kILP)) =
+          make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi, (unsigned)(hi >> 32));
+  }
+  // close a plane `len` particles long: its sentinel up to a whole group
+  __device__ __forceinline__ void pad_plane(int len) {
+    while (cnt % kILP) put(plane_entry(len));
+  }
+};
+
+// Entries [kb, ke) of one lane's list (column base lst), both multiples of
+// kILP, S columns, partners staged at s3: entry k goes to term(k % kILP, xj,
+// yj, zj) in list order.  The list streams from memory, so the loads run
+// kPrefetch groups ahead in a ring of registers: group j computes from
+// ring slot j % kPrefetch, which then loads group j + kPrefetch (clamped
+// to the last group: a reload that is never used).  The ring turns once
+// per loop pass, so no register copies; the groups past the last full
+// turn compute from the ring as it stands.
+constexpr int kPrefetch = 2;
 template <int S, class Term>
 __device__ __forceinline__ void plane_groups(const unsigned short *__restrict__ lst, int kb, int ke,
                                              const double *s3, Term term) {
   if (kb >= ke) return;
-  unsigned ga[kILP], gb[kILP];
-  auto load = [&](unsigned(&g)[kILP], int k) {
-    const unsigned short *lp = lst + (size_t)k * S;
-#pragma unroll
-    for (int u = 0; u < kILP; u++) g[u] = lp[u * S];
-  };
-  auto run = [&](const unsigned(&g)[kILP]) {
+  auto load = [&](int k) { return *reinterpret_cast<const uint4 *>(lst + list_index<S>(k)); };
+  auto run = [&](const uint4 &w) {
+    const unsigned h[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
     for (int u = 0; u < kILP; u++) {
-      const double *q = staged_at(s3, g[u]);
+      const double *q = staged_at(s3, (h[u / 2] >> (16 * (u % 2))) & 0xffffu);
       term(u, q[0], q[1], q[2]);
     }
   };
   const int last = ke - kILP, ng = (ke - kb) / kILP;
-  load(ga, kb);
+  uint4 ring[kPrefetch];
+#pragma unroll
+  for (int d = 0; d < kPrefetch; d++) ring[d] = load(min(kb + d * kILP, last));
   int k = kb;
-  for (int j = 0; j < ng / 2; j++, k += 2 * kILP) {
-    load(gb, k + kILP);
-    run(ga);
-    load(ga, min(k + 2 * kILP, last));
-    run(gb);
+  for (int j = 0; j < ng / kPrefetch; j++, k += kPrefetch * kILP) {
+#pragma unroll
+    for (int d = 0; d < kPrefetch; d++) {
+      run(ring[d]);
+      ring[d] = load(min(k + (kPrefetch + d) * kILP, last));
+    }
   }
-  if (ng & 1) run(ga);
+  const int rem = ng % kPrefetch;
+#pragma unroll
+  for (int d = 0; d < kPrefetch - 1; d++)
+    if (d < rem) run(ring[d]);
 }
 
 // Order of the interior cell rows (y, z) in the tile array: strips of h
