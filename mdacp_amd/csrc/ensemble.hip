@@ -631,9 +631,8 @@ __global__ void __launch_bounds__(T) k_e_build(EnsDev d) {
                 len = d.cnt_in[cl];
               }
               for (int o = lo; o < lo + len; o++) {
-                if (o == self) continue;
                 const int e = 3 * (o - pb);
-                if (list_r2(sx[e], sy[e], sz[e], qi.x, qi.y, qi.z) <= g.SL2)
+                if (list_r2(sx[e], sy[e], sz[e], qi.x, qi.y, qi.z) <= g.SL2 && o != self)
                   gw.put(plane_entry(o - pb));
               }
             }

@@ -442,11 +442,13 @@ __global__ void k_row_tiles(Geom g, Cells c, int *__restrict__ row_tiles) {
   row_tiles[r] = (cnt + kTile - 1) / kTile;
 }
 
+// Tile descriptors and segments: one wavefront per cell row, one lane per
+// tile of the row; block maxima of the staging sizes to partial.
 __global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict__ pos,
                               const int *__restrict__ row_tiles, const int *__restrict__ row_off,
                               TileDesc *__restrict__ tiles, int *__restrict__ tseg,
                               double *__restrict__ partial) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
   const int rows = g.ms[1] * g.ms[2];
   double smax = 0.0, pmax = 0.0;
   if (r < rows) {
@@ -454,7 +456,7 @@ __global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict_
     row_of(g, r, &y, &z);
     const int a = cell_linear(g, 1, y, z), b = cell_linear(g, g.ms[0], y, z);
     const int first = c.scan_in[a], cnt = c.scan_in[b] + c.cnt_in[b] - first;
-    for (int k = 0; k < row_tiles[r]; k++) {
+    for (int k = lane; k < row_tiles[r]; k += 64) {
       TileDesc t;
       t.i0 = first + k * kTile;
       t.ni = min(kTile, cnt - k * kTile);
@@ -546,10 +548,9 @@ __global__ void __launch_bounds__(kTile) k_build_tiles(
             len = c.cnt_in[cl];
           }
           for (int o = lo; o < lo + len; o++) {
-            if (o == self) continue;
             const int e = 3 * (o - pb);
             const double r2 = list_r2(sx[e], sy[e], sz[e], qi.x, qi.y, qi.z);
-            if (r2 <= g.SL2) gw.put(plane_entry(o - pb));
+            if (r2 <= g.SL2 && o != self) gw.put(plane_entry(o - pb));
           }
         }
       }
@@ -1463,11 +1464,12 @@ int build_pairlist(mdx_system *s) {
   }
   cells.tseg = s->tseg;
   cells.tperm = s->tperm;
-  if ((rc = ensure_partials(s, std::max(tile_grid(s->ntiles), 2 * nblk(rows))))) return rc;
-  k_write_tiles<<<nblk(rows), kBlock, 0, s->stream>>>(s->geom, cells, n, s->pos, s->row_tiles,
+  const int wt_blocks = (int)(((size_t)rows * 64 + kBlock - 1) / kBlock);
+  if ((rc = ensure_partials(s, std::max(tile_grid(s->ntiles), 2 * wt_blocks)))) return rc;
+  k_write_tiles<<<wt_blocks, kBlock, 0, s->stream>>>(s->geom, cells, n, s->pos, s->row_tiles,
                                                        s->row_off, s->tiles, s->tseg, s->pmax);
-  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblk(rows), &s->dstat->stage_max);
-  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax + nblk(rows), nblk(rows), &s->dstat->plane_max);
+  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, wt_blocks, &s->dstat->stage_max);
+  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax + wt_blocks, wt_blocks, &s->dstat->plane_max);
   if ((rc = sync_status(s))) return rc;
   s->stage_max = (int)s->hstat->stage_max;
   s->plane_max = (int)s->hstat->plane_max;

@@ -175,24 +175,20 @@ __host__ __device__ constexpr size_t list_index(int k) {
   return (size_t)(k / kILP) * S * kILP + k % kILP;
 }
 
-// Writes one column of a list under construction a group at a time: a
-// group's entries gather in registers (entry k of the group in bits 16k ..
-// 16k+15) and leave as one 16-byte store once the group is full.  Entries
-// past kcap (a multiple of kILP) are counted, not stored.
+// Writes one column of a list under construction in the grouped layout:
+// the place of the next entry advances by one within a group and on to the
+// column's next group after the last.  Entries past kcap (a multiple of
+// kILP) are counted, not stored.
 template <int S>
 struct GroupWriter {
-  unsigned short *col;   // column base
+  unsigned short *ptr;   // the next entry's place
   int kcap, cnt = 0;
-  unsigned long long lo = 0, hi = 0;
-  __device__ GroupWriter(unsigned short *c, int kc) : col(c), kcap(kc) {}
+  __device__ GroupWriter(unsigned short *c, int kc) : ptr(c), kcap(kc) {}
   __device__ __forceinline__ void put(unsigned e) {
-    lo = (lo >> 16) | (hi << 48);
-    hi = (hi >> 16) | ((unsigned long long)e << 48);
-    if (++cnt % kILP == 0 && cnt <= kcap)
-      *reinterpret_cast<uint4 *>(col + list_index<S>(cnt - kILP)) =
-          make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi, (unsigned)(hi >> 32));
+    if (cnt < kcap) *ptr = (unsigned short)e;
+    ++cnt;
+    ptr += (cnt % kILP) ? 1 : S * kILP - (kILP - 1);
   }
-  // close a plane `len` particles long: its sentinel up to a whole group
   __device__ __forceinline__ void pad_plane(int len) {
     while (cnt % kILP) put(plane_entry(len));
   }
