@@ -67,7 +67,8 @@ struct DevStatus {
 
 // Block maximum of a non-negative value written to partial[blockIdx.x];
 // every thread of the block must call it.  A one-block kernel folds the
-// partials (k_max_partials), so no address sees more than one writer.
+// partials (k_max_partials), so no address sees more than one writer.  It
+// ends with a barrier, so calls may follow one another.
 __device__ __forceinline__ void block_max_to(double v, double *__restrict__ partial) {
   __shared__ double lds[kBlock / 64];
   v = wave_max(v);
@@ -78,6 +79,7 @@ __device__ __forceinline__ void block_max_to(double v, double *__restrict__ part
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, lds[w]);
     partial[blockIdx.x] = m;
   }
+  __syncthreads();
 }
 
 __global__ void k_max_partials(const double *__restrict__ partial, int nb, double *__restrict__ out) {
