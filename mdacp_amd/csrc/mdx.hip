@@ -408,26 +408,54 @@ __device__ __forceinline__ void tile_load_segments(const Cells &c, int tile, Til
   __syncthreads();
 }
 
-// segments and staging; every thread of the block calls it
-__device__ void tile_stage(int tile, const Cells &c, const double4 *__restrict__ pos, TileSmem &sm,
-                           double *s3) {
-  tile_load_segments(c, tile, sm);
-  stage_segments(sm, s3, [&](int, int slot) { return pos[slot]; });
-}
-
 // Partners of one list column in list order, for kernels that stage the
-// whole tile: f(o) with o = 3 x the partner's index in that staging;
-// sentinel entries are skipped.
+// whole tile: f(o) with o = 3 x the partner's index in that staging (an
+// entry is 24 x the index in its plane, so o = 3 x the plane's first index
+// + entry / 8); sentinel entries are skipped.  One 16-byte load per group.
 template <class F>
 __device__ __forceinline__ void for_each_partner(const unsigned short *__restrict__ lst, int m,
                                                  int nb, const TileSmem &sm, F f) {
   for (int p = 0; p < kPlanes; p++) {
     const int pb = sm.seg_base[6 * p];
-    const int lim = kEntryBytes * (sm.seg_base[6 * p + 6] - pb);
+    const unsigned lim = (unsigned)(kEntryBytes * (sm.seg_base[6 * p + 6] - pb));
     const int ke = plane_end(p, nb, m);
-    for (int k = plane_begin(p, nb); k < ke; k++) {
-      const int e = lst[list_index<kTile>(k)];
-      if (e < lim) f(3 * (pb + e / kEntryBytes));
+    for (int k = plane_begin(p, nb); k < ke; k += kILP) {
+      const uint4 w = *reinterpret_cast<const uint4 *>(lst + list_index<kTile>(k));
+      const unsigned h[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int u = 0; u < kILP; u++) {
+        const unsigned e = (h[u / 2] >> (16 * (u % 2))) & 0xffffu;
+        if (e < lim) f(3 * pb + (int)(e >> 3));
+      }
+    }
+  }
+}
+
+// For kernels that walk the lists a plane at a time (the LDS of one plane):
+// every thread stages plane p and runs prep(p) (per-plane LDS of the
+// kernel's own, ending with a barrier if it writes any); then each active
+// lane calls f(o) for its list's partners in that plane, in list order,
+// with o = 3 x the partner's index in the plane's staging (= entry / 8);
+// sentinel entries are skipped.  One 16-byte list load per group.
+template <class Load, class Prep, class F>
+__device__ __forceinline__ void walk_planes(const TileSmem &sm, double *s3, Load load, Prep prep,
+                                            bool act, const unsigned short *__restrict__ lst, int m,
+                                            int nb, F f) {
+  for (int p = 0; p < kPlanes; p++) {
+    if (p > 0) __syncthreads();   // the previous plane is read
+    stage_range(sm, 6 * p, 6 * p + 6, s3, load);
+    prep(p);
+    if (!act) continue;
+    const unsigned lim = (unsigned)(kEntryBytes * (sm.seg_base[6 * p + 6] - sm.seg_base[6 * p]));
+    const int ke = plane_end(p, nb, m);
+    for (int k = plane_begin(p, nb); k < ke; k += kILP) {
+      const uint4 w = *reinterpret_cast<const uint4 *>(lst + list_index<kTile>(k));
+      const unsigned h[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int u = 0; u < kILP; u++) {
+        const unsigned e = (h[u / 2] >> (16 * (u % 2))) & 0xffffu;
+        if (e < lim) f((int)(e >> 3));
+      }
     }
   }
 }
@@ -837,18 +865,20 @@ __global__ void __launch_bounds__(kTile) k_lj_observe_tiles(
   }
   const TileDesc t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-  tile_stage(tile, c, pos, sm, sx);
+  tile_load_segments(c, tile, sm);
   const int lane = threadIdx.x;
+  const bool act = lane < t.ni;
+  const int i = t.i0 + (act ? c.tperm[(size_t)tile * kTile + lane] : 0);
+  const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
+  const double4 qi = act ? pos[i] : z4;
+  const double4 p = act ? mom[i] : z4;
+  const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
   double acc[K];
   for (int k = 0; k < K; k++) acc[k] = 0.0;
-  if (lane < t.ni) {
-    const int i = t.i0 + c.tperm[(size_t)tile * kTile + lane];
-    const double4 qi = pos[i];
-    const double4 p = mom[i];
-    const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
-    double pe = 0.0, w = 0.0, dpe_s = 0.0, dw_s = 0.0;
-    double gwx = 0.0, gwy = 0.0, gwz = 0.0, gex = 0.0, gey = 0.0, gez = 0.0;
-    for_each_partner(lst, nn[i], nb[i], sm, [&](int o) {
+  double pe = 0.0, w = 0.0, dpe_s = 0.0, dw_s = 0.0;
+  double gwx = 0.0, gwy = 0.0, gwz = 0.0, gex = 0.0, gey = 0.0, gez = 0.0;
+  walk_planes(sm, sx, [&](int, int slot) { return pos[slot]; }, [](int) {}, act, lst,
+              act ? nn[i] : 0, act ? nb[i] : 0, [&](int o) {
       const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
       const double s2 = dx * dx + dy * dy + dz * dz;
       double phi, dphi, V;
@@ -867,7 +897,8 @@ __global__ void __launch_bounds__(kTile) k_lj_observe_tiles(
         gey += phi * dy;
         gez += phi * dz;
       }
-    });
+  });
+  if (act) {
     if (!GRAD) {
       acc[0] = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z;
       acc[1] = 0.5 * pe;
@@ -932,22 +963,32 @@ __global__ void __launch_bounds__(kTile) k_adj_hvp_tiles(
   }
   const TileDesc t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-  int *sidx = reinterpret_cast<int *>(stage_mem + 3 * t.stage);   // slot of staged particle e / 3
-  tile_stage(tile, c, pos, sm, sx);
-  for (int k = 0; k < kSeg; k++) {
-    const int b = sm.seg_base[k], len = sm.seg_base[k + 1] - b, st = sm.seg_start[k];
-    for (int u = threadIdx.x; u < len; u += blockDim.x) sidx[b + u] = st + u;
-  }
-  __syncthreads();
+  tile_load_segments(c, tile, sm);
+  // slot of the plane's staged particle o / 3, after the largest plane of
+  // the tile and its sentinel
+  int lmax = 0;
+  for (int p = 0; p < kPlanes; p++) lmax = max(lmax, sm.seg_base[6 * p + 6] - sm.seg_base[6 * p]);
+  int *sidx = reinterpret_cast<int *>(stage_mem + 3 * (lmax + 1));
   const int lane = threadIdx.x;
+  const bool act = lane < t.ni;
+  const int i = t.i0 + (act ? c.tperm[(size_t)tile * kTile + lane] : 0);
+  const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
+  const double4 qi = act ? pos[i] : z4;
+  const double4 ai = act ? ext[i] : z4;
+  const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
   double acc[2] = {0.0, 0.0};
-  if (lane < t.ni) {
-    const int i = t.i0 + c.tperm[(size_t)tile * kTile + lane];
-    const double4 qi = pos[i];
-    const double4 ai = ext[i];
-    const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
-    double jx = 0.0, jy = 0.0, jz = 0.0, te = 0.0, ts = 0.0;
-    for_each_partner(lst, nn[i], nb[i], sm, [&](int o) {
+  double jx = 0.0, jy = 0.0, jz = 0.0, te = 0.0, ts = 0.0;
+  auto slots = [&](int p) {
+    const int p0 = sm.seg_base[6 * p];
+    for (int k = 6 * p; k < 6 * p + 6; k++) {
+      const int b = sm.seg_base[k] - p0, len = sm.seg_base[k + 1] - sm.seg_base[k];
+      const int st = sm.seg_start[k];
+      for (int u = threadIdx.x; u < len; u += blockDim.x) sidx[b + u] = st + u;
+    }
+    __syncthreads();
+  };
+  walk_planes(sm, sx, [&](int, int slot) { return pos[slot]; }, slots, act, lst,
+              act ? nn[i] : 0, act ? nb[i] : 0, [&](int o) {
       const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
       const double s2 = dx * dx + dy * dy + dz * dz;
       double phi, dphi, V;
@@ -961,7 +1002,8 @@ __global__ void __launch_bounds__(kTile) k_adj_hvp_tiles(
       const double ad = ai.x * dx + ai.y * dy + ai.z * dz;
       te += phi * ad;                         // eps dF/deps . a
       ts += (phi + s2 * dphi) * ad;           // -(sigma/2) dF/dsigma . a
-    });
+  });
+  if (act) {
     acc[0] = dt * te / lj.eps;
     acc[1] = dt * (-2.0 / lj.sigma) * ts;
     double4 a = aq[i];
@@ -1012,20 +1054,22 @@ __global__ void __launch_bounds__(kTile) k_observe_tiles(
   }
   const TileDesc t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-  tile_stage(tile, c, pos, sm, sx);
+  tile_load_segments(c, tile, sm);
   const int lane = threadIdx.x;
+  const bool act = lane < t.ni;
+  const int i = t.i0 + (act ? c.tperm[(size_t)tile * kTile + lane] : 0);
+  const double4 qi = act ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+  const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
   double ke = 0.0, pe = 0.0, vir = 0.0;
-  if (lane < t.ni) {
-    const int i = t.i0 + c.tperm[(size_t)tile * kTile + lane];
+  walk_planes(sm, sx, [&](int, int slot) { return pos[slot]; }, [](int) {}, act, lst,
+              act ? nn[i] : 0, act ? nb[i] : 0, [&](int o) {
+    const double dx = qi.x - sx[o], dy = qi.y - sy[o], dz = qi.z - sz[o];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 <= g.CL2) observe_term(r2, C0, C2, pe, vir);
+  });
+  if (act) {
     const double4 p = mom[i];
     ke = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z;
-    const double4 qi = pos[i];
-    const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
-    for_each_partner(lst, nn[i], nb[i], sm, [&](int o) {
-      const double dx = qi.x - sx[o], dy = qi.y - sy[o], dz = qi.z - sz[o];
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 <= g.CL2) observe_term(r2, C0, C2, pe, vir);
-    });
     pe *= 0.5;
     vir *= 0.5;
   }
@@ -1359,7 +1403,10 @@ int ensure_list_capacity(mdx_system *s, int kcap) {
 }
 
 // dynamic LDS of the tile kernels: the staged particles plus the sentinel
-size_t stage_bytes(const mdx_system *s) { return (size_t)(s->stage_max + 1) * 3 * sizeof(double); }
+// (the adjoint's HVP adds a slot index per staged particle)
+size_t adj_stage_bytes(const mdx_system *s) {
+  return (size_t)(s->plane_max + 1) * (3 * sizeof(double) + sizeof(int)) + 8;
+}
 size_t plane_bytes(const mdx_system *s) { return (size_t)(s->plane_max + 1) * 3 * sizeof(double); }
 
 Cells cells_of(const mdx_system *s) {
@@ -1475,9 +1522,9 @@ int build_pairlist(mdx_system *s) {
   if ((rc = sync_status(s))) return rc;
   s->stage_max = (int)s->hstat->stage_max;
   s->plane_max = (int)s->hstat->plane_max;
-  if (stage_bytes(s) + 4096 > s->lds_limit)
-    return fail(s, MDX_ERR_CAPACITY, "tile staging of %d particles exceeds LDS (%zu B)",
-                s->stage_max, s->lds_limit);
+  if (adj_stage_bytes(s) + 4096 > s->lds_limit)
+    return fail(s, MDX_ERR_CAPACITY, "tile plane staging of %d particles exceeds LDS (%zu B)",
+                s->plane_max, s->lds_limit);
   if (s->plane_max > kMaxPlane)
     return fail(s, MDX_ERR_CAPACITY,
                 "a tile plane of %d particles exceeds the 16-bit list offsets (%d particles)",
@@ -1652,7 +1699,7 @@ int expired(mdx_system *s, int *out) {
 int reduce_observe(mdx_system *s, double *ke, double *pe, double *vir) {
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "observe before a pairlist exists");
   const int nb = tile_grid(s->ntiles);
-  k_observe_tiles<<<nb, kTile, stage_bytes(s), s->stream>>>(s->pos, s->mom, s->tiles, s->ntiles,
+  k_observe_tiles<<<nb, kTile, plane_bytes(s), s->stream>>>(s->pos, s->mom, s->tiles, s->ntiles,
                                                             s->geom, cells_of(s), s->n, s->kcap,
                                                             s->list16, s->nn, s->nb, s->C0,
                                                             s->C2, s->partial);
@@ -1700,9 +1747,6 @@ LJ make_lj(const mdx_system *s, double eps, double sigma) {
   return p;
 }
 
-size_t adj_stage_bytes(const mdx_system *s) {
-  return (size_t)s->stage_max * (3 * sizeof(double) + sizeof(int)) + 8;
-}
 
 // one step of the differentiable form: drift + image refresh into mid,
 // kick + drift back into pos
@@ -1768,7 +1812,7 @@ int lj_trajectory(mdx_system *s, double eps, double sigma, int nsteps, int nobs,
       if (k >= nsteps - nobs) {
         if ((rc = refresh(s))) return rc;
         const int nb = tile_grid(s->ntiles);
-        k_lj_observe_tiles<false><<<nb, kTile, stage_bytes(s), s->stream>>>(
+        k_lj_observe_tiles<false><<<nb, kTile, plane_bytes(s), s->stream>>>(
             s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16,
             s->nn, s->nb, lj, 0.0, 0.0, 0.0, nullptr, nullptr, s->partial);
         k_sum_partials_k<3><<<1, kBlock, 0, s->stream>>>(s->partial, nb, obs + 3 * (size_t)k, 1.0);
@@ -1842,7 +1886,7 @@ int lj_trajectory(mdx_system *s, double eps, double sigma, int nsteps, int nobs,
       for (int k = k1 - 1; k >= k0; k--) {
         if (k >= nsteps - nobs) {
           // sensitivity of the observation after step k
-          k_lj_observe_tiles<true><<<nb, kTile, stage_bytes(s), s->stream>>>(
+          k_lj_observe_tiles<true><<<nb, kTile, plane_bytes(s), s->stream>>>(
               st_end + (size_t)(k - k0) * totcap, st_mom + (size_t)(k - k0) * n, s->tiles,
               s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, lj, cw, ce,
               ck, aq, ap, s->partial);
