@@ -42,6 +42,9 @@ struct orc_sys {
   double (*qb_old)[4];
   double *disp;
   int *cand;
+  /* heat bath (simulationinfo.cc:11-23, variables.h:29) */
+  orc_heatbath hb;
+  uint64_t step;         /* steps since orc_set_state (Langevin counter) */
 };
 
 /* variables.cc:21-27 */
@@ -125,6 +128,7 @@ int orc_set_state(orc_sys *s, int n, const double *q3, const double *p3, const i
     s->tag[i] = tag ? tag[i] : i;
   }
   s->pn = s->tn = n;
+  s->step = 0;
   s->initialized = 0; /* PairList ctor (pairlist.h:25) */
   s->npairs = 0;
   return ORC_OK;
@@ -551,8 +555,12 @@ int orc_forces(orc_sys *s, double *f3) {
   return ORC_OK;
 }
 
-/* MDManager::Calculate (mdmanager.cc:154-186), ControlTemperature = no;
- * CalculateForce includes the trailing half drift (mdmanager.cc:227-252) */
+/* MDManager::Calculate (mdmanager.cc:154-186).  NVE: CalculateForce with
+ * its trailing half drift (mdmanager.cc:227-252).  Nose-Hoover
+ * (CalculateNoseHoover, :255-287): zeta from the temperature, momenta
+ * scaled by exp(-zeta dt/2) around the kick, zeta again, half drift.
+ * Langevin (CalculateLangevin, :289-315): kick, then the noise step, and no
+ * trailing half drift. */
 int orc_calculate(orc_sys *s, int nsteps, int *rebuilds) {
   int nreb = 0;
   for (int step = 0; step < nsteps; step++) {
@@ -566,11 +574,104 @@ int orc_calculate(orc_sys *s, int nsteps, int *rebuilds) {
     }
     orc_update_position_half(s);
     orc_refresh_images(s);
-    orc_calculate_force(s);
-    orc_update_position_half(s);
+    if (s->hb.type == 1) {
+      orc_heatbath_zeta(s, orc_temperature(s));
+      orc_heatbath_momenta(s);
+      orc_calculate_force(s);
+      orc_heatbath_momenta(s);
+      orc_heatbath_zeta(s, orc_temperature(s));
+      orc_update_position_half(s);
+    } else if (s->hb.type == 2) {
+      orc_calculate_force(s);
+      orc_langevin(s);
+    } else {
+      orc_calculate_force(s);
+      orc_update_position_half(s);
+    }
+    s->step++;
   }
   if (rebuilds) *rebuilds = nreb;
   return ORC_OK;
+}
+
+void orc_set_heatbath(orc_sys *s, const orc_heatbath *hb) { s->hb = *hb; }
+void orc_get_heatbath(const orc_sys *s, orc_heatbath *hb) { *hb = s->hb; }
+
+/* ForceCalculator::HeatbathZeta (fcalculator.cc:979-985) */
+void orc_heatbath_zeta(orc_sys *s, double current_temperature) {
+  const double dt2 = s->prm.dt * 0.5;
+  const double tau = s->hb.tau;
+  const double t1 = (current_temperature - s->hb.aimed_temperature) / (tau * tau);
+  s->hb.zeta += t1 * dt2;
+}
+
+/* ForceCalculator::HeatbathMomenta (fcalculator.cc:987-999), all four lanes */
+void orc_heatbath_momenta(orc_sys *s) {
+  const double dt2 = s->prm.dt * 0.5;
+  const double exp1 = exp(-dt2 * s->hb.zeta);
+  for (int i = 0; i < s->pn; i++)
+    for (int d = 0; d < 4; d++) s->p[i][d] *= exp1;
+}
+
+/* ForceCalculator::Langevin (fcalculator.cc:1001-1017): all four lanes, the
+ * padding lane included, with r ~ N(0, sqrt(2 gamma T / dt)) */
+void orc_langevin(orc_sys *s) {
+  const double dt = s->prm.dt;
+  const double hb_gamma = s->hb.gamma;
+  const double T = s->hb.aimed_temperature;
+  const double hb_D = sqrt(2.0 * hb_gamma * T / dt);
+  for (int i = 0; i < s->pn; i++) {
+    double nrm[4];
+    orc_langevin_normals(s->hb.seed, (uint32_t)s->tag[i], s->step, nrm);
+    for (int d = 0; d < 4; d++) {
+      const double r = hb_D * nrm[d];
+      s->p[i][d] += (-hb_gamma * s->p[i][d] + r) * dt;
+    }
+  }
+}
+
+/* MDManager::Temperature (mdmanager.cc:531-533) over KineticEnergy
+ * (mdmanager.cc:543-548) */
+double orc_temperature(orc_sys *s) {
+  return orc_kinetic_energy(s) / (double)s->pn / 1.5;
+}
+
+/* Philox4x32-10: ten rounds of two 32x32->64 multiplies and xors, the key
+ * bumped by the Weyl constants between rounds (Salmon et al., SC'11) */
+void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; r++) {
+    const uint64_t a = (uint64_t)0xD2511F53u * c0, b = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(b >> 32) ^ c1 ^ k0, n1 = (uint32_t)b;
+    const uint32_t n2 = (uint32_t)(a >> 32) ^ c3 ^ k1, n3 = (uint32_t)a;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static double uniform53(uint32_t a, uint32_t b) {   /* (0, 1] */
+  const uint64_t m = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
+  return ((double)m + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+void orc_langevin_normals(uint64_t seed, uint32_t tag, uint64_t step, double out[4]) {
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t w[8];
+  for (int h = 0; h < 2; h++) {
+    const uint32_t ctr[4] = {tag, (uint32_t)step, (uint32_t)(step >> 32), (uint32_t)h};
+    orc_philox4x32(ctr, key, w + 4 * h);
+  }
+  const double two_pi = 6.283185307179586;
+  for (int k = 0; k < 2; k++) {
+    const double u1 = uniform53(w[4 * k], w[4 * k + 1]);
+    const double u2 = uniform53(w[4 * k + 2], w[4 * k + 3]);
+    const double rr = sqrt(-2.0 * log(u1));
+    out[2 * k] = rr * cos(two_pi * u2);
+    out[2 * k + 1] = rr * sin(two_pi * u2);
+  }
 }
 
 /* KineticEnergyObserver::Observe (observer.cc:5-15) */

@@ -18,7 +18,12 @@
  *   - drift, step order and pairlist expiry: against the reference's
  *     1000-step trajectory fixture (observables and final state);
  *   - image construction: every pair found through an image is re-found by
- *     the minimum-image brute force.
+ *     the minimum-image brute force;
+ *   - Nose-Hoover heat bath: against the reference's thermostatted trajectory
+ *     fixture, and zeta against its closed-form update on a frozen system;
+ *   - Langevin heat bath: its noise (Philox4x32-10 + Box-Muller) against the
+ *     published Philox known-answer vectors and normal statistics, its
+ *     momentum update against the closed form p(1 - gamma dt) + r dt.
  */
 #ifndef MDORACLE_H
 #define MDORACLE_H
@@ -38,6 +43,22 @@ typedef struct {
 } orc_params;
 
 typedef struct orc_sys orc_sys;
+
+/* Heat bath (SimulationInfo::ControlTemperature, HeatbathType,
+ * AimedTemperature, HeatbathTau, HeatbathGamma, simulationinfo.cc:11-23;
+ * Variables::Zeta, variables.h:29).  type 0 = no control (NVE), 1 =
+ * Nose-Hoover, 2 = Langevin.  The reference draws Langevin noise from a
+ * per-thread std::mt19937 (fcalculator.cc:1002-1017), which fixes no
+ * particle-to-number map; here the noise of particle `tag` at step k (steps
+ * counted from orc_set_state) is orc_langevin_normals(seed, tag, k). */
+typedef struct {
+  int type;
+  double aimed_temperature;
+  double tau;
+  double gamma;
+  double zeta;
+  uint64_t seed;
+} orc_heatbath;
 
 /* status codes: 0 ok, negative = fatal condition the reference exit()s on */
 enum {
@@ -70,8 +91,29 @@ void orc_update_position_half(orc_sys *s);
 void orc_refresh_images(orc_sys *s);
 /* ForceCalculator::CalculateForce, half list with reaction (fcalculator.cc:182-223) */
 void orc_calculate_force(orc_sys *s);
-/* MDManager::Calculate (mdmanager.cc:154-186), NVE branch */
+/* MDManager::Calculate (mdmanager.cc:154-186): NVE, or the heat bath set
+ * by orc_set_heatbath (CalculateNoseHoover / CalculateLangevin,
+ * mdmanager.cc:255-315) */
 int orc_calculate(orc_sys *s, int nsteps, int *rebuilds);
+
+void orc_set_heatbath(orc_sys *s, const orc_heatbath *hb);
+void orc_get_heatbath(const orc_sys *s, orc_heatbath *hb);
+/* ForceCalculator::HeatbathZeta / HeatbathMomenta / Langevin
+ * (fcalculator.cc:979-1017) */
+void orc_heatbath_zeta(orc_sys *s, double current_temperature);
+void orc_heatbath_momenta(orc_sys *s);
+void orc_langevin(orc_sys *s);
+/* MDManager::Temperature (mdmanager.cc:531-533): KE per particle / 1.5 */
+double orc_temperature(orc_sys *s);
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+ * easy as 1, 2, 3", SC'11): the 128-bit block of counter ctr under key. */
+void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* four standard normals: two Philox blocks with counter (tag, step lo,
+ * step hi, 0 / 1) and key (seed lo, seed hi), eight words -> four 53-bit
+ * uniforms u in (0, 1] -> two Box-Muller pairs
+ * sqrt(-2 ln u1) (cos 2 pi u2, sin 2 pi u2) */
+void orc_langevin_normals(uint64_t seed, uint32_t tag, uint64_t step, double out[4]);
 
 /* observers (observer.cc:5-79); sums, not per-particle */
 double orc_kinetic_energy(orc_sys *s);

@@ -22,6 +22,15 @@ class OrcParams(ctypes.Structure):
                 ("check_list_length", ctypes.c_int)]
 
 
+class OrcHeatbath(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("aimed_temperature", ctypes.c_double),
+                ("tau", ctypes.c_double), ("gamma", ctypes.c_double), ("zeta", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
+
+
+HEATBATH = {"none": 0, "nose_hoover": 1, "langevin": 2}
+
+
 def build(force: bool = False) -> None:
     if force or not os.path.exists(LIB_PATH):
         subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
@@ -64,6 +73,22 @@ def lib():
         L.orc_bruteforce_forces.restype = None
         L.orc_constants.argtypes = [ctypes.c_double, dp, dp]
         L.orc_constants.restype = None
+        L.orc_set_heatbath.argtypes = [ctypes.c_void_p, P(OrcHeatbath)]
+        L.orc_set_heatbath.restype = None
+        L.orc_get_heatbath.argtypes = [ctypes.c_void_p, P(OrcHeatbath)]
+        L.orc_get_heatbath.restype = None
+        L.orc_heatbath_zeta.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_heatbath_zeta.restype = None
+        for fn in ("orc_heatbath_momenta", "orc_langevin"):
+            getattr(L, fn).argtypes = [ctypes.c_void_p]
+            getattr(L, fn).restype = None
+        L.orc_temperature.argtypes = [ctypes.c_void_p]
+        L.orc_temperature.restype = ctypes.c_double
+        U4, U2 = ctypes.c_uint32 * 4, ctypes.c_uint32 * 2
+        L.orc_philox4x32.argtypes = [U4, U2, U4]
+        L.orc_philox4x32.restype = None
+        L.orc_langevin_normals.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, dp]
+        L.orc_langevin_normals.restype = None
         _lib = L
     return _lib
 
@@ -162,6 +187,41 @@ class Oracle:
 
     def total_particle_number(self) -> int:
         return int(self.L.orc_total_particle_number(self.h))
+
+    def set_heatbath(self, kind: str, aimed_temperature: float = 1.0, tau: float = 0.1,
+                     gamma: float = 0.1, zeta: float = 0.0, seed: int = 0):
+        hb = OrcHeatbath(HEATBATH[kind], aimed_temperature, tau, gamma, zeta, seed)
+        self.L.orc_set_heatbath(self.h, ctypes.byref(hb))
+
+    def heatbath(self) -> dict:
+        hb = OrcHeatbath()
+        self.L.orc_get_heatbath(self.h, ctypes.byref(hb))
+        return {f: getattr(hb, f) for f, _ in OrcHeatbath._fields_}
+
+    def heatbath_zeta(self, current_temperature: float):
+        self.L.orc_heatbath_zeta(self.h, current_temperature)
+
+    def heatbath_momenta(self):
+        self.L.orc_heatbath_momenta(self.h)
+
+    def langevin(self):
+        self.L.orc_langevin(self.h)
+
+    def temperature(self) -> float:
+        return self.L.orc_temperature(self.h)
+
+
+def philox4x32(ctr, key):
+    U4, U2 = ctypes.c_uint32 * 4, ctypes.c_uint32 * 2
+    out = U4()
+    lib().orc_philox4x32(U4(*ctr), U2(*key), out)
+    return list(out)
+
+
+def langevin_normals(seed: int, tag: int, step: int) -> np.ndarray:
+    out = np.zeros(4)
+    lib().orc_langevin_normals(seed, tag, step, _dp(out))
+    return out
 
 
 def bruteforce_pairs(state, r2max: float) -> np.ndarray:

@@ -17,7 +17,10 @@
 //   pairs      one domain: canonical pair set, forces, PE, virial, KE
 //   forces     any number of OpenMP domains: forces, PE, virial, KE
 //   run K M    K steps of MDManager::Calculate, observables every M steps,
-//              final state
+//              final state; with MDX_REF_HEATBATH=NoseHoover:T:tau in the
+//              environment the steps run under the reference's Nose-Hoover
+//              heat bath (ControlTemperature, simulationinfo.cc:11-23) and
+//              Variables::Zeta is recorded with the observables
 //   bench K W  W warm-up steps then K timed steps; prints atom-steps/s
 #include <mpi.h>
 #include <omp.h>
@@ -125,6 +128,12 @@ class ForceCollector : public DoubleObserver {
   }
   std::vector<double> f;
   SimulationInfo *sinfo;
+};
+
+// Variables::Zeta summed over the units (one unit: its zeta)
+class ZetaReader : public DoubleObserver {
+ public:
+  double Observe(Variables *vars, MeshList *) override { return vars->Zeta; }
 };
 
 class ConstructionCounter : public DoubleObserver {
@@ -236,7 +245,14 @@ std::string WriteConfig(const State &s) {
   fprintf(f, "Mode=Harness\n");
   fprintf(f, "SystemSizeX=%.17g\nSystemSizeY=%.17g\nSystemSizeZ=%.17g\n", s.L[0], s.L[1], s.L[2]);
   fprintf(f, "TimeStep=%.17g\n", s.dt);
-  fprintf(f, "ControlTemperature=no\n");
+  const char *hb = getenv("MDX_REF_HEATBATH");
+  double T = 1.0, tau = 0.1;
+  if (hb && sscanf(hb, "NoseHoover:%lf:%lf", &T, &tau) == 2) {
+    fprintf(f, "ControlTemperature=yes\nHeatbathType=NoseHoover\n");
+    fprintf(f, "AimedTemperature=%.17g\nHeatbathTau=%.17g\n", T, tau);
+  } else {
+    fprintf(f, "ControlTemperature=no\n");
+  }
   fclose(f);
   return path;
 }
@@ -312,7 +328,9 @@ int main(int argc, char **argv) {
     const int K = argc > 4 ? atoi(argv[4]) : 1000;
     const int M = argc > 5 ? atoi(argv[5]) : 100;
     ConstructionCounter cc;
+    ZetaReader zr;
     std::vector<double> series;  // step, KE, PE, virial, constructions
+    std::vector<double> zeta;
     for (int i = 0; i < K; i++) {
       mdm->Calculate();
       if ((i + 1) % M == 0) {
@@ -323,11 +341,13 @@ int main(int argc, char **argv) {
         series.push_back(mdm->PotentialEnergy() * N);
         series.push_back(mdm->ConfigurationTemperature() * N);
         series.push_back(mdm->ObserveDouble(&cc));
+        zeta.push_back(mdm->ObserveDouble(&zr));
       }
     }
     StateCollector sc(st.n);
     mdm->ObserveDouble(&sc);
     WriteArray(out + "_series.f64", series);
+    WriteArray(out + "_zeta.f64", zeta);
     WriteArray(out + "_q.f64", sc.q);
     WriteArray(out + "_p.f64", sc.p);
     printf("steps %d\n", K);

@@ -10,7 +10,10 @@ states from mdacp_amd.workloads and stores its outputs as .npz fixtures:
               partner counts), forces at dt=1, and (KE, PE, virial) sums;
   traj_<case>.npz  the reference's 1000-step trajectory: observables every
               100 steps (step, KE, PE, virial, pairlist constructions) and the
-              final q, p.
+              final q, p;
+  traj_nh_fcc4000.npz  the same under the reference's Nose-Hoover heat bath
+              (AimedTemperature 1.5, HeatbathTau 0.1), with Variables::Zeta
+              at every observation.
 
 usage: python tests/golden/make_golden.py
 """
@@ -50,10 +53,13 @@ def pair_digest(triples: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(triples, dtype="<i4").tobytes()).hexdigest()
 
 
-def run(mode, state, tmp, *args):
+def run(mode, state, tmp, *args, heatbath=None):
     path = os.path.join(tmp, "state.bin")
     W.write_state(path, state)
     env = dict(os.environ, OMP_NUM_THREADS="1")
+    env.pop("MDX_REF_HEATBATH", None)
+    if heatbath:
+        env["MDX_REF_HEATBATH"] = heatbath
     out = os.path.join(tmp, "out")
     r = subprocess.run([HARNESS, mode, path, out, *map(str, args)], env=env,
                        capture_output=True, text=True)
@@ -87,6 +93,16 @@ def main():
                             q=np.fromfile(out + "_q.f64").reshape(-1, 3),
                             p=np.fromfile(out + "_p.f64").reshape(-1, 3))
         print("traj_fcc4000", series[-1])
+        T_aim, tau = 1.5, 0.1
+        out = run("run", s, tmp, 1000, 100, heatbath=f"NoseHoover:{T_aim!r}:{tau!r}")
+        series = np.fromfile(out + "_series.f64").reshape(-1, 5)
+        np.savez_compressed(os.path.join(HERE, "traj_nh_fcc4000.npz"), q0=s.q, p0=s.p, L=s.L,
+                            dt=np.float64(s.dt), aimed_temperature=np.float64(T_aim),
+                            tau=np.float64(tau), series=series,
+                            zeta=np.fromfile(out + "_zeta.f64"),
+                            q=np.fromfile(out + "_q.f64").reshape(-1, 3),
+                            p=np.fromfile(out + "_p.f64").reshape(-1, 3))
+        print("traj_nh_fcc4000", series[-1])
 
 
 if __name__ == "__main__":

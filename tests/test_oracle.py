@@ -154,3 +154,96 @@ def test_oracle_error_paths():
     o.make_pairlist()
     with pytest.raises(O.OracleError):
         o.calculate(1)
+
+
+# ------------------------------------------------------------ heat baths
+def test_oracle_nose_hoover_matches_reference():
+    """1000 steps under the reference's Nose-Hoover heat bath (target 1.5,
+    tau 0.1, N=4000): the same rebuild schedule, observables and zeta at
+    every observation, and the final state."""
+    fx = load_golden("traj_nh_fcc4000")
+    s = golden_state(fx)
+    o = O.Oracle(s)
+    o.set_heatbath("nose_hoover", float(fx["aimed_temperature"]), float(fx["tau"]))
+    rows, zeta, nreb = [], [], 0
+    for k in range(10):
+        nreb += o.calculate(100)
+        o.refresh_images()
+        ke, pe, vir = o.observe()
+        rows.append((100 * (k + 1), ke, pe, vir, nreb))
+        zeta.append(o.heatbath()["zeta"])
+    rows = np.array(rows)
+    ref = fx["series"]
+    np.testing.assert_array_equal(rows[:, 4], ref[:, 4])
+    np.testing.assert_allclose(rows[:, 1:4], ref[:, 1:4], rtol=1e-10)
+    np.testing.assert_allclose(zeta, fx["zeta"], rtol=0, atol=1e-10)
+    q, p = o.state()
+    d = q - fx["q"]
+    d -= s.L * np.round(d / s.L)
+    assert np.abs(d).max() < 1e-9
+    assert np.abs(p - fx["p"]).max() < 1e-8
+
+
+def test_heatbath_zeta_and_momenta_closed_form():
+    """HeatbathZeta adds (T - T0) / tau^2 * dt/2 (fcalculator.cc:979-985);
+    HeatbathMomenta scales every lane by exp(-zeta dt/2)."""
+    s = W.perturbed(W.plain_small(), 0.1, 61)
+    o = O.Oracle(s)
+    o.set_heatbath("nose_hoover", 1.3, 0.2, zeta=0.25)
+    T = o.temperature()
+    assert T == pytest.approx(np.sum(0.5 * s.p ** 2) / s.n / 1.5, rel=1e-13)
+    o.heatbath_zeta(T)
+    z = o.heatbath()["zeta"]
+    assert z == pytest.approx(0.25 + (T - 1.3) / 0.2 ** 2 * (s.dt * 0.5), rel=1e-15)
+    o.heatbath_momenta()
+    _, p = o.state()
+    np.testing.assert_allclose(p, s.p * np.exp(-(s.dt * 0.5) * z), rtol=1e-15)
+
+
+def test_philox_known_answers():
+    """The Random123 known-answer vectors of philox4x32-10."""
+    kat = [([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+           ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+           ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+            [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1])]
+    for ctr, key, want in kat:
+        assert O.philox4x32(ctr, key) == want
+
+
+def test_langevin_normals_are_standard_normal():
+    z = np.array([O.langevin_normals(9, t, k) for t in range(4000) for k in (0, 7, 123456789)])
+    assert z.shape == (12000, 4)
+    flat = z.ravel()
+    n = flat.size
+    assert abs(flat.mean()) < 4 / np.sqrt(n)
+    assert abs(flat.var() - 1.0) < 4 * np.sqrt(2.0 / n)
+    assert abs(np.mean(flat ** 4) - 3.0) < 0.1
+    # lanes, tags and steps uncorrelated; another seed, another stream
+    c = np.corrcoef(z.T)
+    assert np.abs(c - np.eye(4)).max() < 4 / np.sqrt(z.shape[0])
+    assert not np.array_equal(O.langevin_normals(9, 5, 3), O.langevin_normals(10, 5, 3))
+    assert not np.array_equal(O.langevin_normals(9, 5, 3), O.langevin_normals(9, 5, 4))
+
+
+def test_langevin_stationary_variance_and_no_second_drift():
+    """Without forces (a lattice too dilute for any pair within rc) the
+    Langevin step p += (-gamma p + r) dt, r ~ N(0, 2 gamma T / dt), is an
+    Ornstein-Uhlenbeck update whose stationary variance per lane is
+    2T / (2 - gamma dt); the step has no trailing half drift
+    (mdmanager.cc:289-315), so one step moves q by p_old dt / 2."""
+    s = W.fcc_state(14, 0.01, 1.0, 62)      # 10976 particles, nearest neighbour 5.2 > 3.3
+    o = O.Oracle(s)
+    gamma, T = 50.0, 1.2
+    o.set_heatbath("langevin", T, gamma=gamma, seed=123)
+    o.calculate(1)
+    q1, p1 = o.state()
+    d = q1 - s.q
+    d -= s.L * np.round(d / s.L)
+    np.testing.assert_allclose(d, s.p * (s.dt * 0.5), atol=1e-15)
+    o.calculate(300)                         # 15 relaxation times
+    _, p = o.state()
+    var = p.var()
+    want = 2 * T / (2 - gamma * s.dt)
+    assert abs(var - want) < 4 * want * np.sqrt(2.0 / p.size)
+    # the padding lane carries noise too: KE over four lanes exceeds the xyz sum
+    assert o.observe()[0] > 0.5 * np.sum(p ** 2) * 1.2
