@@ -88,13 +88,59 @@ int mdx_update_position_half(mdx_system *sys);
 int mdx_refresh_images(mdx_system *sys);
 /* ForceCalculator::CalculateForce (fcalculator.cc:33-50): p += F dt */
 int mdx_calculate_force(mdx_system *sys);
-/* MDManager::Calculate (mdmanager.cc:154-186), NVE, nsteps times;
- * *rebuilds (may be NULL) receives the number of pairlist constructions. */
+/* MDManager::Calculate (mdmanager.cc:154-186), nsteps times: NVE, or
+ * under the heat bath set by mdx_set_heatbath (CalculateNoseHoover /
+ * CalculateLangevin, mdmanager.cc:255-315).  *rebuilds (may be NULL)
+ * receives the number of pairlist constructions. */
 int mdx_calculate(mdx_system *sys, int nsteps, int *rebuilds);
 
+/* Heat bath: SimulationInfo::ControlTemperature / HeatbathType /
+ * AimedTemperature / HeatbathTau / HeatbathGamma (simulationinfo.cc:11-23)
+ * and Variables::Zeta (variables.h:29).
+ *   MDX_HEATBATH_NONE         NVE (ControlTemperature = no)
+ *   MDX_HEATBATH_NOSE_HOOVER  zeta += (T - T0) / tau^2 dt/2 from the
+ *                             temperature KE/N/1.5, p *= exp(-zeta dt/2),
+ *                             before and after the kick (mdmanager.cc:255-287)
+ *   MDX_HEATBATH_LANGEVIN     after the kick p += (-gamma p + r) dt with
+ *                             r ~ N(0, 2 gamma T0 / dt) on all four lanes
+ *                             (the padding lane too, which then enters KE),
+ *                             and no trailing half drift (mdmanager.cc:289-315)
+ * The reference draws Langevin noise from a per-thread std::mt19937
+ * (fcalculator.cc:1001-1017), which fixes no particle-to-number map; here
+ * the noise of particle `id` in the k-th Langevin kick since mdx_set_state
+ * (k from 0) is a pure function of (seed, id, k): Philox4x32-10 blocks with counter
+ * (id, k lo, k hi, 0 / 1) and key (seed lo, seed hi), eight words -> four
+ * 53-bit uniforms in (0, 1] -> two Box-Muller pairs.  The same stream
+ * whatever the binning or the GPU.  Zeta is kept on the device;
+ * mdx_get_heatbath reads it back. */
+enum {
+  MDX_HEATBATH_NONE = 0,
+  MDX_HEATBATH_NOSE_HOOVER = 1,
+  MDX_HEATBATH_LANGEVIN = 2
+};
+typedef struct mdx_heatbath {
+  int type;                  /* MDX_HEATBATH_* */
+  double aimed_temperature;  /* AimedTemperature, 1.0 */
+  double tau;                /* HeatbathTau, 0.1 (Nose-Hoover, > 0) */
+  double gamma;              /* HeatbathGamma, 0.1 (Langevin, >= 0) */
+  double zeta;               /* Variables::Zeta (Nose-Hoover state) */
+  uint64_t seed;             /* Langevin noise stream */
+} mdx_heatbath;
+int mdx_set_heatbath(mdx_system *sys, const mdx_heatbath *hb);
+int mdx_get_heatbath(mdx_system *sys, mdx_heatbath *hb);
+/* ForceCalculator::HeatbathZeta (fcalculator.cc:979-985): zeta += (t -
+ * T0) / tau^2 dt/2 for the given current temperature t */
+int mdx_heatbath_zeta(mdx_system *sys, double current_temperature);
+/* ForceCalculator::HeatbathMomenta (fcalculator.cc:987-999): p *= exp(-zeta dt/2) */
+int mdx_heatbath_momenta(mdx_system *sys);
+/* ForceCalculator::Langevin (fcalculator.cc:1001-1017): the next Langevin
+ * kick (its noise counter advances by one) */
+int mdx_langevin(mdx_system *sys);
+
 /* KineticEnergyObserver / PotentialEnergyObserver / VirialObserver
- * (observer.cc:5-79): sums over the system (not per particle); virial is
- * sum r V'(r) / 3.  Any pointer may be NULL. */
+ * (observer.cc:5-79): sums over the system (not per particle); KE over all
+ * four momentum lanes as the reference's (the fourth is zero unless the
+ * Langevin bath ran); virial is sum r V'(r) / 3.  Any pointer may be NULL. */
 int mdx_observe(mdx_system *sys, double *ke, double *pe, double *virial);
 /* Forces F_i = sum_j f_ij of the current list, [n][3] by id; p untouched
  * (the reference's kick at dt = 1 on p = 0, fcalculator.cc:182-223). */

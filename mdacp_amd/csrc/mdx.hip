@@ -806,6 +806,127 @@ __global__ void k_vmax(const double4 *__restrict__ mom, int n, double *__restric
   block_max_to(v2, partial);
 }
 
+// ------------------------------------------------------------ heat baths
+// Nose-Hoover state on the device: Variables::Zeta (variables.h:29)
+struct HeatDev {
+  double zeta;
+};
+
+// Fixed-order block sum (kBlock threads) to partial[blockIdx.x]
+__device__ __forceinline__ void block_sum_to(double v, double *__restrict__ partial) {
+  __shared__ double lds[kBlock];
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
+}
+
+// KineticEnergyObserver (observer.cc:5-15) per block: 0.5 p^2 over all
+// four lanes
+__global__ void k_ke_partials(const double4 *__restrict__ mom, int n, double *__restrict__ partial) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0;
+  if (i < n) {
+    const double4 p = mom[i];
+    e = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z + 0.5 * p.w * p.w;
+  }
+  block_sum_to(e, partial);
+}
+
+// ForceCalculator::HeatbathZeta (fcalculator.cc:979-985) with the
+// temperature MDManager::Temperature = KE / N / 1.5 (mdmanager.cc:531-548)
+// of the block sums (folded in a fixed order), or with t given (nb = 0).
+__global__ void k_nh_zeta(const double *__restrict__ partial, int nb, int n, double t_given,
+                          double T0, double tau, double dt, HeatDev *__restrict__ hd) {
+  __shared__ double lds[kBlock];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) acc += partial[b];
+  lds[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double t = nb > 0 ? lds[0] / (double)n / 1.5 : t_given;
+    const double t1 = (t - T0) / (tau * tau);
+    hd->zeta += t1 * (dt * 0.5);
+  }
+}
+
+// ForceCalculator::HeatbathMomenta (fcalculator.cc:987-999), all four lanes
+__global__ void k_nh_scale(double4 *__restrict__ mom, int n, double dt,
+                           const HeatDev *__restrict__ hd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double exp1 = exp(-(dt * 0.5) * hd->zeta);
+  double4 p = mom[i];
+  p.x *= exp1;
+  p.y *= exp1;
+  p.z *= exp1;
+  p.w *= exp1;
+  mom[i] = p;
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): ten rounds of two
+// 32x32 -> 64-bit products and xors, the key bumped by the Weyl constants
+__device__ __forceinline__ uint4 philox4x32(uint4 c, uint2 k) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    const unsigned long long a = (unsigned long long)0xD2511F53u * c.x;
+    const unsigned long long b = (unsigned long long)0xCD9E8D57u * c.z;
+    c = make_uint4((unsigned)(b >> 32) ^ c.y ^ k.x, (unsigned)b, (unsigned)(a >> 32) ^ c.w ^ k.y,
+                   (unsigned)a);
+    k.x += 0x9E3779B9u;
+    k.y += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// (0, 1] from 53 bits of two words
+__device__ __forceinline__ double uniform53(unsigned a, unsigned b) {
+  const unsigned long long m = ((unsigned long long)(a >> 5) << 26) | (unsigned long long)(b >> 6);
+  return ((double)m + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+// ForceCalculator::Langevin (fcalculator.cc:1001-1017) for the k-th kick:
+// p += (-gamma p + r) dt on all four lanes, r = hb_D n with n four standard
+// normals of (seed, id, k): two Philox blocks, counter (id, k lo, k hi,
+// 0 / 1), key (seed lo, seed hi), Box-Muller on 53-bit uniforms.
+__global__ void k_langevin(double4 *__restrict__ mom, const int *__restrict__ tag, int n,
+                           unsigned long long seed, unsigned long long k, double gamma, double hb_D,
+                           double dt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
+  const unsigned id = (unsigned)tag[i];
+  const uint4 w0 = philox4x32(make_uint4(id, (unsigned)k, (unsigned)(k >> 32), 0u), key);
+  const uint4 w1 = philox4x32(make_uint4(id, (unsigned)k, (unsigned)(k >> 32), 1u), key);
+  const double two_pi = 6.283185307179586;
+  double nrm[4];
+  {
+    const double r = sqrt(-2.0 * log(uniform53(w0.x, w0.y)));
+    const double a = two_pi * uniform53(w0.z, w0.w);
+    nrm[0] = r * cos(a);
+    nrm[1] = r * sin(a);
+  }
+  {
+    const double r = sqrt(-2.0 * log(uniform53(w1.x, w1.y)));
+    const double a = two_pi * uniform53(w1.z, w1.w);
+    nrm[2] = r * cos(a);
+    nrm[3] = r * sin(a);
+  }
+  double4 p = mom[i];
+  p.x += (-gamma * p.x + hb_D * nrm[0]) * dt;
+  p.y += (-gamma * p.y + hb_D * nrm[1]) * dt;
+  p.z += (-gamma * p.z + hb_D * nrm[2]) * dt;
+  p.w += (-gamma * p.w + hb_D * nrm[3]) * dt;
+  mom[i] = p;
+}
+
 // ------------------------------------------------------------ adjoint
 // Fixed-order block sum of K values per thread into partial[block * K + k].
 template <int K>
@@ -1068,8 +1189,10 @@ __global__ void __launch_bounds__(kTile) k_observe_tiles(
     if (r2 <= g.CL2) observe_term(r2, C0, C2, pe, vir);
   });
   if (act) {
+    // all four lanes, as the reference's observer (the fourth is zero
+    // unless the Langevin bath ran)
     const double4 p = mom[i];
-    ke = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z;
+    ke = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z + 0.5 * p.w * p.w;
     pe *= 0.5;
     vir *= 0.5;
   }
@@ -1275,6 +1398,10 @@ struct mdx_system {
   double vmax2_host = 0.0;          // max |p|^2 after the last kick
   StepCtl *dctl = nullptr, *hctl = nullptr;
   hipEvent_t evk[2 * kRing] = {};
+  // heat bath (SimulationInfo::ControlTemperature ..., simulationinfo.cc:11-23)
+  mdx_heatbath hb{};           // zeta here is stale; the device holds it
+  HeatDev *dhb = nullptr;
+  unsigned long long langevin_k = 0;   // Langevin kicks since mdx_set_state
   // timing
   bool timing = false;
   hipEvent_t ev[8] = {};
@@ -1648,6 +1775,53 @@ int refresh(mdx_system *s) {
   return launch_check(s, "refresh");
 }
 
+// ForceCalculator::HeatbathZeta with the current temperature, on the device
+int nh_zeta(mdx_system *s) {
+  const int n = s->n;
+  int rc = ensure_partials(s, nblk(n));
+  if (rc) return rc;
+  k_ke_partials<<<nblk(n), kBlock, 0, s->stream>>>(s->mom, n, s->partial);
+  k_nh_zeta<<<1, kBlock, 0, s->stream>>>(s->partial, nblk(n), n, 0.0, s->hb.aimed_temperature,
+                                         s->hb.tau, s->prm.dt, s->dhb);
+  return launch_check(s, "heatbath zeta");
+}
+
+int nh_scale(mdx_system *s) {
+  k_nh_scale<<<nblk(s->n), kBlock, 0, s->stream>>>(s->mom, s->n, s->prm.dt, s->dhb);
+  s->vmax_valid = false;
+  return launch_check(s, "heatbath momenta");
+}
+
+int langevin(mdx_system *s) {
+  const double hb_D = std::sqrt(2.0 * s->hb.gamma * s->hb.aimed_temperature / s->prm.dt);
+  k_langevin<<<nblk(s->n), kBlock, 0, s->stream>>>(s->mom, s->tag, s->n, s->hb.seed, s->langevin_k,
+                                                   s->hb.gamma, hb_D, s->prm.dt);
+  s->langevin_k++;
+  s->vmax_valid = false;
+  return launch_check(s, "langevin");
+}
+
+// One step of MDManager::Calculate under the heat bath, after its expiry
+// check (mdmanager.cc:154-186): half drift, image refresh, then
+//   Nose-Hoover (CalculateNoseHoover, mdmanager.cc:255-287): zeta from the
+//     temperature, momenta scaled, kick, scaled, zeta again, half drift;
+//   Langevin (CalculateLangevin, mdmanager.cc:289-315): kick, noise, and no
+//     trailing half drift.
+// Zeta stays on the device; nothing waits for the host.
+int heatbath_step(mdx_system *s) {
+  int rc;
+  if ((rc = drift(s)) || (rc = refresh(s))) return rc;
+  if (s->hb.type == MDX_HEATBATH_NOSE_HOOVER) {
+    if ((rc = nh_zeta(s)) || (rc = nh_scale(s)) || (rc = kick(s)) || (rc = nh_scale(s)) ||
+        (rc = nh_zeta(s)) || (rc = drift(s)))
+      return rc;
+  } else {
+    if ((rc = kick(s)) || (rc = langevin(s))) return rc;
+  }
+  return MDX_OK;
+}
+
+
 // PairList::IsPairListExpired (pairlist.cc:26-45)
 int expired(mdx_system *s, int *out) {
   if (!s->list_valid) {
@@ -2020,6 +2194,16 @@ int mdx_create(const mdx_params *prm, mdx_system **out) {
   if (!rc && hipHostMalloc((void **)&s->hstat, sizeof(DevStatus)) != hipSuccess) rc = MDX_ERR_HIP;
   if (!rc && hipHostMalloc((void **)&s->hctl, sizeof(StepCtl)) != hipSuccess) rc = MDX_ERR_HIP;
   if (!rc && hipMalloc((void **)&s->dctl, sizeof(StepCtl)) != hipSuccess) rc = MDX_ERR_HIP;
+  if (!rc && (rc = dev_alloc(s, &s->dhb, 1)) == MDX_OK &&
+      hipMemsetAsync(s->dhb, 0, sizeof(HeatDev), s->stream) != hipSuccess)
+    rc = MDX_ERR_HIP;
+  // SimulationInfo defaults (simulationinfo.cc:11-23); ControlTemperature = no
+  s->hb.type = MDX_HEATBATH_NONE;
+  s->hb.aimed_temperature = 1.0;
+  s->hb.tau = 0.1;
+  s->hb.gamma = 0.1;
+  s->hb.zeta = 0.0;
+  s->hb.seed = 0;
   if (!rc) {
     for (auto &e : s->evk)
       if (hipEventCreate(&e) != hipSuccess) rc = MDX_ERR_HIP;
@@ -2041,8 +2225,8 @@ int mdx_destroy(mdx_system *s) {
                   s->pos_tmp, s->ref_pos, s->mom, s->mom_tmp, s->gpos_tmp, s->vec_out,
                   s->tag, s->tag_tmp, s->gsrc, s->gcode, s->gsrc_tmp, s->gcode_tmp, s->cellid,
                   s->slot, s->order, s->inv, s->nimg, s->goff, s->cell_count, s->cell_start,
-                  s->cnt_in, s->cnt_out, s->scan_in, s->scan_out, s->scan_aux, s->nn,
-                  s->disp, s->partial, s->cand, s->io_buf, s->i_buf, s->dstat};
+                  s->cnt_in, s->cnt_out, s->scan_in, s->scan_out, s->scan_aux, s->nn, s->nb,
+                  s->disp, s->partial, s->cand, s->io_buf, s->i_buf, s->dstat, s->dhb};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (s->hstat) (void)hipHostFree(s->hstat);
@@ -2066,6 +2250,7 @@ int mdx_set_state(mdx_system *s, int n, const double *q3, const double *p3) {
   int rc = ensure_capacity(s, n);
   if (rc) return rc;
   s->n = n;
+  s->langevin_k = 0;
   // initial image capacity: the SL shell around the box at the mean
   // density, doubled, plus slack; grown on demand
   const double V = s->geom.L[0] * s->geom.L[1] * s->geom.L[2];
@@ -2142,6 +2327,52 @@ int mdx_calculate_force(mdx_system *s) {
   return MDX_OK;
 }
 
+int mdx_set_heatbath(mdx_system *s, const mdx_heatbath *hb) {
+  if (!s || !hb) return fail(s, MDX_ERR_ARG, "bad arguments to mdx_set_heatbath");
+  if (hb->type < MDX_HEATBATH_NONE || hb->type > MDX_HEATBATH_LANGEVIN)
+    return fail(s, MDX_ERR_ARG, "unknown heat bath type %d", hb->type);
+  if (!std::isfinite(hb->aimed_temperature) || hb->aimed_temperature < 0.0 ||
+      !std::isfinite(hb->zeta))
+    return fail(s, MDX_ERR_ARG, "heat bath temperature and zeta must be finite, temperature >= 0");
+  if (hb->type == MDX_HEATBATH_NOSE_HOOVER && !(hb->tau > 0.0 && std::isfinite(hb->tau)))
+    return fail(s, MDX_ERR_ARG, "Nose-Hoover needs a finite tau > 0");
+  if (hb->type == MDX_HEATBATH_LANGEVIN && !(hb->gamma >= 0.0 && std::isfinite(hb->gamma)))
+    return fail(s, MDX_ERR_ARG, "Langevin needs a finite gamma >= 0");
+  s->hb = *hb;
+  HeatDev h{hb->zeta};
+  HIPCHK(s, hipMemcpyAsync(s->dhb, &h, sizeof h, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return MDX_OK;
+}
+
+int mdx_get_heatbath(mdx_system *s, mdx_heatbath *hb) {
+  if (!s || !hb) return fail(s, MDX_ERR_ARG, "bad arguments to mdx_get_heatbath");
+  HeatDev h{};
+  HIPCHK(s, hipMemcpyAsync(&h, s->dhb, sizeof h, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  *hb = s->hb;
+  hb->zeta = h.zeta;
+  return MDX_OK;
+}
+
+int mdx_heatbath_zeta(mdx_system *s, double current_temperature) {
+  if (!s) return MDX_ERR_ARG;
+  if (!(s->hb.tau > 0.0)) return fail(s, MDX_ERR_STATE, "heat bath tau must be > 0");
+  k_nh_zeta<<<1, kBlock, 0, s->stream>>>(s->partial, 0, 1, current_temperature,
+                                         s->hb.aimed_temperature, s->hb.tau, s->prm.dt, s->dhb);
+  return launch_check(s, "heatbath zeta");
+}
+
+int mdx_heatbath_momenta(mdx_system *s) {
+  if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no state");
+  return nh_scale(s);
+}
+
+int mdx_langevin(mdx_system *s) {
+  if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no state");
+  return langevin(s);
+}
+
 int mdx_calculate(mdx_system *s, int nsteps, int *rebuilds) {
   if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no state");
   int nreb = 0, rc;
@@ -2153,6 +2384,12 @@ int mdx_calculate(mdx_system *s, int nsteps, int *rebuilds) {
     if (exp) {
       if ((rc = build_pairlist(s))) return rc;
       nreb++;
+    }
+    if (s->hb.type != MDX_HEATBATH_NONE) {
+      if ((rc = heatbath_step(s))) return rc;
+      done++;
+      s->n_steps++;
+      continue;
     }
     // queue as many steps as the budget is expected to last at the last
     // |p|max, so the guard halts near the end of the batch

@@ -208,6 +208,41 @@ void UpdatePositionHalf(Variables *vars, SimulationInfo *sinfo) {
   check(s, mdx_update_position_half(s));
 }
 
+// the heat bath of sinfo with zeta, handed to the device
+static void PushHeatbath(Variables *vars, SimulationInfo *sinfo, int type) {
+  mdx_system *s = vars->Device();
+  mdx_heatbath hb{};
+  hb.type = type;
+  hb.aimed_temperature = sinfo->AimedTemperature;
+  hb.tau = sinfo->HeatbathTau;
+  hb.gamma = sinfo->HeatbathGamma;
+  hb.zeta = vars->Zeta;
+  hb.seed = sinfo->LangevinSeed;
+  check(s, mdx_set_heatbath(s, &hb));
+}
+
+// fcalculator.cc:979-985, on the host as the reference keeps Zeta
+void HeatbathZeta(Variables *vars, double current_temperature, SimulationInfo *sinfo) {
+  const double dt2 = sinfo->TimeStep * 0.5;
+  const double tau = sinfo->HeatbathTau;
+  const double t1 = (current_temperature - sinfo->AimedTemperature) / (tau * tau);
+  vars->Zeta += t1 * dt2;
+}
+
+// fcalculator.cc:987-999
+void HeatbathMomenta(Variables *vars, SimulationInfo *sinfo) {
+  PushHeatbath(vars, sinfo, MDX_HEATBATH_NOSE_HOOVER);
+  mdx_system *s = vars->Device();
+  check(s, mdx_heatbath_momenta(s));
+}
+
+// fcalculator.cc:1001-1017 (the noise: mdx_langevin)
+void Langevin(Variables *vars, SimulationInfo *sinfo) {
+  PushHeatbath(vars, sinfo, MDX_HEATBATH_LANGEVIN);
+  mdx_system *s = vars->Device();
+  check(s, mdx_langevin(s));
+}
+
 std::vector<double> Forces(Variables *vars, MeshList *mesh) {
   (void)mesh;
   mdx_system *s = vars->Device();
@@ -264,12 +299,26 @@ void MDUnit::MakePairList() {   // MDUnit::MakePairList (mdunit.cc:211-215)
   mesh->MakeList(vars, sinfo, myrect);
 }
 
-void MDUnit::Calculate() {      // MDManager::Calculate, NVE (mdmanager.cc:154-186)
+// MDManager::Calculate (mdmanager.cc:154-186) with CalculateNoseHoover
+// (:255-287) and CalculateLangevin (:289-315) for one domain
+void MDUnit::Calculate() {
   if (IsPairListExpired()) MakePairList();
   ForceCalculator::UpdatePositionHalf(vars, sinfo);
   mesh->RefreshImages(vars);
-  ForceCalculator::CalculateForce(vars, mesh, sinfo);
-  ForceCalculator::UpdatePositionHalf(vars, sinfo);
+  if (sinfo->ControlTemperature && sinfo->HeatbathType == HT_NOSEHOOVER) {
+    ForceCalculator::HeatbathZeta(vars, Temperature(), sinfo);
+    ForceCalculator::HeatbathMomenta(vars, sinfo);
+    ForceCalculator::CalculateForce(vars, mesh, sinfo);
+    ForceCalculator::HeatbathMomenta(vars, sinfo);
+    ForceCalculator::HeatbathZeta(vars, Temperature(), sinfo);
+    ForceCalculator::UpdatePositionHalf(vars, sinfo);
+  } else if (sinfo->ControlTemperature) {
+    ForceCalculator::CalculateForce(vars, mesh, sinfo);
+    ForceCalculator::Langevin(vars, sinfo);
+  } else {
+    ForceCalculator::CalculateForce(vars, mesh, sinfo);
+    ForceCalculator::UpdatePositionHalf(vars, sinfo);
+  }
   vars->SimulationTime += sinfo->TimeStep;
 }
 

@@ -45,6 +45,8 @@ struct Error : std::runtime_error {
   int code;
 };
 
+enum { HT_NOSEHOOVER, HT_LANGEVIN };   // heat bath types (mdconfig.h)
+
 // the SimulationInfo fields on the path (simulationinfo.cc:5-51)
 struct SimulationInfo {
   SimulationInfo(double lx, double ly, double lz, double time_step = 0.001);
@@ -53,6 +55,13 @@ struct SimulationInfo {
   double SearchLength;   // CUTOFF_LENGTH + BufferLength
   double BufferLength;   // 0.3
   int CheckListLength;   // 500
+  // heat bath (simulationinfo.cc:11-23)
+  bool ControlTemperature = false;
+  int HeatbathType = HT_NOSEHOOVER;
+  double AimedTemperature = 1.0;
+  double HeatbathTau = 0.1;
+  double HeatbathGamma = 0.1;
+  unsigned long long LangevinSeed = 0;   // noise stream (mdx_heatbath.seed)
   int Device = 0;
 };
 
@@ -134,6 +143,10 @@ class PairList {  // pairlist.h:10-31
 namespace ForceCalculator {  // fcalculator.h:10-44
 void CalculateForce(Variables *vars, MeshList *mesh, SimulationInfo *sinfo);
 void UpdatePositionHalf(Variables *vars, SimulationInfo *sinfo);
+// heat bath (fcalculator.cc:979-1017); Variables::Zeta is the host value
+void HeatbathZeta(Variables *vars, double current_temperature, SimulationInfo *sinfo);
+void HeatbathMomenta(Variables *vars, SimulationInfo *sinfo);
+void Langevin(Variables *vars, SimulationInfo *sinfo);
 // forces at dt = 1 with p untouched, [pn][3] by particle id (test hook)
 std::vector<double> Forces(Variables *vars, MeshList *mesh);
 }  // namespace ForceCalculator
@@ -168,7 +181,7 @@ class MDUnit {
   void AddParticle(double x[D], double v[D], int type = 1);
   void MakePairList();
   bool IsPairListExpired() { return plist->IsPairListExpired(vars, mesh, sinfo); }
-  void Calculate();  // one NVE step
+  void Calculate();  // one step, NVE or under the heat bath of SimulationInfo
   double ObserveDouble(DoubleObserver *obs) { return obs->Observe(vars, mesh); }
   double Temperature();        // KE/N/1.5 (mdmanager.cc:531-533)
   double PotentialEnergy();    // per particle (mdmanager.cc:551-555)

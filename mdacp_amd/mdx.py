@@ -19,6 +19,8 @@ EXPORTS = (
     "mdx_update_position_half", "mdx_refresh_images", "mdx_calculate_force", "mdx_calculate",
     "mdx_observe", "mdx_forces", "mdx_pair_number", "mdx_pairs_canonical", "mdx_timing",
     "mdx_timing_reset", "mdx_kick_profile", "mdx_lj_trajectory",
+    "mdx_set_heatbath", "mdx_get_heatbath", "mdx_heatbath_zeta", "mdx_heatbath_momenta",
+    "mdx_langevin",
     "mdx_ensemble_create", "mdx_ensemble_destroy", "mdx_ensemble_last_error", "mdx_ensemble_size",
     "mdx_ensemble_set_state", "mdx_ensemble_get_state", "mdx_ensemble_calculate",
     "mdx_ensemble_status", "mdx_ensemble_observe", "mdx_ensemble_get_config",
@@ -45,6 +47,15 @@ class MdxLjResult(ctypes.Structure):
 class MdxEnsembleConfig(ctypes.Structure):
     _fields_ = [("tile", ctypes.c_int), ("grid_mult", ctypes.c_int), ("step_grid", ctypes.c_int),
                 ("stage_cap", ctypes.c_int), ("ms_per_step", ctypes.c_double)]
+
+
+class MdxHeatbath(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("aimed_temperature", ctypes.c_double),
+                ("tau", ctypes.c_double), ("gamma", ctypes.c_double), ("zeta", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
+
+
+HEATBATH = {"none": 0, "nose_hoover": 1, "langevin": 2}
 
 
 class MdxError(RuntimeError):
@@ -90,6 +101,11 @@ def load() -> ctypes.CDLL:
         "mdx_lj_trajectory": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                              ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_int, P(MdxLjResult), dp]),
+        "mdx_set_heatbath": (ctypes.c_int, [vp, P(MdxHeatbath)]),
+        "mdx_get_heatbath": (ctypes.c_int, [vp, P(MdxHeatbath)]),
+        "mdx_heatbath_zeta": (ctypes.c_int, [vp, ctypes.c_double]),
+        "mdx_heatbath_momenta": (ctypes.c_int, [vp]),
+        "mdx_langevin": (ctypes.c_int, [vp]),
         "mdx_ensemble_create": (ctypes.c_int, [P(MdxParams), ctypes.c_int, ip, dp, P(vp)]),
         "mdx_ensemble_destroy": (ctypes.c_int, [vp]),
         "mdx_ensemble_last_error": (ctypes.c_char_p, [vp]),
@@ -240,6 +256,27 @@ class System:
         a, b = ctypes.c_double(), ctypes.c_double()
         self._chk(self.lib.mdx_kick_profile(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def set_heatbath(self, kind: str, aimed_temperature: float = 1.0, tau: float = 0.1,
+                     gamma: float = 0.1, zeta: float = 0.0, seed: int = 0):
+        """SimulationInfo's heat bath (ControlTemperature / HeatbathType /
+        AimedTemperature / HeatbathTau / HeatbathGamma) and Variables::Zeta."""
+        hb = MdxHeatbath(HEATBATH[kind], aimed_temperature, tau, gamma, zeta, seed)
+        self._chk(self.lib.mdx_set_heatbath(self.h, ctypes.byref(hb)))
+
+    def heatbath(self) -> dict:
+        hb = MdxHeatbath()
+        self._chk(self.lib.mdx_get_heatbath(self.h, ctypes.byref(hb)))
+        return {k: getattr(hb, k) for k, _ in MdxHeatbath._fields_}
+
+    def heatbath_zeta(self, current_temperature: float):
+        self._chk(self.lib.mdx_heatbath_zeta(self.h, current_temperature))
+
+    def heatbath_momenta(self):
+        self._chk(self.lib.mdx_heatbath_momenta(self.h))
+
+    def langevin(self):
+        self._chk(self.lib.mdx_langevin(self.h))
 
     def lj_trajectory(self, eps=1.0, sigma=1.0, nsteps=200, nobs=100, checkpoint_every=20,
                       p_target=1.5, pe_target=-4.0, grad=True, want_dq0=True):

@@ -44,7 +44,7 @@ struct orc_sys {
   int *cand;
   /* heat bath (simulationinfo.cc:11-23, variables.h:29) */
   orc_heatbath hb;
-  uint64_t step;         /* steps since orc_set_state (Langevin counter) */
+  uint64_t step;         /* Langevin kicks since orc_set_state (noise counter) */
 };
 
 /* variables.cc:21-27 */
@@ -588,7 +588,6 @@ int orc_calculate(orc_sys *s, int nsteps, int *rebuilds) {
       orc_calculate_force(s);
       orc_update_position_half(s);
     }
-    s->step++;
   }
   if (rebuilds) *rebuilds = nreb;
   return ORC_OK;
@@ -628,6 +627,7 @@ void orc_langevin(orc_sys *s) {
       s->p[i][d] += (-hb_gamma * s->p[i][d] + r) * dt;
     }
   }
+  s->step++;
 }
 
 /* MDManager::Temperature (mdmanager.cc:531-533) over KineticEnergy

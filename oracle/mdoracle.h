@@ -49,8 +49,9 @@ typedef struct orc_sys orc_sys;
  * Variables::Zeta, variables.h:29).  type 0 = no control (NVE), 1 =
  * Nose-Hoover, 2 = Langevin.  The reference draws Langevin noise from a
  * per-thread std::mt19937 (fcalculator.cc:1002-1017), which fixes no
- * particle-to-number map; here the noise of particle `tag` at step k (steps
- * counted from orc_set_state) is orc_langevin_normals(seed, tag, k). */
+ * particle-to-number map; here the noise of particle `tag` in the k-th
+ * Langevin kick since orc_set_state (k from 0) is
+ * orc_langevin_normals(seed, tag, k). */
 typedef struct {
   int type;
   double aimed_temperature;

@@ -2,8 +2,10 @@
 // the reference's Benchmark project drives MDManager (benchmark.cc:13-52):
 // inject particles, MakePairList, then Calculate() steps with observations.
 //
-// usage: shim_driver <state.bin> <steps> <observe_every> <out_prefix>
-// prints one line per observation: step KE PE virial constructions
+// usage: shim_driver <state.bin> <steps> <observe_every> <out_prefix> [T tau]
+// prints one line per observation: step KE PE virial constructions zeta;
+// with T and tau the steps run under the Nose-Hoover heat bath
+// (ControlTemperature, AimedTemperature = T, HeatbathTau = tau).
 // writes <out_prefix>_q.f64 / _p.f64 (by particle id) at the end.
 #include <cstdint>
 #include <cstdio>
@@ -29,6 +31,12 @@ int main(int argc, char **argv) {
   const int steps = atoi(argv[2]), every = atoi(argv[3]);
   try {
     mdx::SimulationInfo sinfo(L[0], L[1], L[2], dt);
+    if (argc >= 7) {
+      sinfo.ControlTemperature = true;
+      sinfo.HeatbathType = mdx::HT_NOSEHOOVER;
+      sinfo.AimedTemperature = atof(argv[5]);
+      sinfo.HeatbathTau = atof(argv[6]);
+    }
     mdx::MDUnit unit(&sinfo);
     for (int64_t i = 0; i < n; i++) {
       double x[mdx::D] = {q[3 * i], q[3 * i + 1], q[3 * i + 2], 0};
@@ -43,8 +51,9 @@ int main(int argc, char **argv) {
       unit.Calculate();
       if (s % every == 0) {
         unit.GetMeshList()->RefreshImages(unit.GetVariables());
-        printf("%d %.17g %.17g %.17g %d\n", s, unit.ObserveDouble(&ke), unit.ObserveDouble(&pe),
-               unit.ObserveDouble(&vir), unit.GetMeshList()->GetNumberOfConstructions());
+        printf("%d %.17g %.17g %.17g %d %.17g\n", s, unit.ObserveDouble(&ke),
+               unit.ObserveDouble(&pe), unit.ObserveDouble(&vir),
+               unit.GetMeshList()->GetNumberOfConstructions(), unit.GetVariables()->Zeta);
       }
     }
     mdx::Variables *vars = unit.GetVariables();
