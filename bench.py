@@ -26,6 +26,9 @@ Two more objects ride on the same line:
              T 1, seed 5): 200 steps, loss over the last 100, checkpoints
              every 20; forward and reverse-sweep device time, fwd+adjoint
              steps per second and the adjoint/forward cost ratio.
+  `heatbath` the N=2.05M workload stepped under the Nose-Hoover (target
+             1.0, tau 0.1) and Langevin (target 1.0, gamma 0.1) heat baths,
+             200 steps each after 20 warm-up steps: atom-steps/s.
 """
 from __future__ import annotations
 
@@ -179,6 +182,26 @@ def adjoint():
             "dloss_deps": g["dloss_deps"], "dloss_dsigma": g["dloss_dsigma"]}
 
 
+def heatbath(state, steps: int = 200, warmup: int = 20):
+    import torch
+    from mdacp_amd.mdx import System
+
+    out = {}
+    for kind, kw in (("nose_hoover", dict(tau=0.1)), ("langevin", dict(gamma=0.1, seed=1))):
+        with System.from_state(state, device=0) as sy:
+            sy.set_heatbath(kind, 1.0, **kw)
+            sy.make_pairlist()
+            sy.calculate(warmup)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sy.calculate(steps)
+            torch.cuda.synchronize()
+            sec = time.perf_counter() - t0
+        out[kind] = {"value": state.n * steps / sec, "unit": "atom-steps/s",
+                     "ms_per_step": sec * 1e3 / steps, "steps": steps}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +212,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-heatbath", action="store_true")
     ap.add_argument("--only", choices=("batched", "adjoint"),
                     help="run one form alone and print its object (for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -256,6 +280,8 @@ def main():
         out["batched"] = batched(args.batched_steps, args.batched_steps)
     if not args.no_adjoint:
         out["adjoint"] = adjoint()
+    if not args.no_heatbath:
+        out["heatbath"] = heatbath(state)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(state, args.cpu_steps, 2)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
