@@ -123,6 +123,26 @@ def test_oracle_momentum_and_reversibility():
     assert np.abs(p2 + s.p).max() < 1e-8
 
 
+def test_oracle_energy_error_second_order_in_dt():
+    """The oracle's integrator is second order: over one time unit at
+    N=4000 the largest total energy error shrinks 4x when dt halves."""
+    errs = []
+    for dt in (0.004, 0.002):
+        s = golden_state(load_golden("fcc4000"))
+        s.dt = dt
+        o = O.Oracle(s)
+        o.make_pairlist()
+        ke, pe, _ = o.observe()
+        e0, worst = ke + pe, 0.0
+        for _ in range(10):
+            o.calculate(int(round(0.1 / dt)))
+            o.refresh_images()
+            ke, pe, _ = o.observe()
+            worst = max(worst, abs(ke + pe - e0) / abs(e0))
+        errs.append(worst)
+    assert 3.5 < errs[0] / errs[1] < 4.5
+
+
 def test_planted_mistakes_are_caught():
     """A wrong term or a missed image changes a pinned quantity."""
     s = golden_state(load_golden("fcc4000_pert"))
