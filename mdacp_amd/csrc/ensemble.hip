@@ -243,7 +243,7 @@ __device__ void e_stage_plane(const EnsDev &d, const Rep &rp, int p, const doubl
   if (mid) {
     // branch-free two-level gather: every item reads an image entry (a
     // clamped index for reals) and selects one load at a valid address
-    stage_range(sm, 6 * p, 6 * p + 6, s3, [&](int k, int slot) {
+    stage_plane(sm, p, s3, [&](int k, int slot) {
       const int idx = max(slot - d.NR, rp.ibase);
       const int gs = d.gsrc[idx], gc = d.gcode[idx];
       const bool img = (k & 1) != 0;
@@ -251,7 +251,7 @@ __device__ void e_stage_plane(const EnsDev &d, const Rep &rp, int p, const doubl
       return image_of(src[img ? gs : slot], img ? gc : kCodeNone, rp.g.L);
     });
   } else {
-    stage_range(sm, 6 * p, 6 * p + 6, s3,
+    stage_plane(sm, p, s3,
                 [&](int k, int slot) { return (k & 1) ? d.qe[slot] : reals[slot]; });
   }
 }

@@ -443,7 +443,7 @@ __device__ __forceinline__ void walk_planes(const TileSmem &sm, double *s3, Load
                                             int nb, F f) {
   for (int p = 0; p < kPlanes; p++) {
     if (p > 0) __syncthreads();   // the previous plane is read
-    stage_range(sm, 6 * p, 6 * p + 6, s3, load);
+    stage_plane(sm, p, s3, load);
     prep(p);
     if (!act) continue;
     const unsigned lim = (unsigned)(kEntryBytes * (sm.seg_base[6 * p + 6] - sm.seg_base[6 * p]));
@@ -561,7 +561,7 @@ __global__ void __launch_bounds__(kTile) k_build_tiles(
   int b1 = 0, b2 = 0;
   for (int p = 0; p < kPlanes; p++) {
     if (p > 0) __syncthreads();   // the previous plane is read
-    stage_range(sm, 6 * p, 6 * p + 6, sx, [&](int, int slot) { return pos[slot]; });
+    stage_plane(sm, p, sx, [&](int, int slot) { return pos[slot]; });
     const int pb = sm.seg_base[6 * p];
     if (act) {
       for (int r = 3 * p; r < 3 * p + 3; r++) {
@@ -730,7 +730,7 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
   const double xi = qi.x, yi = qi.y, zi = qi.z, CL2 = g.CL2;
   for (int p = 0; p < kPlanes; p++) {
     if (p > 0) __syncthreads();   // the previous plane is read
-    stage_range(sm, 6 * p, 6 * p + 6, sx, [&](int, int slot) { return pos[slot]; });
+    stage_plane(sm, p, sx, [&](int, int slot) { return pos[slot]; });
     if (!act) continue;
     plane_groups<kTile>(lst, plane_begin(p, bnd), plane_end(p, bnd, m), stage_mem,
                         [&](int u, double xj, double yj, double zj) {

@@ -73,45 +73,57 @@ struct TileSmem {
   int seg_base[kSeg + 1];
 };
 
-// Copy segments [k0, k1) of a tile (seg_base prefix sums filled) into LDS
-// as interleaved (x, y, z), positions relative to seg_base[k0]: staged
-// particle a of segment k comes from load(k, seg_start[k] + a - seg_base[k]).
-// The segments are walked as one index space, four items per thread at a
-// time: the segment of each item is found first (an unrolled binary search
-// over the LDS prefix sums), then all four global loads are issued
-// together, so their latencies overlap.  Items past the end load the first
-// real of the tile's own row (segment 8, never empty) and are not stored,
-// so no branch separates the loads.  Every thread calls it; it ends with a
-// barrier.
+// Copy one z-plane of a tile (segments 6p .. 6p+5, seg_base prefix sums
+// filled) into LDS as interleaved (x, y, z), positions relative to the
+// plane's first segment: staged particle a of segment k comes from load(k,
+// seg_start[k] + a - seg_base[k]).  The segments are walked as one index
+// space, four items per thread at a time.  The plane's six segment starts
+// and offsets sit in scalar registers, so an item's segment is found by
+// five comparisons with no LDS round trip; then all four global loads are
+// issued together, so their latencies overlap.  Items past the end load
+// the first real of the tile's own row (segment 8, never empty) and are not
+// stored, so no branch separates the loads.  Every thread calls it; it ends
+// with a barrier.
 constexpr double kFar = 1.0e100;
+constexpr int kPlaneSeg = 6;
 
 template <class Load>
-__device__ __forceinline__ void stage_range(const TileSmem &sm, int k0, int k1, double *s3, Load load) {
+__device__ __forceinline__ void stage_plane(const TileSmem &sm, int p, double *s3, Load load) {
   constexpr int kBatch = 4;
-  const int b0 = sm.seg_base[k0], total = sm.seg_base[k1] - b0;
-  for (int base = 0; base < total; base += kBatch * (int)blockDim.x) {
+  const int k0 = kPlaneSeg * p;
+  int base[kPlaneSeg], delta[kPlaneSeg];
+#pragma unroll
+  for (int k = 0; k < kPlaneSeg; k++) {
+    base[k] = __builtin_amdgcn_readfirstlane(sm.seg_base[k0 + k]);
+    delta[k] = __builtin_amdgcn_readfirstlane(sm.seg_start[k0 + k] - sm.seg_base[k0 + k]);
+  }
+  const int b0 = base[0];
+  const int total = __builtin_amdgcn_readfirstlane(sm.seg_base[k0 + kPlaneSeg]) - b0;
+  const int pad_slot = __builtin_amdgcn_readfirstlane(sm.seg_start[8]);
+  for (int bb = 0; bb < total; bb += kBatch * (int)blockDim.x) {
     int seg[kBatch], slot[kBatch];
 #pragma unroll
     for (int j = 0; j < kBatch; j++) {
-      const int u = base + j * (int)blockDim.x + (int)threadIdx.x;
+      const int u = bb + j * (int)blockDim.x + (int)threadIdx.x;
       const int a = b0 + u;
-      int k = k0;
+      // the last segment starting at or before a (empty ones are passed)
+      int dk = delta[0], sk = k0;
 #pragma unroll
-      for (int step = 16; step > 0; step >>= 1) {
-        const int kk = k + step;
-        const bool up = kk < k1 && sm.seg_base[kk < kSeg ? kk : kSeg] <= a;
-        k = up ? kk : k;
+      for (int k = 1; k < kPlaneSeg; k++) {
+        const bool up = a >= base[k];
+        dk = up ? delta[k] : dk;
+        sk = up ? k0 + k : sk;
       }
       const bool in = u < total;
-      seg[j] = in ? k : 8;
-      slot[j] = in ? sm.seg_start[k] + (a - sm.seg_base[k]) : sm.seg_start[8];
+      seg[j] = in ? sk : 8;
+      slot[j] = in ? a + dk : pad_slot;
     }
     double4 q[kBatch];
 #pragma unroll
     for (int j = 0; j < kBatch; j++) q[j] = load(seg[j], slot[j]);
 #pragma unroll
     for (int j = 0; j < kBatch; j++) {
-      const int u = base + j * (int)blockDim.x + (int)threadIdx.x;
+      const int u = bb + j * (int)blockDim.x + (int)threadIdx.x;
       if (u < total) {
         s3[3 * u] = q[j].x;
         s3[3 * u + 1] = q[j].y;
@@ -123,11 +135,6 @@ __device__ __forceinline__ void stage_range(const TileSmem &sm, int k0, int k1, 
   // any cutoff, so a masked list entry can point here
   if (threadIdx.x == 0) s3[3 * total] = s3[3 * total + 1] = s3[3 * total + 2] = kFar;
   __syncthreads();
-}
-
-template <class Load>
-__device__ __forceinline__ void stage_segments(const TileSmem &sm, double *s3, Load load) {
-  stage_range(sm, 0, kSeg, s3, load);
 }
 
 // A tile's partners lie in three z-planes of cell rows (dz = -1, 0, +1):
@@ -201,37 +208,57 @@ struct GroupWriter {
 // ring slot j % kPrefetch, which then loads group j + kPrefetch (clamped
 // to the last group: a reload that is never used).  The ring turns once
 // per loop pass, so no register copies; the groups past the last full
-// turn compute from the ring as it stands.
-constexpr int kPrefetch = 2;
+// turn compute from the ring as it stands.  Partner positions are read
+// from LDS one entry ahead: the reads of entry k + 1 (the next group's
+// first entry after a group's last, the clamped reload past the end) are
+// issued before the arithmetic of entry k, and scheduling barriers keep
+// them there, so their latency runs under that arithmetic.
+constexpr int kPrefetch = 3;
 template <int S, class Term>
 __device__ __forceinline__ void plane_groups(const unsigned short *__restrict__ lst, int kb, int ke,
                                              const double *s3, Term term) {
   if (kb >= ke) return;
   auto load = [&](int k) { return *reinterpret_cast<const uint4 *>(lst + list_index<S>(k)); };
-  auto run = [&](const uint4 &w) {
-    const unsigned h[4] = {w.x, w.y, w.z, w.w};
+  auto entry = [&](const uint4 &w, int u) {
+    const unsigned h = u < 2 ? w.x : (u < 4 ? w.y : (u < 6 ? w.z : w.w));
+    return staged_at(s3, (h >> (16 * (u % 2))) & 0xffffu);
+  };
+  double x, y, z;   // the partner of the entry in hand
+  auto run = [&](const uint4 &w, const uint4 &next) {
 #pragma unroll
     for (int u = 0; u < kILP; u++) {
-      const double *q = staged_at(s3, (h[u / 2] >> (16 * (u % 2))) & 0xffffu);
-      term(u, q[0], q[1], q[2]);
+      const double *q = u + 1 < kILP ? entry(w, u + 1) : entry(next, 0);
+      const double nx = q[0], ny = q[1], nz = q[2];
+      __builtin_amdgcn_sched_barrier(0);
+      term(u, x, y, z);
+      __builtin_amdgcn_sched_barrier(0);
+      x = nx;
+      y = ny;
+      z = nz;
     }
   };
   const int last = ke - kILP, ng = (ke - kb) / kILP;
   uint4 ring[kPrefetch];
 #pragma unroll
   for (int d = 0; d < kPrefetch; d++) ring[d] = load(min(kb + d * kILP, last));
+  {
+    const double *q = entry(ring[0], 0);
+    x = q[0];
+    y = q[1];
+    z = q[2];
+  }
   int k = kb;
   for (int j = 0; j < ng / kPrefetch; j++, k += kPrefetch * kILP) {
 #pragma unroll
     for (int d = 0; d < kPrefetch; d++) {
-      run(ring[d]);
+      run(ring[d], ring[(d + 1) % kPrefetch]);
       ring[d] = load(min(k + (kPrefetch + d) * kILP, last));
     }
   }
   const int rem = ng % kPrefetch;
 #pragma unroll
   for (int d = 0; d < kPrefetch - 1; d++)
-    if (d < rem) run(ring[d]);
+    if (d < rem) run(ring[d], ring[d + 1]);
 }
 
 // Order of the interior cell rows (y, z) in the tile array: strips of h
