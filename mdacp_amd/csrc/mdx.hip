@@ -99,20 +99,27 @@ __global__ void k_max_partials(const double *__restrict__ partial, int nb, doubl
 // steps can be queued without a host round trip each.  The guard kernel
 // ending step b folds |p|max^2 of that step's kick, records it in ring[b],
 // and applies the budget update the reference makes at the start of the
-// next step; if the budget could be spent (below a margin that covers
-// rounding of the device sqrt) or a particle is too fast, it raises halt
-// and every later kernel of the batch returns at once.  The host replays
-// the budget exactly from ring[] (steps_done entries).
+// next step.  Once the budget is spent the reference checks displacements
+// at the start of every step (pairlist.cc:40-43).  Steps from index
+// first_checked on carry that check on the device (k_chk_* below): the
+// guard queues it when the budget is surely negative.  Otherwise, if the
+// budget could be spent (below a margin that covers rounding of the device
+// sqrt) or a particle is too fast, the guard raises halt and every later
+// kernel of the batch returns at once; the check raises halt when the list
+// expired.  The host replays the budget exactly from ring[] (steps_done
+// entries) and redoes the check that expired the list.
 constexpr int kRing = 64;
 struct StepCtl {
   double budget, dt, buffer;
-  int halt, steps_done, too_fast, pad;
+  int halt, steps_done, too_fast, first_checked;
+  int check, cand, expired, checks_passed;
+  double cd2;
   double ring[kRing];
 };
 
 constexpr int kGuard = 1024;   // threads of the guard: four independent loads each per round
-__global__ void __launch_bounds__(kGuard) k_step_guard(const double *__restrict__ partial, int nb,
-                                                       StepCtl *__restrict__ ctl) {
+__device__ __forceinline__ void step_guard(const double *__restrict__ partial, int nb,
+                                           StepCtl *__restrict__ ctl) {
   __shared__ double lds[kGuard];
   if (ctl->halt) return;
   double m[4] = {0.0, 0.0, 0.0, 0.0};
@@ -142,10 +149,105 @@ __global__ void __launch_bounds__(kGuard) k_step_guard(const double *__restrict_
       b = b - vmax * 2.0 * ctl->dt;
       ctl->budget = b;
     }
-    if (b < 1e-9 || k + 1 >= kRing) ctl->halt = 1;
+    if (k + 1 >= kRing) {
+      ctl->halt = 1;
+    } else if (b < 1e-9) {
+      if (k + 1 >= ctl->first_checked && b < -1e-9 && !ctl->too_fast) ctl->check = 1;
+      else ctl->halt = 1;
+    }
   }
 }
 
+// The displacement check (PairList::CheckByDisplacement, pairlist.cc:68-121)
+// at the start of a step queued by the guard, in four kernels: the
+// displacement of every real (end of the last step, in pos) and image (its
+// mid-step refresh, in mid) from its place at the build, with block maxima;
+// the threshold; the candidates; the decision.  Each returns at once unless
+// a check is queued and the batch runs.  Arithmetic as the host path's:
+// disp2 for the distances, correctly rounded sqrt and no contraction for
+// the threshold.
+__device__ __forceinline__ bool check_queued(const StepCtl *ctl) { return ctl->check && !ctl->halt; }
+
+__device__ __forceinline__ double4 check_pos(const double4 *pos, const double4 *mid, int n, int i) {
+  return i < n ? pos[i] : mid[i];
+}
+
+__global__ void k_chk_disp(const StepCtl *__restrict__ ctl, const double4 *__restrict__ pos,
+                           const double4 *__restrict__ mid, const double4 *__restrict__ ref, int n,
+                           int tn, double *__restrict__ disp, double *__restrict__ partial) {
+  if (!check_queued(ctl)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  if (i < tn) {
+    d2 = disp2(ref[i], check_pos(pos, mid, n, i));
+    disp[i] = d2;
+  }
+  block_max_to(d2, partial);
+}
+
+__global__ void k_chk_fold(StepCtl *__restrict__ ctl, const double *__restrict__ partial, int nb) {
+#pragma clang fp contract(off)
+  if (!check_queued(ctl)) return;
+  __shared__ double lds[kBlock];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) m = fmax(m, partial[b]);
+  lds[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double c_distance = ctl->buffer - __dsqrt_rn(lds[0]);
+    ctl->cd2 = c_distance * c_distance;
+    ctl->cand = 0;
+  }
+}
+
+__global__ void k_chk_cand(StepCtl *__restrict__ ctl, const double *__restrict__ disp, int tn,
+                           int cap, int *__restrict__ cand) {
+  if (!check_queued(ctl)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= tn || !(disp[i] > ctl->cd2)) return;
+  const int k = atomicAdd(&ctl->cand, 1);
+  if (k < cap) cand[k] = i;
+}
+
+__global__ void __launch_bounds__(kGuard) k_chk_decide(StepCtl *__restrict__ ctl,
+                                                       const int *__restrict__ cand, int cap,
+                                                       const double4 *__restrict__ pos,
+                                                       const double4 *__restrict__ mid,
+                                                       const double4 *__restrict__ ref, int n,
+                                                       double SL2, double CL2) {
+  __shared__ int hit;
+  if (!check_queued(ctl)) return;
+  const int l = ctl->cand;
+  int expired = 0;
+  if (l >= cap) {
+    expired = 1;
+  } else if (l > 1) {
+    if (threadIdx.x == 0) hit = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < l * l; t += kGuard) {
+      const int a = t / l, b = t % l;
+      if (b <= a) continue;
+      const int i1 = cand[a], i2 = cand[b];
+      if (disp2(ref[i1], ref[i2]) < SL2) continue;
+      if (disp2(check_pos(pos, mid, n, i1), check_pos(pos, mid, n, i2)) < CL2) hit = 1;
+    }
+    __syncthreads();
+    expired = hit;
+  }
+  if (threadIdx.x == 0) {
+    ctl->check = 0;
+    if (expired) {
+      ctl->expired = 1;
+      ctl->halt = 1;
+    } else {
+      ctl->checks_passed = ctl->checks_passed + 1;
+    }
+  }
+}
 
 // ---------------------------------------------------------------- scans
 // Exclusive scan of an int array, three passes, each block staging a
@@ -774,13 +876,22 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
   if (MODE != 1) block_max_to(v2, partial);
 }
 
-// images of a mid-step buffer from its reals (the refresh of
-// mdunit.cc:181-191 at the next step's mid-step positions)
-__global__ void k_refresh_mid(double4 *__restrict__ mid, const int *__restrict__ gsrc,
-                              const int *__restrict__ gcode, int n, int g, Geom geom,
-                              const int *__restrict__ halt) {
-  if (*halt) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+// The guard ending a step (block 0) and the image refresh of the next
+// step's mid-step buffer (the other blocks, kGuard images each) in one
+// launch.  The refresh blocks may read halt before or after block 0 raises
+// it: a refresh of a step that will not run is never read.
+__global__ void __launch_bounds__(kGuard) k_guard_refresh(const double *__restrict__ partial, int nb,
+                                                          StepCtl *__restrict__ ctl,
+                                                          double4 *__restrict__ mid,
+                                                          const int *__restrict__ gsrc,
+                                                          const int *__restrict__ gcode, int n,
+                                                          int g, Geom geom) {
+  if (blockIdx.x == 0) {
+    step_guard(partial, nb, ctl);
+    return;
+  }
+  if (__hip_atomic_load(&ctl->halt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const int k = (blockIdx.x - 1) * kGuard + threadIdx.x;
   if (k >= g) return;
   mid[n + k] = image_of(mid[gsrc[k]], gcode[k], geom.L);
 }
@@ -1397,6 +1508,7 @@ struct mdx_system {
   bool vmax_valid = false;
   double vmax2_host = 0.0;          // max |p|^2 after the last kick
   StepCtl *dctl = nullptr, *hctl = nullptr;
+  int check_steps = 4;              // device-checked steps queued after a spent budget
   hipEvent_t evk[2 * kRing] = {};
   // heat bath (SimulationInfo::ControlTemperature ..., simulationinfo.cc:11-23)
   mdx_heatbath hb{};           // zeta here is stale; the device holds it
@@ -1706,9 +1818,10 @@ int kick(mdx_system *s) {
 // state after Calculate(): reals at the end of the step, images at their
 // mid-step refresh.  Same arithmetic as drift(); refresh(); kick(); drift().
 // *ran receives the number of steps the device completed (>= 1).
-int run_batch(mdx_system *s, int want, int *ran) {
-  const int n = s->n, g = s->g, grid = tile_grid(s->ntiles);
+int run_batch(mdx_system *s, int want, int first_checked, int *ran) {
+  const int n = s->n, g = s->g, grid = tile_grid(s->ntiles), tn = n + g;
   const int B = std::max(1, std::min(want, kRing));
+  const int Nl = tn > s->prm.check_list_length ? s->prm.check_list_length : tn;
   const double dt = s->prm.dt;
   StepCtl *h = s->hctl;
   h->budget = s->buffer_length;
@@ -1717,6 +1830,12 @@ int run_batch(mdx_system *s, int want, int *ran) {
   h->halt = 0;
   h->steps_done = 0;
   h->too_fast = 0;
+  h->first_checked = std::max(1, first_checked);
+  h->check = 0;
+  h->cand = 0;
+  h->expired = 0;
+  h->checks_passed = 0;
+  h->cd2 = 0.0;
   HIPCHK(s, hipMemcpyAsync(s->dctl, h, offsetof(StepCtl, ring), hipMemcpyHostToDevice, s->stream));
   ev_record(s, 4);
   k_drift_refresh<<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->gsrc, s->gcode, n, g,
@@ -1725,15 +1844,25 @@ int run_batch(mdx_system *s, int want, int *ran) {
   for (int b = 0; b < B; b++) {
     const double4 *cur = (b & 1) ? s->pos_mid : s->pos_alt;
     double4 *nxt = (b & 1) ? s->pos_alt : s->pos_mid;
+    if (b >= h->first_checked) {
+      // displacement check at the start of step b: reals at the end of step
+      // b - 1 (pos), images at its mid-step refresh (its mid-step buffer)
+      const double4 *prev = ((b - 1) & 1) ? s->pos_mid : s->pos_alt;
+      k_chk_disp<<<nblk(tn), kBlock, 0, s->stream>>>(s->dctl, s->pos, prev, s->ref_pos, n, tn,
+                                                      s->disp, s->pmax);
+      k_chk_fold<<<1, kBlock, 0, s->stream>>>(s->dctl, s->pmax, nblk(tn));
+      k_chk_cand<<<nblk(tn), kBlock, 0, s->stream>>>(s->dctl, s->disp, tn, Nl, s->cand);
+      k_chk_decide<<<1, kGuard, 0, s->stream>>>(s->dctl, s->cand, Nl, s->pos, prev, s->ref_pos, n,
+                                                s->geom.SL2, s->geom.CL2);
+    }
     if (s->timing) (void)hipEventRecord(s->evk[2 * b], s->stream);
     k_kick_tiles<3><<<grid, kTile, plane_bytes(s), s->stream>>>(
         cur, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn,
         s->nb, make_pairk(s->C2, dt), dt, s->pos, s->pmax, LJ{}, &s->dctl->halt, nxt);
     if (s->timing) (void)hipEventRecord(s->evk[2 * b + 1], s->stream);
-    k_step_guard<<<1, kGuard, 0, s->stream>>>(s->pmax, grid, s->dctl);
-    if (g > 0 && b + 1 < B)
-      k_refresh_mid<<<nblk(g), kBlock, 0, s->stream>>>(nxt, s->gsrc, s->gcode, n, g, s->geom,
-                                                        &s->dctl->halt);
+    const int refresh_blocks = (g > 0 && b + 1 < B) ? (g + kGuard - 1) / kGuard : 0;
+    k_guard_refresh<<<1 + refresh_blocks, kGuard, 0, s->stream>>>(s->pmax, grid, s->dctl, nxt,
+                                                                   s->gsrc, s->gcode, n, g, s->geom);
   }
   if (g > 0)
     k_images_from_mid<<<nblk(g), kBlock, 0, s->stream>>>(s->pos, s->pos_alt, s->pos_mid, n, g,
@@ -2392,18 +2521,25 @@ int mdx_calculate(mdx_system *s, int nsteps, int *rebuilds) {
       continue;
     }
     // queue as many steps as the budget is expected to last at the last
-    // |p|max, so the guard halts near the end of the batch
-    int want = nsteps - done;
+    // |p|max, then a few steps that check displacements on the device (as
+    // many as the last pairlist survived, plus one), so that the batch runs
+    // on into the checks and ends near the expiry
+    int want = nsteps - done, first_checked = kRing;
     if (!(s->buffer_length > 0.0)) {
-      want = 1;
+      want = std::min(want, s->check_steps + 1);
+      first_checked = 1;
     } else if (s->vmax_valid && s->vmax2_host > 0.0) {
       const double per = 2.0 * std::sqrt(s->vmax2_host) * s->prm.dt;
-      want = std::min(want, (int)std::min(s->buffer_length / per + 1.0, (double)kRing));
+      const int budget_steps = (int)std::min(s->buffer_length / per + 1.0, (double)kRing);
+      first_checked = budget_steps;
+      want = std::min(want, std::min(budget_steps + s->check_steps, kRing));
     }
     int ran = 0;
-    if ((rc = run_batch(s, want, &ran))) return rc;
+    if ((rc = run_batch(s, want, first_checked, &ran))) return rc;
     // replay the budget at the starts of the batch's later steps exactly;
-    // the device only went on while the budget stayed positive
+    // the device only went on while the budget stayed positive, or past a
+    // displacement check it passed
+    const int checked_from = s->hctl->first_checked;
     for (int b = 0; b + 1 < ran; b++) {
       const double vmax = std::sqrt(s->hctl->ring[b]);
       if (s->buffer_length > 0.0) {
@@ -2411,9 +2547,13 @@ int mdx_calculate(mdx_system *s, int nsteps, int *rebuilds) {
           return fail(s, MDX_ERR_TOO_FAST, "Too fast particles exists. max_velocity = %f", vmax);
         s->buffer_length = s->buffer_length - vmax * 2.0 * s->prm.dt;
       }
-      if (s->buffer_length < 0.0)
+      if (s->buffer_length < 0.0 && b + 1 < checked_from)
         return fail(s, MDX_ERR_STATE, "step guard ran past a spent pairlist budget");
     }
+    if (s->hctl->expired)
+      s->check_steps = std::min(16, std::max(2, s->hctl->checks_passed + 2));
+    else if (s->hctl->checks_passed > 0 && ran == want)
+      s->check_steps = std::min(16, 2 * s->check_steps);
     s->vmax2_host = s->hctl->ring[ran - 1];
     s->vmax_valid = true;
     done += ran;
