@@ -117,25 +117,40 @@ struct StepCtl {
   double ring[kRing];
 };
 
-constexpr int kGuard = 1024;   // threads of the guard: four independent loads each per round
+// Maximum of nb non-negative partials by one block of kGuard threads: up
+// to kFoldLoads loads per thread go out together (one memory round trip for
+// grids of up to kGuard x kFoldLoads blocks), then wavefront and block
+// maxima.  Every thread calls it; the result is valid in thread 0.
+constexpr int kGuard = 1024;
+constexpr int kFoldLoads = 16;
+__device__ __forceinline__ double fold_max(const double *__restrict__ partial, int nb) {
+  __shared__ double lds[kGuard / 64];
+  double m[kFoldLoads];
+#pragma unroll
+  for (int u = 0; u < kFoldLoads; u++) {
+    const int b = u * kGuard + (int)threadIdx.x;
+    m[u] = b < nb ? partial[b] : 0.0;
+  }
+  for (int b = kFoldLoads * kGuard + (int)threadIdx.x; b < nb; b += kGuard) m[0] = fmax(m[0], partial[b]);
+#pragma unroll
+  for (int w = kFoldLoads / 2; w > 0; w >>= 1)
+#pragma unroll
+    for (int u = 0; u < w; u++) m[u] = fmax(m[u], m[u + w]);
+  const double v = wave_max(m[0]);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kGuard / 64; w++) r = fmax(r, lds[w]);
+  return r;
+}
+
 __device__ __forceinline__ void step_guard(const double *__restrict__ partial, int nb,
                                            StepCtl *__restrict__ ctl) {
-  __shared__ double lds[kGuard];
   if (ctl->halt) return;
-  double m[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int b = threadIdx.x; b < nb; b += 4 * kGuard) {
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-      if (b + u * kGuard < nb) m[u] = fmax(m[u], partial[b + u * kGuard]);
-  }
-  lds[threadIdx.x] = fmax(fmax(m[0], m[1]), fmax(m[2], m[3]));
-  __syncthreads();
-  for (int off = kGuard / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + off]);
-    __syncthreads();
-  }
+  const double fold = fold_max(partial, nb);
   if (threadIdx.x == 0) {
-    const double v2 = lds[0];
+    const double v2 = fold;
     const int k = ctl->steps_done;
     ctl->ring[k] = v2;
     ctl->steps_done = k + 1;
@@ -168,6 +183,11 @@ __device__ __forceinline__ void step_guard(const double *__restrict__ partial, i
 // the threshold.
 __device__ __forceinline__ bool check_queued(const StepCtl *ctl) { return ctl->check && !ctl->halt; }
 
+// candidate list the device check can hold (CheckListLength of the
+// reference configuration, simulationinfo.cc:29, is 500); a larger one
+// keeps the check on the host
+constexpr int kMaxCheck = 512;
+
 __device__ __forceinline__ double4 check_pos(const double4 *pos, const double4 *mid, int n, int i) {
   return i < n ? pos[i] : mid[i];
 }
@@ -185,20 +205,13 @@ __global__ void k_chk_disp(const StepCtl *__restrict__ ctl, const double4 *__res
   block_max_to(d2, partial);
 }
 
-__global__ void k_chk_fold(StepCtl *__restrict__ ctl, const double *__restrict__ partial, int nb) {
+__global__ void __launch_bounds__(kGuard) k_chk_fold(StepCtl *__restrict__ ctl,
+                                                     const double *__restrict__ partial, int nb) {
 #pragma clang fp contract(off)
   if (!check_queued(ctl)) return;
-  __shared__ double lds[kBlock];
-  double m = 0.0;
-  for (int b = threadIdx.x; b < nb; b += kBlock) m = fmax(m, partial[b]);
-  lds[threadIdx.x] = m;
-  __syncthreads();
-  for (int off = kBlock / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + off]);
-    __syncthreads();
-  }
+  const double dmax2 = fold_max(partial, nb);
   if (threadIdx.x == 0) {
-    const double c_distance = ctl->buffer - __dsqrt_rn(lds[0]);
+    const double c_distance = ctl->buffer - __dsqrt_rn(dmax2);
     ctl->cd2 = c_distance * c_distance;
     ctl->cand = 0;
   }
@@ -220,20 +233,25 @@ __global__ void __launch_bounds__(kGuard) k_chk_decide(StepCtl *__restrict__ ctl
                                                        const double4 *__restrict__ ref, int n,
                                                        double SL2, double CL2) {
   __shared__ int hit;
+  __shared__ double4 cref[kMaxCheck], cnow[kMaxCheck];
   if (!check_queued(ctl)) return;
   const int l = ctl->cand;
   int expired = 0;
   if (l >= cap) {
     expired = 1;
   } else if (l > 1) {
+    // the candidates' places at the build and now, staged once
+    for (int a = threadIdx.x; a < l; a += kGuard) {
+      cref[a] = ref[cand[a]];
+      cnow[a] = check_pos(pos, mid, n, cand[a]);
+    }
     if (threadIdx.x == 0) hit = 0;
     __syncthreads();
     for (int t = threadIdx.x; t < l * l; t += kGuard) {
       const int a = t / l, b = t % l;
       if (b <= a) continue;
-      const int i1 = cand[a], i2 = cand[b];
-      if (disp2(ref[i1], ref[i2]) < SL2) continue;
-      if (disp2(check_pos(pos, mid, n, i1), check_pos(pos, mid, n, i2)) < CL2) hit = 1;
+      if (disp2(cref[a], cref[b]) < SL2) continue;
+      if (disp2(cnow[a], cnow[b]) < CL2) hit = 1;
     }
     __syncthreads();
     expired = hit;
@@ -1830,7 +1848,9 @@ int run_batch(mdx_system *s, int want, int first_checked, int *ran) {
   h->halt = 0;
   h->steps_done = 0;
   h->too_fast = 0;
-  h->first_checked = std::max(1, first_checked);
+  // steps from first_checked on check displacements on the device (not at
+  // all when the candidate list is longer than the device check holds)
+  h->first_checked = Nl <= kMaxCheck ? std::max(1, first_checked) : kRing;
   h->check = 0;
   h->cand = 0;
   h->expired = 0;
@@ -1850,7 +1870,7 @@ int run_batch(mdx_system *s, int want, int first_checked, int *ran) {
       const double4 *prev = ((b - 1) & 1) ? s->pos_mid : s->pos_alt;
       k_chk_disp<<<nblk(tn), kBlock, 0, s->stream>>>(s->dctl, s->pos, prev, s->ref_pos, n, tn,
                                                       s->disp, s->pmax);
-      k_chk_fold<<<1, kBlock, 0, s->stream>>>(s->dctl, s->pmax, nblk(tn));
+      k_chk_fold<<<1, kGuard, 0, s->stream>>>(s->dctl, s->pmax, nblk(tn));
       k_chk_cand<<<nblk(tn), kBlock, 0, s->stream>>>(s->dctl, s->disp, tn, Nl, s->cand);
       k_chk_decide<<<1, kGuard, 0, s->stream>>>(s->dctl, s->cand, Nl, s->pos, prev, s->ref_pos, n,
                                                 s->geom.SL2, s->geom.CL2);
