@@ -29,6 +29,14 @@ Two more objects ride on the same line:
   `heatbath` the N=2.05M workload stepped under the Nose-Hoover (target
              1.0, tau 0.1) and Langevin (target 1.0, gamma 0.1) heat baths,
              200 steps each after 20 warm-up steps: atom-steps/s.
+  `benchmark_mode`  the reference's own Benchmark mode (benchmark.cc:12-52)
+             at the configuration of its published 64-unit runs: the initial
+             state of 8 MPI processes x 8 OpenMP threads (N = 4,152,384,
+             L = 180, density 0.712, InitialVelocity 0.9) on this one GPU,
+             150 thermalizing steps, then 1000 timed steps observed every
+             100: MUPS = TotalLoop N / seconds / 1e6 beside the MUPS the
+             reference publishes for that run (CPU: 4 nodes, 64 cores;
+             hybrid: 8 Tesla K40t), and the first and last observations.
 """
 from __future__ import annotations
 
@@ -204,6 +212,30 @@ def heatbath(state, steps: int = 200, warmup: int = 20):
     return out
 
 
+# MUPS the reference publishes for its 64-unit Benchmark run (N = 4,152,384):
+# profile/cpu_profile/mdunit64/mdacp-stdout:36 and
+# profile/gpu_profile/mdunit64/mdacp-stdout:74
+PUBLISHED_MUPS_CPU = 78.0973
+PUBLISHED_MUPS_HYBRID = 138.482
+
+
+def benchmark_mode():
+    import torch
+    from mdacp_amd import benchmark as B
+    from mdacp_amd import workloads as W
+
+    state = W.benchmark_state(8, 8, 45.0, 0.712, 0.9)
+    r = B.run(state, sync=torch.cuda.synchronize)
+    return {"config": "Benchmark mode, 8 MPI x 8 OpenMP units of the reference on one GPU, "
+                      "UnitLength 45, Density 0.712, InitialVelocity 0.9, 150 + 1000 steps",
+            "particles": r["n"], "seconds": r["seconds"], "mups": r["mups"],
+            "published_mups_cpu_64_cores": PUBLISHED_MUPS_CPU,
+            "published_mups_hybrid_8_k40t": PUBLISHED_MUPS_HYBRID,
+            "over_published_cpu": r["mups"] / PUBLISHED_MUPS_CPU,
+            "over_published_hybrid": r["mups"] / PUBLISHED_MUPS_HYBRID,
+            "first_observation": list(r["rows"][0]), "last_observation": list(r["rows"][-1])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +247,7 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-heatbath", action="store_true")
+    ap.add_argument("--no-benchmark-mode", action="store_true")
     ap.add_argument("--only", choices=("batched", "adjoint"),
                     help="run one form alone and print its object (for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -290,6 +323,8 @@ def main():
         out["adjoint"] = adjoint()
     if not args.no_heatbath:
         out["heatbath"] = heatbath(state)
+    if not args.no_benchmark_mode:
+        out["benchmark_mode"] = benchmark_mode()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(state, args.cpu_steps, 2)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

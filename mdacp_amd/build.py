@@ -39,20 +39,34 @@ def hipcc() -> str:
 
 HOST_SRC = os.path.join(HERE, "host", "mdacp_gpu.cpp")
 HOST_LIB = os.path.join(HERE, "libmdacp_gpu.so")
+CONF_SRC = os.path.join(HERE, "host", "confmaker.cpp")
+CONF_OBJ = os.path.join(HERE, "host", "_build", "confmaker.o")
+BENCH_SRC = os.path.join(HERE, "host", "benchmark.cpp")
+BENCH_EXE = os.path.join(HERE, "mdx_benchmark")
 DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "shim_driver.cc")
 DRIVER = os.path.join(ROOT, "tests", "cpp", "_build", "shim_driver")
 
+# The reference's floating-point flags (its CMakeLists.txt with GCC and
+# USE_AVX2): the Benchmark-mode initial conditions contract a + b c the same
+# way, so they match a reference run bit for bit.
+REF_FP_FLAGS = ["-O3", "-mavx2", "-mfma"]
+
 
 def build_host(verbose: bool = False) -> None:
-    """The C++ mirror of the reference interface over libmdx, and the test
-    driver that runs it (g++; no device code)."""
+    """The C++ mirror of the reference interface over libmdx with the
+    Benchmark-mode initial conditions, the Benchmark driver, and the test
+    driver that runs the mirror (g++; no device code)."""
     cmds = [
-        ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", HOST_LIB, HOST_SRC,
+        ["g++", *REF_FP_FLAGS, "-std=c++17", "-fPIC", "-Wall", "-c", "-o", CONF_OBJ, CONF_SRC],
+        ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", HOST_LIB, HOST_SRC, CONF_OBJ,
          f"-L{HERE}", "-lmdx", "-Wl,-rpath,$ORIGIN"],
+        ["g++", "-O2", "-std=c++17", "-Wall", "-o", BENCH_EXE, BENCH_SRC, f"-L{HERE}",
+         "-lmdacp_gpu", "-lmdx", "-Wl,-rpath,$ORIGIN"],
         ["g++", "-O2", "-std=c++17", "-Wall", "-o", DRIVER, DRIVER_SRC, f"-L{HERE}",
          "-lmdacp_gpu", "-lmdx", "-Wl,-rpath,$ORIGIN/../../../mdacp_amd"],
     ]
     os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    os.makedirs(os.path.dirname(CONF_OBJ), exist_ok=True)
     for cmd in cmds:
         if verbose:
             print(" ".join(cmd), flush=True)

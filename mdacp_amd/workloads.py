@@ -4,6 +4,9 @@ Every state is generated here, once, and handed identically to the GPU
 library, the C oracle and the compiled reference harness, so that parity
 never depends on the reference's own ``<random>`` usage
 (``variables.cc:87-113`` uses ``std::mt19937``; ``confmaker.cc:193`` too).
+The one exception is ``benchmark_state``: the reference's Benchmark-mode
+initial conditions themselves, made by the C++ restatement in
+``host/confmaker.cpp`` with the same ``<random>`` engines.
 
 Positions follow the FCC site order of ``ConfigurationMaker::Execute``
 (``confmaker.cc:9-44``): cells iz -> iy -> ix, basis (0,0,0), (0,h,h),
@@ -14,8 +17,10 @@ removed and no rescale (mass = k_B = 1, so p = v).
 """
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import math
+import os
 
 import numpy as np
 
@@ -185,3 +190,31 @@ def write_state(path: str, s: State) -> None:
         np.array([s.dt], dtype=np.float64).tofile(f)
         np.ascontiguousarray(s.q, dtype=np.float64).tofile(f)
         np.ascontiguousarray(s.p, dtype=np.float64).tofile(f)
+
+
+def benchmark_state(procs: int = 8, threads: int = 8, unit_length: float = 45.0,
+                    density: float = 0.712, v0: float = 0.9, dt: float = 0.001) -> State:
+    """The initial state of the reference's Benchmark mode (benchmark.cc:12-19)
+    for a run of ``procs`` MPI processes x ``threads`` OpenMP threads: box
+    ``unit_length`` x the unit grid (simulationinfo.cc:35-39), each unit
+    filled by ExtractConfigurationMaker (confmaker.cc:174-255) and given
+    velocities by SetInitialVelocity(v0) (variables.cc:87-113), units in id
+    order.  Defaults: the published 64-unit run (N = 4,152,384, L = 180,
+    density 0.712, InitialVelocity 0.9).  Made by libmdacp_gpu.so (host
+    code, no GPU)."""
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmdacp_gpu.so"))
+    f = lib.mdacp_benchmark_state
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
+                  ctypes.POINTER(ctypes.c_long), ctypes.c_void_p]
+    n = ctypes.c_long()
+    L = np.zeros(3)
+    if f(procs, threads, unit_length, density, v0, None, None, 0, ctypes.byref(n), L.ctypes.data):
+        raise ValueError("benchmark_state: bad arguments (process and thread counts are powers of two)")
+    q = np.empty((n.value, 3))
+    p = np.empty((n.value, 3))
+    if f(procs, threads, unit_length, density, v0, q.ctypes.data, p.ctypes.data, n.value,
+         ctypes.byref(n), L.ctypes.data):
+        raise ValueError("benchmark_state failed")
+    return State(q, p, L, dt, name=f"benchmark_{procs}x{threads}_L{unit_length:g}_rho{density:g}")
