@@ -73,6 +73,14 @@ int mdx_set_state(mdx_system *sys, int n, const double *q3, const double *p3);
 int mdx_get_state(mdx_system *sys, double *q3, double *p3);
 /* Variables::GetParticleNumber / GetTotalParticleNumber (variables.h:25-26) */
 int mdx_particle_number(const mdx_system *sys, int *pn, int *tn);
+/* The periodic images, as the reference keeps its ghosts in Variables
+ * beyond particle_number (q, p, type, variables.h:19-28; a ghost carries
+ * its source's momentum, mdunit.cc:181-191): position at the last refresh,
+ * the source's momentum, the source's id and the image shift code
+ * (sx+1) + 3(sy+1) + 9(sz+1).  Each array holds tn - pn entries
+ * (mdx_particle_number), in the library's image order; any may be NULL.
+ * Valid once a pairlist exists. */
+int mdx_get_images(mdx_system *sys, double *q3, double *p3, int32_t *id, int32_t *code);
 
 /* MDManager::MakePairList (mdmanager.cc:388-417): wrap into the box
  * (variables.cc:64-84), build periodic images on the device (replacing

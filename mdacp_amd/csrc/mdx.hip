@@ -2437,6 +2437,47 @@ int mdx_get_state(mdx_system *s, double *q3, double *p3) {
   return launch_check(s, "get_state");
 }
 
+// images [k0, k0 + m) into host-transfer rows: positions, source momenta
+// (3 doubles each) and source ids, shift codes
+__global__ void k_image_rows(const double4 *__restrict__ pos, const double4 *__restrict__ mom,
+                             const int *__restrict__ gsrc, const int *__restrict__ gcode,
+                             const int *__restrict__ tag, int n, int k0, int m, int what,
+                             double *__restrict__ out3, int *__restrict__ outi) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const int k = k0 + j, src = gsrc[k];
+  if (what == 0 || what == 1) {
+    const double4 v = what == 0 ? pos[n + k] : mom[src];
+    out3[3 * (size_t)j + 0] = v.x;
+    out3[3 * (size_t)j + 1] = v.y;
+    out3[3 * (size_t)j + 2] = v.z;
+  } else {
+    outi[j] = what == 2 ? tag[src] : gcode[k];
+  }
+}
+
+int mdx_get_images(mdx_system *s, double *q3, double *p3, int32_t *id, int32_t *code) {
+  if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no state");
+  if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist (no images yet)");
+  // in chunks of the host-transfer buffers (cap particles)
+  for (int k0 = 0; k0 < s->g; k0 += s->cap) {
+    const int m = std::min(s->cap, s->g - k0);
+    void *dst[4] = {q3 ? q3 + 3 * (size_t)k0 : nullptr, p3 ? p3 + 3 * (size_t)k0 : nullptr,
+                    id ? id + k0 : nullptr, code ? code + k0 : nullptr};
+    for (int what = 0; what < 4; what++) {
+      if (!dst[what]) continue;
+      k_image_rows<<<nblk(m), kBlock, 0, s->stream>>>(s->pos, s->mom, s->gsrc, s->gcode, s->tag,
+                                                        s->n, k0, m, what, s->io_buf, s->i_buf);
+      if (what < 2)
+        HIPCHK(s, hipMemcpyAsync(dst[what], s->io_buf, sizeof(double) * 3 * m, hipMemcpyDeviceToHost, s->stream));
+      else
+        HIPCHK(s, hipMemcpyAsync(dst[what], s->i_buf, sizeof(int) * m, hipMemcpyDeviceToHost, s->stream));
+      HIPCHK(s, hipStreamSynchronize(s->stream));
+    }
+  }
+  return launch_check(s, "get_images");
+}
+
 int mdx_particle_number(const mdx_system *s, int *pn, int *tn) {
   if (!s) return MDX_ERR_ARG;
   if (pn) *pn = s->n;

@@ -2,7 +2,9 @@
 #include "mdacp_gpu.h"
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <random>
 
 namespace mdx {
@@ -133,6 +135,29 @@ void Variables::SyncToHost() {
     }
     type[i] = host_ids[i];
   }
+}
+
+void Variables::SaveConfiguration(std::ostream &fs) {
+  SyncToHost();
+  auto row = [&fs](const double *a, const double *b) {
+    fs << a[X];
+    fs << " " << a[Y];
+    fs << " " << a[Z];
+    fs << " " << b[X];
+    fs << " " << b[Y];
+    fs << " " << b[Z];
+    fs << "\n";
+  };
+  for (int i = 0; i < particle_number; i++) row(q[i], p[i]);
+  if (!sys) return;
+  int pn = 0, tn = 0;
+  check(sys, mdx_particle_number(sys, &pn, &tn));
+  const int g = tn - pn;
+  if (g <= 0) return;
+  std::vector<double> gq(3 * static_cast<size_t>(g));
+  if (mdx_get_images(sys, gq.data(), nullptr, nullptr, nullptr) != MDX_OK) return;  // no list yet
+  const double zero[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < g; k++) row(&gq[3 * k], zero);
 }
 
 mdx_system *Variables::Device() {
@@ -320,6 +345,40 @@ void MDUnit::Calculate() {
     ForceCalculator::UpdatePositionHalf(vars, sinfo);
   }
   vars->SimulationTime += sinfo->TimeStep;
+}
+
+void MDUnit::SaveAsCdview(std::ostream &ofs) {
+  vars->SyncToHost();
+  for (int i = 0; i < vars->GetParticleNumber(); i++) {
+    ofs << "0 ";
+    ofs << vars->type[i] << " ";
+    ofs << vars->q[i][X] << " ";
+    ofs << vars->q[i][Y] << " ";
+    ofs << vars->q[i][Z] << " ";
+    ofs << vars->p[i][X] << " ";
+    ofs << vars->p[i][Y] << " ";
+    ofs << vars->p[i][Z] << "\n";
+  }
+}
+
+void MDUnit::SaveConfiguration(int id) {
+  char filename[256];
+  std::snprintf(filename, sizeof filename, "conf%03d.cnf", id);
+  std::ofstream fs(filename);
+  vars->SaveConfiguration(fs);
+}
+
+void SaveAsCdview(MDUnit &unit, const char *filename) {
+  std::ofstream ofs(filename);
+  unit.SaveAsCdview(ofs);
+}
+
+void SaveAsCdviewSequential(MDUnit &unit) {
+  static int index = 0;
+  char filename[256];
+  std::snprintf(filename, sizeof filename, "conf%04d.cd", index);
+  index++;
+  SaveAsCdview(unit, filename);
 }
 
 double MDUnit::Temperature() {

@@ -28,6 +28,7 @@
 #ifndef MDACP_GPU_H
 #define MDACP_GPU_H
 
+#include <ostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -96,6 +97,12 @@ class Variables {
   int GetTotalParticleNumber() const;
   void AddParticle(double x[D], double v[D], int t = 0);   // variables.cc:36-47
   void SetInitialVelocity(double V0, int seed);             // variables.cc:87-113
+  // variables.cc:49-61: "q p" per line for the reals, then the ghosts
+  // (here the periodic images, mdx_get_images), default stream format.
+  // The reference's border exchange copies ghost positions only
+  // (mdunit.cc:181-208), so its file shows whatever its ghost momentum
+  // slots last held (0 in fresh storage); they are written as 0 here.
+  void SaveConfiguration(std::ostream &fs);
   void SyncToHost();
   void SyncToDevice();
   mdx_system *Device();   // uploads pending host changes first
@@ -183,6 +190,10 @@ class MDUnit {
   bool IsPairListExpired() { return plist->IsPairListExpired(vars, mesh, sinfo); }
   void Calculate();  // one step, NVE or under the heat bath of SimulationInfo
   double ObserveDouble(DoubleObserver *obs) { return obs->Observe(vars, mesh); }
+  // mdunit.cc:67-82: "0 type q p" per real, the CDView particle format
+  void SaveAsCdview(std::ostream &ofs);
+  // mdunit.cc:85-91: conf<id>.cnf (id as %03d) in the working directory
+  void SaveConfiguration(int id = 0);
   double Temperature();        // KE/N/1.5 (mdmanager.cc:531-533)
   double PotentialEnergy();    // per particle (mdmanager.cc:551-555)
   double Pressure();           // (T - vir/N) N/V (mdmanager.cc:563-570)
@@ -194,6 +205,11 @@ class MDUnit {
   PairList *plist;
   MDRect myrect;
 };
+
+// MDManager::SaveAsCdview / SaveAsCdviewSequential (mdmanager.cc:118-143):
+// the CDView file of a unit, and the numbered conf%04d.cd series
+void SaveAsCdview(MDUnit &unit, const char *filename);
+void SaveAsCdviewSequential(MDUnit &unit);
 
 }  // namespace mdx
 #endif

@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(HERE, "libmdx.so")
 
 EXPORTS = (
     "mdx_default_params", "mdx_create", "mdx_destroy", "mdx_last_error", "mdx_set_state",
-    "mdx_get_state", "mdx_particle_number", "mdx_make_pairlist", "mdx_is_pairlist_expired",
+    "mdx_get_state", "mdx_particle_number", "mdx_get_images", "mdx_make_pairlist",
+    "mdx_is_pairlist_expired",
     "mdx_update_position_half", "mdx_refresh_images", "mdx_calculate_force", "mdx_calculate",
     "mdx_observe", "mdx_forces", "mdx_pair_number", "mdx_pairs_canonical", "mdx_timing",
     "mdx_timing_reset", "mdx_kick_profile", "mdx_lj_trajectory",
@@ -85,6 +86,7 @@ def load() -> ctypes.CDLL:
         "mdx_set_state": (ctypes.c_int, [vp, ctypes.c_int, dp, dp]),
         "mdx_get_state": (ctypes.c_int, [vp, dp, dp]),
         "mdx_particle_number": (ctypes.c_int, [vp, ip, ip]),
+        "mdx_get_images": (ctypes.c_int, [vp, dp, dp, P(ctypes.c_int32), P(ctypes.c_int32)]),
         "mdx_make_pairlist": (ctypes.c_int, [vp]),
         "mdx_is_pairlist_expired": (ctypes.c_int, [vp, ip]),
         "mdx_update_position_half": (ctypes.c_int, [vp]),
@@ -193,6 +195,20 @@ class System:
         pn, tn = ctypes.c_int(), ctypes.c_int()
         self._chk(self.lib.mdx_particle_number(self.h, ctypes.byref(pn), ctypes.byref(tn)))
         return pn.value, tn.value
+
+    def images(self):
+        """(q, p, source id, shift code) of the periodic images (the
+        reference's ghosts), in the library's image order."""
+        pn, tn = self.particle_number()
+        g = tn - pn
+        q = np.empty((g, 3))
+        p = np.empty((g, 3))
+        ids = np.empty(g, dtype=np.int32)
+        code = np.empty(g, dtype=np.int32)
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        self._chk(self.lib.mdx_get_images(self.h, _dp(q), _dp(p), ids.ctypes.data_as(i32),
+                                          code.ctypes.data_as(i32)))
+        return q, p, ids, code
 
     def make_pairlist(self):
         self._chk(self.lib.mdx_make_pairlist(self.h))

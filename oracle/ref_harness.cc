@@ -22,6 +22,9 @@
 //              heat bath (ControlTemperature, simulationinfo.cc:11-23) and
 //              Variables::Zeta is recorded with the observables
 //   bench K W  W warm-up steps then K timed steps; prints atom-steps/s
+//   save K     MakePairList and K steps, then in <out_prefix> (a directory)
+//              MDManager::SaveAsCdviewSequential (conf0000.cd) and
+//              MDManager::SaveConfiguration (conf000.cnf)
 #include <mpi.h>
 #include <omp.h>
 #include <stdio.h>
@@ -351,6 +354,17 @@ int main(int argc, char **argv) {
     WriteArray(out + "_q.f64", sc.q);
     WriteArray(out + "_p.f64", sc.p);
     printf("steps %d\n", K);
+  } else if (mode == "save") {
+    const int K = argc > 4 ? atoi(argv[4]) : 0;
+    mdm->MakePairList();
+    for (int i = 0; i < K; i++) mdm->Calculate();
+    if (chdir(out.c_str()) != 0) {
+      fprintf(stderr, "cannot enter %s\n", out.c_str());
+      rc = 2;
+    } else {
+      mdm->SaveAsCdviewSequential();
+      mdm->SaveConfiguration();
+    }
   } else if (mode == "bench") {
     const int K = argc > 4 ? atoi(argv[4]) : 20;
     const int W = argc > 5 ? atoi(argv[5]) : 2;

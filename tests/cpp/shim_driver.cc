@@ -7,15 +7,27 @@
 // with T and tau the steps run under the Nose-Hoover heat bath
 // (ControlTemperature, AimedTemperature = T, HeatbathTau = tau).
 // writes <out_prefix>_q.f64 / _p.f64 (by particle id) at the end.
+//        shim_driver save <state.bin> <steps> <dir>
+// MakePairList and <steps> steps, then in <dir> the reference's output
+// files: SaveAsCdviewSequential (conf0000.cd) and SaveConfiguration
+// (conf000.cnf).
+#include <unistd.h>
+
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "../../mdacp_amd/host/mdacp_gpu.h"
 
 int main(int argc, char **argv) {
-  if (argc < 5) {
+  const bool save = argc >= 2 && std::strcmp(argv[1], "save") == 0;
+  if (save) {
+    argv++;
+    argc--;
+  }
+  if (argc < 4 || (!save && argc < 5)) {
     fprintf(stderr, "usage: %s state.bin steps every out_prefix\n", argv[0]);
     return 2;
   }
@@ -44,6 +56,13 @@ int main(int argc, char **argv) {
       unit.AddParticle(x, v, static_cast<int>(i));
     }
     unit.MakePairList();
+    if (save) {
+      for (int s = 0; s < atoi(argv[2]); s++) unit.Calculate();
+      if (chdir(argv[3]) != 0) return 2;
+      mdx::SaveAsCdviewSequential(unit);
+      unit.SaveConfiguration(0);
+      return 0;
+    }
     mdx::KineticEnergyObserver ke;
     mdx::PotentialEnergyObserver pe(&sinfo);
     mdx::VirialObserver vir;

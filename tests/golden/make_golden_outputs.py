@@ -1,0 +1,57 @@
+"""Writes tests/golden/outputs_fcc500.npz: the reference's output files for
+one seeded state, from oracle/_ref/ref_harness (``save`` mode) run as one
+domain in the build container, where the reference tree exists.
+
+For K = 0 (right after MakePairList) and K = 50 steps: the lines of
+MDManager::SaveAsCdviewSequential's conf0000.cd (CDView: "0 type q p" per
+particle, mdunit.cc:67-82; type carries the particle id) and of
+MDManager::SaveConfiguration's conf000.cnf ("q p" per particle, reals then
+ghosts, variables.cc:49-61), as written with the default stream format
+(byte strings, one per line).
+
+usage: python tests/golden/make_golden_outputs.py
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from mdacp_amd import workloads as W  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+def state():
+    return W.perturbed(W.fcc_state(5, 0.7, 1.0, 5, name="fcc500_rho0.7"), 0.25, 12)
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    s = state()
+    out = {"q0": s.q, "p0": s.p, "L": s.L, "dt": s.dt}
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "state.bin")
+        W.write_state(path, s)
+        for k in (0, 50):
+            d = os.path.join(tmp, f"k{k}")
+            os.mkdir(d)
+            subprocess.run([HARNESS, "save", path, d, str(k)], check=True,
+                           env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True)
+            for key, name in (("cd", "conf0000.cd"), ("cnf", "conf000.cnf")):
+                lines = open(os.path.join(d, name), "rb").read().splitlines()
+                out[f"{key}_k{k}"] = np.array(lines, dtype=bytes)
+    np.savez_compressed(os.path.join(HERE, "outputs_fcc500.npz"), **out)
+    print({k: v.shape for k, v in out.items() if k.startswith("c")})
+
+
+if __name__ == "__main__":
+    main()
