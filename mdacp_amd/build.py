@@ -25,6 +25,9 @@ HIP_FLAGS = [
     "-shared",
     # packed FP32 forms stay off; the library is FP64 throughout
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # a + b c still fuses into one FMA, except where a function asks for
+    # operation-by-operation rounding (#pragma clang fp contract(off))
+    "-ffp-contract=fast-honor-pragmas",
     "-Wall",
     f"-I{os.path.join(ROOT, 'include')}",
 ]

@@ -145,6 +145,29 @@ __device__ __forceinline__ double fold_max(const double *__restrict__ partial, i
   return r;
 }
 
+// Sum of nb partials in a fixed order by one block of kGuard threads (the
+// loads as fold_max); the result is valid in thread 0.
+__device__ __forceinline__ double fold_sum(const double *__restrict__ partial, int nb) {
+  __shared__ double lds[kGuard / 64];
+  double m[kFoldLoads];
+#pragma unroll
+  for (int u = 0; u < kFoldLoads; u++) {
+    const int b = u * kGuard + (int)threadIdx.x;
+    m[u] = b < nb ? partial[b] : 0.0;
+  }
+  double v = 0.0;
+#pragma unroll
+  for (int u = 0; u < kFoldLoads; u++) v += m[u];
+  for (int b = kFoldLoads * kGuard + (int)threadIdx.x; b < nb; b += kGuard) v += partial[b];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kGuard / 64; w++) r += lds[w];
+  return r;
+}
+
 __device__ __forceinline__ void step_guard(const double *__restrict__ partial, int nb,
                                            StepCtl *__restrict__ ctl) {
   if (ctl->halt) return;
@@ -810,19 +833,113 @@ __device__ __forceinline__ void lj_terms(double s, const LJ &p, double *phi, dou
 //   MODE 3  MODE 2, plus the next step's leading half drift of the reals:
 //           qn[i] = out[i] + p_i dt/2
 //   SCALED  the pair term of the differentiable form (pair_df_lj)
+// Nose-Hoover state on the device: Variables::Zeta (variables.h:29) at the
+// end of the last step, and inside a batch of steps zeta after the first
+// update of the step running (zeta_mid) with its momentum factor
+// exp(-zeta_mid dt/2) (scale)
+struct HeatDev {
+  double zeta, zeta_mid, scale;
+};
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): ten rounds of two
+// 32x32 -> 64-bit products and xors, the key bumped by the Weyl constants
+__device__ __forceinline__ uint4 philox4x32(uint4 c, uint2 k) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    const unsigned long long a = (unsigned long long)0xD2511F53u * c.x;
+    const unsigned long long b = (unsigned long long)0xCD9E8D57u * c.z;
+    c = make_uint4((unsigned)(b >> 32) ^ c.y ^ k.x, (unsigned)b, (unsigned)(a >> 32) ^ c.w ^ k.y,
+                   (unsigned)a);
+    k.x += 0x9E3779B9u;
+    k.y += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// (0, 1] from 53 bits of two words
+__device__ __forceinline__ double uniform53(unsigned a, unsigned b) {
+  const unsigned long long m = ((unsigned long long)(a >> 5) << 26) | (unsigned long long)(b >> 6);
+  return ((double)m + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+// four standard normals of (seed, id, k): two Philox blocks, counter (id,
+// k lo, k hi, 0 / 1), key (seed lo, seed hi), Box-Muller on 53-bit uniforms
+__device__ __forceinline__ void langevin_normals(unsigned long long seed, unsigned id,
+                                                 unsigned long long k, double nrm[4]) {
+  const uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
+  const uint4 w0 = philox4x32(make_uint4(id, (unsigned)k, (unsigned)(k >> 32), 0u), key);
+  const uint4 w1 = philox4x32(make_uint4(id, (unsigned)k, (unsigned)(k >> 32), 1u), key);
+  const double two_pi = 6.283185307179586;
+  {
+    const double r = sqrt(-2.0 * log(uniform53(w0.x, w0.y)));
+    const double a = two_pi * uniform53(w0.z, w0.w);
+    nrm[0] = r * cos(a);
+    nrm[1] = r * sin(a);
+  }
+  {
+    const double r = sqrt(-2.0 * log(uniform53(w1.x, w1.y)));
+    const double a = two_pi * uniform53(w1.z, w1.w);
+    nrm[2] = r * cos(a);
+    nrm[3] = r * sin(a);
+  }
+}
+
+// p += (-gamma p + D n) dt on all four lanes (ForceCalculator::Langevin,
+// fcalculator.cc:1001-1017)
+__device__ __forceinline__ void langevin_update(double4 &p, const double nrm[4], double gamma,
+                                                double D, double dt) {
+  p.x += (-gamma * p.x + D * nrm[0]) * dt;
+  p.y += (-gamma * p.y + D * nrm[1]) * dt;
+  p.z += (-gamma * p.z + D * nrm[2]) * dt;
+  p.w += (-gamma * p.w + D * nrm[3]) * dt;
+}
+
+// The heat bath a batched kick applies around the force (MDManager::
+// CalculateNoseHoover / CalculateLangevin, mdmanager.cc:255-315):
+//   Nose-Hoover: p <- s (s p + F dt), s = hd->scale, all four lanes, and
+//     the block's kinetic energy of the new momenta into ke_partial;
+//   Langevin: p <- p + F dt, then the noise update of kick number k, and
+//     the reals' end-of-step positions are the mid-step ones (no trailing
+//     half drift).
+struct HeatKick {
+  int type = 0;   // mdx_heatbath type
+  const HeatDev *hd = nullptr;
+  double *ke_partial = nullptr;
+  const int *tag = nullptr;
+  unsigned long long seed = 0, k = 0;
+  double gamma = 0.0, D = 0.0;
+};
+
+// Fixed-order sum over the block (blockDim.x <= kTile threads, a power of
+// two) to partial[blockIdx.x]; every thread calls it
+__device__ __forceinline__ void tile_sum_to(double v, double *__restrict__ partial) {
+  __shared__ double lds[kTile];
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
+}
+
 template <int MODE, bool SCALED = false>
 __global__ void __launch_bounds__(kTile) k_kick_tiles(
     const double4 *__restrict__ pos, double4 *__restrict__ mom, const TileDesc *__restrict__ tiles,
     int ntiles, Geom g, Cells c, int n, int kcap, const unsigned short *__restrict__ list,
     const int *__restrict__ nn, const int *__restrict__ nb, PairK pk, double dt,
     double4 *__restrict__ out, double *__restrict__ partial, LJ lj = LJ{},
-    const int *__restrict__ halt = nullptr, double4 *__restrict__ qn = nullptr) {
+    const int *__restrict__ halt = nullptr, double4 *__restrict__ qn = nullptr,
+    HeatKick hk = HeatKick{}) {
   extern __shared__ double stage_mem[];
   __shared__ TileSmem sm;
   if (halt && *halt) return;
+  const bool nh = MODE == 3 && hk.type == MDX_HEATBATH_NOSE_HOOVER;
+  const bool lgv = MODE == 3 && hk.type == MDX_HEATBATH_LANGEVIN;
   const int tile = tile_of_block(blockIdx.x, ntiles);
   if (tile >= ntiles) {
     if (MODE != 1 && threadIdx.x == 0) partial[blockIdx.x] = 0.0;
+    if (nh && threadIdx.x == 0) hk.ke_partial[blockIdx.x] = 0.0;
     return;
   }
   const TileDesc t = tiles[tile];
@@ -865,7 +982,7 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
       }
     });
   }
-  double v2 = 0.0;
+  double v2 = 0.0, ke = 0.0;
   if (act) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll
@@ -878,34 +995,86 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
       out[i] = make_double4(fx, fy, fz, 0.0);
     } else {
       double4 p = pm;
-      p.x += fx;
-      p.y += fy;
-      p.z += fz;
+      if (nh) {
+        // HeatbathMomenta, kick, HeatbathMomenta (mdmanager.cc:255-287)
+        const double sc = hk.hd->scale;
+        p.x *= sc;
+        p.y *= sc;
+        p.z *= sc;
+        p.w *= sc;
+        p.x += fx;
+        p.y += fy;
+        p.z += fz;
+        p.x *= sc;
+        p.y *= sc;
+        p.z *= sc;
+        p.w *= sc;
+        ke = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z + 0.5 * p.w * p.w;
+      } else {
+        p.x += fx;
+        p.y += fy;
+        p.z += fz;
+        if (lgv) {
+          double nrm[4];
+          langevin_normals(hk.seed, (unsigned)hk.tag[i], hk.k, nrm);
+          langevin_update(p, nrm, hk.gamma, hk.D, dt);
+        }
+      }
       mom[i] = p;
       v2 = speed2(p);
       if (MODE == 2) out[i] = drift_half(qi, p, dt * 0.5);
       if (MODE == 3) {
-        const double4 q = drift_half(qi, p, dt * 0.5);
+        // Langevin steps end at the mid-step positions (no trailing drift)
+        const double4 q = lgv ? qi : drift_half(qi, p, dt * 0.5);
         out[i] = q;
         qn[i] = drift_half(q, p, dt * 0.5);
       }
     }
   }
   if (MODE != 1) block_max_to(v2, partial);
+  if (nh) tile_sum_to(ke, hk.ke_partial);
 }
 
 // The guard ending a step (block 0) and the image refresh of the next
 // step's mid-step buffer (the other blocks, kGuard images each) in one
 // launch.  The refresh blocks may read halt before or after block 0 raises
 // it: a refresh of a step that will not run is never read.
+// Nose-Hoover zeta around a batched step (CalculateNoseHoover,
+// mdmanager.cc:255-287): the step's second update from the temperature of
+// its new momenta, HeatbathZeta (fcalculator.cc:979-985) with the
+// temperature of MDManager::Temperature (mdmanager.cc:531-548), gives
+// zeta at the end of the step; the next step's first update sees the same
+// momenta, so it adds the same increment.
+struct HeatGuard {
+  int type = 0;
+  const double *ke_partial = nullptr;
+  int n = 0;
+  double T0 = 0.0, tau = 0.0, dt = 0.0;
+  HeatDev *hd = nullptr;
+};
+
 __global__ void __launch_bounds__(kGuard) k_guard_refresh(const double *__restrict__ partial, int nb,
                                                           StepCtl *__restrict__ ctl,
                                                           double4 *__restrict__ mid,
                                                           const int *__restrict__ gsrc,
                                                           const int *__restrict__ gcode, int n,
-                                                          int g, Geom geom) {
+                                                          int g, Geom geom,
+                                                          HeatGuard hg = HeatGuard{}) {
   if (blockIdx.x == 0) {
+    const bool ran = !ctl->halt;   // read by every thread before the guard writes it
     step_guard(partial, nb, ctl);
+    if (ran && hg.type == MDX_HEATBATH_NOSE_HOOVER) {
+      const double ke = fold_sum(hg.ke_partial, nb);
+      if (threadIdx.x == 0) {
+        const double t = ke / (double)hg.n / 1.5;
+        const double t1 = (t - hg.T0) / (hg.tau * hg.tau);
+        const double zeta = hg.hd->zeta_mid + t1 * (hg.dt * 0.5);
+        const double next = zeta + t1 * (hg.dt * 0.5);
+        hg.hd->zeta = zeta;
+        hg.hd->zeta_mid = next;
+        hg.hd->scale = exp(-(hg.dt * 0.5) * next);
+      }
+    }
     return;
   }
   if (__hip_atomic_load(&ctl->halt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
@@ -936,10 +1105,6 @@ __global__ void k_vmax(const double4 *__restrict__ mom, int n, double *__restric
 }
 
 // ------------------------------------------------------------ heat baths
-// Nose-Hoover state on the device: Variables::Zeta (variables.h:29)
-struct HeatDev {
-  double zeta;
-};
 
 // Fixed-order block sum (kBlock threads) to partial[blockIdx.x]
 __device__ __forceinline__ void block_sum_to(double v, double *__restrict__ partial) {
@@ -986,6 +1151,28 @@ __global__ void k_nh_zeta(const double *__restrict__ partial, int nb, int n, dou
   }
 }
 
+// The first zeta update of a batch's first step (as k_nh_zeta) into
+// zeta_mid, with its momentum factor; zeta itself stays the end-of-step value
+__global__ void k_nh_begin(const double *__restrict__ partial, int nb, int n, double T0,
+                           double tau, double dt, HeatDev *__restrict__ hd) {
+  __shared__ double lds[kBlock];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) acc += partial[b];
+  lds[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double t = lds[0] / (double)n / 1.5;
+    const double t1 = (t - T0) / (tau * tau);
+    const double zm = hd->zeta + t1 * (dt * 0.5);
+    hd->zeta_mid = zm;
+    hd->scale = exp(-(dt * 0.5) * zm);
+  }
+}
+
 // ForceCalculator::HeatbathMomenta (fcalculator.cc:987-999), all four lanes
 __global__ void k_nh_scale(double4 *__restrict__ mom, int n, double dt,
                            const HeatDev *__restrict__ hd) {
@@ -1000,27 +1187,6 @@ __global__ void k_nh_scale(double4 *__restrict__ mom, int n, double dt,
   mom[i] = p;
 }
 
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): ten rounds of two
-// 32x32 -> 64-bit products and xors, the key bumped by the Weyl constants
-__device__ __forceinline__ uint4 philox4x32(uint4 c, uint2 k) {
-#pragma unroll
-  for (int r = 0; r < 10; r++) {
-    const unsigned long long a = (unsigned long long)0xD2511F53u * c.x;
-    const unsigned long long b = (unsigned long long)0xCD9E8D57u * c.z;
-    c = make_uint4((unsigned)(b >> 32) ^ c.y ^ k.x, (unsigned)b, (unsigned)(a >> 32) ^ c.w ^ k.y,
-                   (unsigned)a);
-    k.x += 0x9E3779B9u;
-    k.y += 0xBB67AE85u;
-  }
-  return c;
-}
-
-// (0, 1] from 53 bits of two words
-__device__ __forceinline__ double uniform53(unsigned a, unsigned b) {
-  const unsigned long long m = ((unsigned long long)(a >> 5) << 26) | (unsigned long long)(b >> 6);
-  return ((double)m + 1.0) * (1.0 / 9007199254740992.0);
-}
-
 // ForceCalculator::Langevin (fcalculator.cc:1001-1017) for the k-th kick:
 // p += (-gamma p + r) dt on all four lanes, r = hb_D n with n four standard
 // normals of (seed, id, k): two Philox blocks, counter (id, k lo, k hi,
@@ -1030,29 +1196,10 @@ __global__ void k_langevin(double4 *__restrict__ mom, const int *__restrict__ ta
                            double dt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
-  const unsigned id = (unsigned)tag[i];
-  const uint4 w0 = philox4x32(make_uint4(id, (unsigned)k, (unsigned)(k >> 32), 0u), key);
-  const uint4 w1 = philox4x32(make_uint4(id, (unsigned)k, (unsigned)(k >> 32), 1u), key);
-  const double two_pi = 6.283185307179586;
   double nrm[4];
-  {
-    const double r = sqrt(-2.0 * log(uniform53(w0.x, w0.y)));
-    const double a = two_pi * uniform53(w0.z, w0.w);
-    nrm[0] = r * cos(a);
-    nrm[1] = r * sin(a);
-  }
-  {
-    const double r = sqrt(-2.0 * log(uniform53(w1.x, w1.y)));
-    const double a = two_pi * uniform53(w1.z, w1.w);
-    nrm[2] = r * cos(a);
-    nrm[3] = r * sin(a);
-  }
+  langevin_normals(seed, (unsigned)tag[i], k, nrm);
   double4 p = mom[i];
-  p.x += (-gamma * p.x + hb_D * nrm[0]) * dt;
-  p.y += (-gamma * p.y + hb_D * nrm[1]) * dt;
-  p.z += (-gamma * p.z + hb_D * nrm[2]) * dt;
-  p.w += (-gamma * p.w + hb_D * nrm[3]) * dt;
+  langevin_update(p, nrm, gamma, hb_D, dt);
   mom[i] = p;
 }
 
@@ -1857,10 +2004,38 @@ int run_batch(mdx_system *s, int want, int first_checked, int *ran) {
   h->checks_passed = 0;
   h->cd2 = 0.0;
   HIPCHK(s, hipMemcpyAsync(s->dctl, h, offsetof(StepCtl, ring), hipMemcpyHostToDevice, s->stream));
+  // the heat bath the kicks apply (MDManager::CalculateNoseHoover /
+  // CalculateLangevin, mdmanager.cc:255-315)
+  HeatKick hk;
+  HeatGuard hgd;
+  hk.type = hgd.type = s->hb.type;
+  if (s->hb.type == MDX_HEATBATH_NOSE_HOOVER) {
+    int rc = ensure_partials(s, std::max(grid, nblk(n)));
+    if (rc) return rc;
+    hk.hd = s->dhb;
+    hk.ke_partial = s->partial;
+    hgd.ke_partial = s->partial;
+    hgd.n = n;
+    hgd.T0 = s->hb.aimed_temperature;
+    hgd.tau = s->hb.tau;
+    hgd.dt = dt;
+    hgd.hd = s->dhb;
+  } else if (s->hb.type == MDX_HEATBATH_LANGEVIN) {
+    hk.tag = s->tag;
+    hk.seed = s->hb.seed;
+    hk.gamma = s->hb.gamma;
+    hk.D = std::sqrt(2.0 * s->hb.gamma * s->hb.aimed_temperature / dt);
+  }
   ev_record(s, 4);
   k_drift_refresh<<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->gsrc, s->gcode, n, g,
                                                           dt * 0.5, s->geom.L[0], s->geom.L[1],
                                                           s->geom.L[2], s->pos_alt, &s->dctl->halt);
+  if (s->hb.type == MDX_HEATBATH_NOSE_HOOVER) {
+    // the first step's first zeta update, from the momenta as they stand
+    k_ke_partials<<<nblk(n), kBlock, 0, s->stream>>>(s->mom, n, s->partial);
+    k_nh_begin<<<1, kBlock, 0, s->stream>>>(s->partial, nblk(n), n, s->hb.aimed_temperature,
+                                            s->hb.tau, dt, s->dhb);
+  }
   for (int b = 0; b < B; b++) {
     const double4 *cur = (b & 1) ? s->pos_mid : s->pos_alt;
     double4 *nxt = (b & 1) ? s->pos_alt : s->pos_mid;
@@ -1876,13 +2051,14 @@ int run_batch(mdx_system *s, int want, int first_checked, int *ran) {
                                                 s->geom.SL2, s->geom.CL2);
     }
     if (s->timing) (void)hipEventRecord(s->evk[2 * b], s->stream);
+    hk.k = s->langevin_k + (unsigned long long)b;   // Langevin kicks since mdx_set_state
     k_kick_tiles<3><<<grid, kTile, plane_bytes(s), s->stream>>>(
         cur, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn,
-        s->nb, make_pairk(s->C2, dt), dt, s->pos, s->pmax, LJ{}, &s->dctl->halt, nxt);
+        s->nb, make_pairk(s->C2, dt), dt, s->pos, s->pmax, LJ{}, &s->dctl->halt, nxt, hk);
     if (s->timing) (void)hipEventRecord(s->evk[2 * b + 1], s->stream);
     const int refresh_blocks = (g > 0 && b + 1 < B) ? (g + kGuard - 1) / kGuard : 0;
-    k_guard_refresh<<<1 + refresh_blocks, kGuard, 0, s->stream>>>(s->pmax, grid, s->dctl, nxt,
-                                                                   s->gsrc, s->gcode, n, g, s->geom);
+    k_guard_refresh<<<1 + refresh_blocks, kGuard, 0, s->stream>>>(
+        s->pmax, grid, s->dctl, nxt, s->gsrc, s->gcode, n, g, s->geom, hgd);
   }
   if (g > 0)
     k_images_from_mid<<<nblk(g), kBlock, 0, s->stream>>>(s->pos, s->pos_alt, s->pos_mid, n, g,
@@ -1908,6 +2084,7 @@ int run_batch(mdx_system *s, int want, int first_checked, int *ran) {
     s->t_other += all - k;
   }
   s->n_kick += *ran;
+  if (s->hb.type == MDX_HEATBATH_LANGEVIN) s->langevin_k += (unsigned long long)*ran;
   return MDX_OK;
 }
 
@@ -1950,25 +2127,6 @@ int langevin(mdx_system *s) {
   return launch_check(s, "langevin");
 }
 
-// One step of MDManager::Calculate under the heat bath, after its expiry
-// check (mdmanager.cc:154-186): half drift, image refresh, then
-//   Nose-Hoover (CalculateNoseHoover, mdmanager.cc:255-287): zeta from the
-//     temperature, momenta scaled, kick, scaled, zeta again, half drift;
-//   Langevin (CalculateLangevin, mdmanager.cc:289-315): kick, noise, and no
-//     trailing half drift.
-// Zeta stays on the device; nothing waits for the host.
-int heatbath_step(mdx_system *s) {
-  int rc;
-  if ((rc = drift(s)) || (rc = refresh(s))) return rc;
-  if (s->hb.type == MDX_HEATBATH_NOSE_HOOVER) {
-    if ((rc = nh_zeta(s)) || (rc = nh_scale(s)) || (rc = kick(s)) || (rc = nh_scale(s)) ||
-        (rc = nh_zeta(s)) || (rc = drift(s)))
-      return rc;
-  } else {
-    if ((rc = kick(s)) || (rc = langevin(s))) return rc;
-  }
-  return MDX_OK;
-}
 
 
 // PairList::IsPairListExpired (pairlist.cc:26-45)
@@ -2574,12 +2732,6 @@ int mdx_calculate(mdx_system *s, int nsteps, int *rebuilds) {
     if (exp) {
       if ((rc = build_pairlist(s))) return rc;
       nreb++;
-    }
-    if (s->hb.type != MDX_HEATBATH_NONE) {
-      if ((rc = heatbath_step(s))) return rc;
-      done++;
-      s->n_steps++;
-      continue;
     }
     // queue as many steps as the budget is expected to last at the last
     // |p|max, then a few steps that check displacements on the device (as

@@ -12,9 +12,16 @@ constexpr int kBlock = 256;
 
 constexpr int kCodeNone = 13;                       // image code of "no shift"
 
+// The helpers below round operation by operation, as written: contraction
+// into FMA is switched off in each (the library is built to honour the
+// pragma; otherwise the compiler fuses a + b c where it sees fit, which can
+// differ from kernel to kernel).  Every kernel that uses them then gets
+// identical bits.
+
 // |p|^2 rounded as written (pairlist.cc:52), identical in every kernel
 __device__ __forceinline__ double speed2(const double4 &p) {
-  return __dadd_rn(__dadd_rn(__dmul_rn(p.x, p.x), __dmul_rn(p.y, p.y)), __dmul_rn(p.z, p.z));
+#pragma clang fp contract(off)
+  return p.x * p.x + p.y * p.y + p.z * p.z;
 }
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -323,9 +330,10 @@ __device__ __forceinline__ void kick_term(double xj, double yj, double zj, doubl
 
 // q + p dt/2 (ForceCalculator::UpdatePositionHalf, fcalculator.cc:26-28)
 __device__ __forceinline__ double4 drift_half(double4 q, const double4 &p, double dt2) {
-  q.x = __dadd_rn(q.x, __dmul_rn(p.x, dt2));
-  q.y = __dadd_rn(q.y, __dmul_rn(p.y, dt2));
-  q.z = __dadd_rn(q.z, __dmul_rn(p.z, dt2));
+#pragma clang fp contract(off)
+  q.x = q.x + p.x * dt2;
+  q.y = q.y + p.y * dt2;
+  q.z = q.z + p.z * dt2;
   return q;
 }
 
@@ -341,8 +349,9 @@ __device__ __forceinline__ double4 image_of(double4 q, int code, const double *L
 // squared displacement old - new (pairlist.cc:73-80), also the pair
 // distances of the candidate test (pairlist.cc:98-112)
 __device__ __forceinline__ double disp2(const double4 &a, const double4 &b) {
-  const double dx = __dadd_rn(a.x, -b.x), dy = __dadd_rn(a.y, -b.y), dz = __dadd_rn(a.z, -b.z);
-  return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+#pragma clang fp contract(off)
+  const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  return dx * dx + dy * dy + dz * dz;
 }
 
 // observer terms of one partner within the cutoff: potential energy
