@@ -265,3 +265,71 @@ def test_energy_error_second_order_in_dt():
         errs.append(worst)
     assert errs[1] < 3e-5
     assert 3.5 < errs[0] / errs[1] < 4.5
+
+
+def _edge_state(case):
+    rng = np.random.default_rng(91)
+    if case == "single":
+        q, L = np.array([[1.0, 2.0, 3.0]]), 10.0
+    elif case == "pair_across_boundary":
+        q, L = np.array([[0.3, 5.0, 5.0], [9.2, 5.0, 5.0]]), 10.0       # 1.1 apart through x = 0
+    elif case == "pair_at_cutoff":
+        q, L = np.array([[2.0, 5.0, 5.0], [5.0, 5.0, 5.0]]), 10.0       # exactly rc apart
+    elif case == "pair_at_search_length":
+        q, L = np.array([[2.0, 5.0, 5.0], [5.3, 5.0, 5.0]]), 10.0       # 3.3 apart (rounded)
+    elif case == "sub_tile":
+        # 37 of the 64 sites of a spacing-2 cubic grid, jittered
+        g = np.stack(np.meshgrid(*[np.arange(4)] * 3, indexing="ij"), -1).reshape(-1, 3) * 2.0
+        q, L = g[rng.permutation(64)[:37]] + rng.uniform(-0.1, 0.1, size=(37, 3)) + 0.5, 8.0
+    else:   # narrow_box: just wider than two search lengths, FCC 3^3 jittered
+        s0 = W.perturbed(W.fcc_state(3, 4.0 / (6.7 / 3) ** 3, 1.0, 92), 0.1, 93)
+        q, L = s0.q, 6.7
+    p = rng.normal(0, 0.5, size=q.shape)
+    p -= p.mean(axis=0)
+    return W.State(q, p, np.array([L, L, L]), 0.001)
+
+
+@pytest.mark.parametrize("case", ["single", "pair_across_boundary", "pair_at_cutoff",
+                                  "pair_at_search_length", "sub_tile", "narrow_box"])
+def test_edge_cases_vs_oracle(case):
+    """Tiny and boundary inputs: one particle, a pair through the periodic
+    boundary, pairs exactly at rc and at the search length, fewer particles
+    than a tile, a box just wider than two search lengths.  Pair set
+    bit-exact, forces and observables to the bars, and 200 steps along the
+    oracle's trajectory."""
+    s = _edge_state(case)
+    o = O.Oracle(s)
+    o.make_pairlist()
+    with System.from_state(s) as g:
+        g.make_pairlist()
+        np.testing.assert_array_equal(g.pairs(), o.pairs())
+        fo = o.forces()
+        fg = g.forces()
+        if np.abs(fo).max() > 1e-6:
+            assert force_err(fg, fo) < FORCE_TOL
+        else:
+            # no force, or the rounding residue of the shifted force at rc
+            # (zero in exact arithmetic): absolute bar
+            assert np.abs(fg - fo).max() < 1e-12
+        np.testing.assert_allclose(g.observe(), o.observe(), rtol=OBS_TOL, atol=1e-14)
+        assert g.calculate(200) == o.calculate(200)
+        qg, pg = g.state()
+        qo, po = o.state()
+        d = qg - qo
+        d -= s.L * np.round(d / s.L)
+        assert np.abs(d).max() < 1e-10 and np.abs(pg - po).max() < 1e-9
+
+
+def test_rejected_inputs():
+    """No particles, and a box narrower than two search lengths, are refused
+    with an error (the reference needs a domain of at least that width for
+    its images; here the status names it)."""
+    with pytest.raises(MdxError):
+        with System(np.array([10.0, 10.0, 10.0])) as g:
+            g.set_state(np.zeros((0, 3)), np.zeros((0, 3)))
+    s = _edge_state("narrow_box")
+    s.L = np.array([6.5, 6.5, 6.5])
+    s.q = np.mod(s.q, 6.5)
+    with pytest.raises(MdxError):
+        with System.from_state(s) as g:
+            g.make_pairlist()
