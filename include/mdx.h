@@ -82,6 +82,12 @@ int mdx_particle_number(const mdx_system *sys, int *pn, int *tn);
  * Valid once a pairlist exists. */
 int mdx_get_images(mdx_system *sys, double *q3, double *p3, int32_t *id, int32_t *code);
 
+/* Waits until the work queued on sys has finished (the reference's GPU
+ * StopWatch waits on its CUDA event before it reads the time,
+ * stopwatch.h:48-82).  Every other call here that returns data waits by
+ * itself; this one is for timing. */
+int mdx_synchronize(mdx_system *sys);
+
 /* MDManager::MakePairList (mdmanager.cc:388-417): wrap into the box
  * (variables.cc:64-84), build periodic images on the device (replacing
  * mdmanager.cc:395-401), bin by counting sort (meshlist.cc:176-228), build

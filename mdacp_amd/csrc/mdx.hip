@@ -2636,6 +2636,12 @@ int mdx_get_images(mdx_system *s, double *q3, double *p3, int32_t *id, int32_t *
   return launch_check(s, "get_images");
 }
 
+int mdx_synchronize(mdx_system *s) {
+  if (!s) return MDX_ERR_ARG;
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return launch_check(s, "synchronize");
+}
+
 int mdx_particle_number(const mdx_system *s, int *pn, int *tn) {
   if (!s) return MDX_ERR_ARG;
   if (pn) *pn = s->n;

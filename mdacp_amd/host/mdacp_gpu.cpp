@@ -1,6 +1,7 @@
 // mdacp_gpu.cpp — see mdacp_gpu.h.
 #include "mdacp_gpu.h"
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -310,6 +311,12 @@ MDUnit::MDUnit(SimulationInfo *si) : sinfo(si) {
 }
 
 MDUnit::~MDUnit() {
+  // the timers write their files (rank 0), as the reference's static ones
+  // do at exit
+  delete swAll;
+  delete swForce;
+  delete swComm;
+  delete swPair;
   delete mesh;
   delete plist;
   delete vars;
@@ -326,10 +333,54 @@ void MDUnit::MakePairList() {   // MDUnit::MakePairList (mdunit.cc:211-215)
 
 // MDManager::Calculate (mdmanager.cc:154-186) with CalculateNoseHoover
 // (:255-287) and CalculateLangevin (:289-315) for one domain
+// ------------------------------------------------------------ StopWatch
+namespace {
+double WallSeconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+void StopWatch::Start() {
+  if (sys) check(sys, mdx_synchronize(sys));
+  start = WallSeconds();
+}
+
+void StopWatch::Stop() {
+  if (sys) check(sys, mdx_synchronize(sys));
+  data.push_back(WallSeconds() - start);
+}
+
+void StopWatch::SaveToFile() const {
+  char filename[256];
+  std::snprintf(filename, sizeof filename, "%s%05d.dat", base.c_str(), id);
+  std::ofstream ofs(filename);
+  for (double d : data) ofs << d << "\n";
+}
+
+void MDUnit::EnableStopWatches(int rank) {
+  if (swAll) return;
+  swAll = new StopWatch(rank, "all");
+  swForce = new StopWatch(rank, "force");
+  swComm = new StopWatch(rank, "comm");
+  swPair = new StopWatch(rank, "pair");
+}
+
 void MDUnit::Calculate() {
-  if (IsPairListExpired()) MakePairList();
+  // the device system exists once particles are loaded; the timers wait on it
+  mdx_system *dev = swAll ? vars->Device() : nullptr;
+  for (StopWatch *w : {swAll, swForce, swComm, swPair})
+    if (w) w->SetSystem(dev);
+  if (swAll) swAll->Start();
+  if (IsPairListExpired()) {
+    if (swPair) swPair->Start();
+    MakePairList();
+    if (swPair) swPair->Stop();
+  }
   ForceCalculator::UpdatePositionHalf(vars, sinfo);
+  if (swComm) swComm->Start();
   mesh->RefreshImages(vars);
+  if (swComm) swComm->Stop();
+  if (swForce && !sinfo->ControlTemperature) swForce->Start();
   if (sinfo->ControlTemperature && sinfo->HeatbathType == HT_NOSEHOOVER) {
     ForceCalculator::HeatbathZeta(vars, Temperature(), sinfo);
     ForceCalculator::HeatbathMomenta(vars, sinfo);
@@ -344,7 +395,9 @@ void MDUnit::Calculate() {
     ForceCalculator::CalculateForce(vars, mesh, sinfo);
     ForceCalculator::UpdatePositionHalf(vars, sinfo);
   }
+  if (swForce && !sinfo->ControlTemperature) swForce->Stop();
   vars->SimulationTime += sinfo->TimeStep;
+  if (swAll) swAll->Stop();
 }
 
 void MDUnit::SaveAsCdview(std::ostream &ofs) {

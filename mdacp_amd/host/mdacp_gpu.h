@@ -177,6 +177,35 @@ class VirialObserver : public DoubleObserver {
   double Observe(Variables *vars, MeshList *mesh) override;
 };
 
+// StopWatch (stopwatch.h:84-130): one duration in seconds per Start/Stop,
+// written one per line to <basename><rank as %05d>.dat when it is
+// destroyed (rank 0 only, as the reference).  With a system given, Start
+// and Stop first wait for its queued work, as the reference's GPU timer
+// waits on its event.
+class StopWatch {
+ public:
+  StopWatch(int rank, const char *basename, mdx_system *sys = nullptr)
+      : id(rank), base(basename), sys(sys) {}
+  ~StopWatch() {
+    if (id == 0) SaveToFile();
+  }
+  StopWatch(const StopWatch &) = delete;
+  StopWatch &operator=(const StopWatch &) = delete;
+  void SetSystem(mdx_system *s) { sys = s; }
+  void Start();
+  void Stop();
+  void SaveToFile() const;
+  double GetBackData() const { return data.back(); }
+  const std::vector<double> &GetData() const { return data; }
+
+ private:
+  int id;
+  std::string base;
+  mdx_system *sys;
+  double start = 0.0;
+  std::vector<double> data;
+};
+
 // One periodic domain driven in the order of MDManager::Calculate
 // (mdmanager.cc:154-186) and MDManager::MakePairList (:388-417).
 class MDUnit {
@@ -189,6 +218,10 @@ class MDUnit {
   void MakePairList();
   bool IsPairListExpired() { return plist->IsPairListExpired(vars, mesh, sinfo); }
   void Calculate();  // one step, NVE or under the heat bath of SimulationInfo
+  // the timers MDManager::Calculate keeps (mdmanager.cc:155-185): all,
+  // force, comm (image refresh), pair (rebuilds); their files are written
+  // when the unit is destroyed
+  void EnableStopWatches(int rank = 0);
   double ObserveDouble(DoubleObserver *obs) { return obs->Observe(vars, mesh); }
   // mdunit.cc:67-82: "0 type q p" per real, the CDView particle format
   void SaveAsCdview(std::ostream &ofs);
@@ -204,6 +237,7 @@ class MDUnit {
   MeshList *mesh;
   PairList *plist;
   MDRect myrect;
+  StopWatch *swAll = nullptr, *swForce = nullptr, *swComm = nullptr, *swPair = nullptr;
 };
 
 // MDManager::SaveAsCdview / SaveAsCdviewSequential (mdmanager.cc:118-143):

@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(HERE, "libmdx.so")
 
 EXPORTS = (
     "mdx_default_params", "mdx_create", "mdx_destroy", "mdx_last_error", "mdx_set_state",
-    "mdx_get_state", "mdx_particle_number", "mdx_get_images", "mdx_make_pairlist",
+    "mdx_get_state", "mdx_particle_number", "mdx_get_images", "mdx_synchronize",
+    "mdx_make_pairlist",
     "mdx_is_pairlist_expired",
     "mdx_update_position_half", "mdx_refresh_images", "mdx_calculate_force", "mdx_calculate",
     "mdx_observe", "mdx_forces", "mdx_pair_number", "mdx_pairs_canonical", "mdx_timing",
@@ -87,6 +88,7 @@ def load() -> ctypes.CDLL:
         "mdx_get_state": (ctypes.c_int, [vp, dp, dp]),
         "mdx_particle_number": (ctypes.c_int, [vp, ip, ip]),
         "mdx_get_images": (ctypes.c_int, [vp, dp, dp, P(ctypes.c_int32), P(ctypes.c_int32)]),
+        "mdx_synchronize": (ctypes.c_int, [vp]),
         "mdx_make_pairlist": (ctypes.c_int, [vp]),
         "mdx_is_pairlist_expired": (ctypes.c_int, [vp, ip]),
         "mdx_update_position_half": (ctypes.c_int, [vp]),

@@ -9,8 +9,8 @@
 // writes <out_prefix>_q.f64 / _p.f64 (by particle id) at the end.
 //        shim_driver save <state.bin> <steps> <dir>
 // MakePairList and <steps> steps, then in <dir> the reference's output
-// files: SaveAsCdviewSequential (conf0000.cd) and SaveConfiguration
-// (conf000.cnf).
+// files: SaveAsCdviewSequential (conf0000.cd), SaveConfiguration
+// (conf000.cnf) and the step timers' all/force/comm/pair00000.dat.
 #include <unistd.h>
 
 #include <cstdint>
@@ -57,11 +57,12 @@ int main(int argc, char **argv) {
     }
     unit.MakePairList();
     if (save) {
-      for (int s = 0; s < atoi(argv[2]); s++) unit.Calculate();
       if (chdir(argv[3]) != 0) return 2;
+      unit.EnableStopWatches(0);
+      for (int s = 0; s < atoi(argv[2]); s++) unit.Calculate();
       mdx::SaveAsCdviewSequential(unit);
       unit.SaveConfiguration(0);
-      return 0;
+      return 0;   // the unit's timers write their files as it goes out of scope
     }
     mdx::KineticEnergyObserver ke;
     mdx::PotentialEnergyObserver pe(&sinfo);

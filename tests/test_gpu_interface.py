@@ -98,6 +98,13 @@ def test_output_files_match_reference(steps):
         assert r.returncode == 0, r.stderr
         cd = open(os.path.join(tmp, "conf0000.cd"), "rb").read().splitlines()
         cnf = open(os.path.join(tmp, "conf000.cnf"), "rb").read().splitlines()
+        # the step timers (stopwatch.h:115-121): one time per line, all /
+        # force / comm per step, pair per rebuild
+        timers = {k: [float(v) for v in open(os.path.join(tmp, f"{k}00000.dat"))]
+                  for k in ("all", "force", "comm", "pair")}
+    assert len(timers["all"]) == len(timers["force"]) == len(timers["comm"]) == steps
+    assert all(t > 0 for k in timers for t in timers[k])
+    assert sum(timers["force"]) + sum(timers["comm"]) + sum(timers["pair"]) <= sum(timers["all"])
     ref_cd = list(fx[f"cd_k{steps}"])
     ref_cnf = list(fx[f"cnf_k{steps}"])
     assert len(cd) == len(ref_cd) == s.n and len(cnf) == len(ref_cnf)
