@@ -106,8 +106,8 @@ __global__ void k_max_partials(const double *__restrict__ partial, int nb, doubl
 // budget could be spent (below a margin that covers rounding of the device
 // sqrt) or a particle is too fast, the guard raises halt and every later
 // kernel of the batch returns at once; the check raises halt when the list
-// expired.  The host replays the budget exactly from ring[] (steps_done
-// entries) and redoes the check that expired the list.
+// expired, and the host then rebuilds it.  The host replays the budget
+// exactly from ring[] (steps_done entries).
 constexpr int kRing = 64;
 struct StepCtl {
   double budget, dt, buffer;
@@ -2731,10 +2731,19 @@ int mdx_calculate(mdx_system *s, int nsteps, int *rebuilds) {
   if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no state");
   int nreb = 0, rc;
   int done = 0;
+  bool device_expired = false;   // the last batch ended on a device check that expired the list
   while (done < nsteps) {
-    // start of step `done`: PairList::IsPairListExpired, then MakePairList
+    // start of step `done`: PairList::IsPairListExpired, then MakePairList.
+    // After a device check that found the list expired the host has the
+    // verdict already (the budget is spent, so the check is all the
+    // reference does at this point) and rebuilds at once.
     int exp = 0;
-    if ((rc = expired(s, &exp))) return rc;
+    if (device_expired) {
+      exp = 1;
+      device_expired = false;
+    } else if ((rc = expired(s, &exp))) {
+      return rc;
+    }
     if (exp) {
       if ((rc = build_pairlist(s))) return rc;
       nreb++;
@@ -2769,6 +2778,7 @@ int mdx_calculate(mdx_system *s, int nsteps, int *rebuilds) {
       if (s->buffer_length < 0.0 && b + 1 < checked_from)
         return fail(s, MDX_ERR_STATE, "step guard ran past a spent pairlist budget");
     }
+    device_expired = s->hctl->expired != 0;
     if (s->hctl->expired)
       s->check_steps = std::min(16, std::max(2, s->hctl->checks_passed + 2));
     else if (s->hctl->checks_passed > 0 && ran == want)
