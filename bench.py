@@ -94,7 +94,8 @@ def cpu_baseline(state, steps: int, warmup: int):
             env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
                        OMP_PLACES="cores")
             r = subprocess.run([harness, "bench", path, os.path.join(tmp, "o"), str(steps),
-                                str(warmup)], env=env, capture_output=True, text=True, timeout=600)
+                                str(warmup)], env=env, capture_output=True, text=True, timeout=600,
+                               cwd=tmp)
             if r.returncode == 0:
                 line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")][-1]
                 res = json.loads(line)

@@ -62,7 +62,7 @@ def run(mode, state, tmp, *args, heatbath=None):
         env["MDX_REF_HEATBATH"] = heatbath
     out = os.path.join(tmp, "out")
     r = subprocess.run([HARNESS, mode, path, out, *map(str, args)], env=env,
-                       capture_output=True, text=True)
+                       capture_output=True, text=True, cwd=tmp)
     if r.returncode != 0:
         raise RuntimeError(f"harness {mode} failed: {r.stdout}\n{r.stderr}")
     return out

@@ -94,10 +94,10 @@ def unit8(tmp):
         f.write(cfg_text(12))
     env = dict(os.environ, OMP_NUM_THREADS="8")
     conf = os.path.join(tmp, "unit8.bin")
-    subprocess.run([HARNESS, "conf", cfg, conf], check=True, env=env, capture_output=True)
+    subprocess.run([HARNESS, "conf", cfg, conf], check=True, env=env, capture_output=True, cwd=tmp)
     q, p = read_conf(conf)
     series = os.path.join(tmp, "unit8.txt")
-    subprocess.run([HARNESS, "run", cfg, series], check=True, env=env, capture_output=True)
+    subprocess.run([HARNESS, "run", cfg, series], check=True, env=env, capture_output=True, cwd=tmp)
     n, rows = read_series(series)
     assert n == len(q)
     # the same state as one periodic domain
@@ -106,7 +106,7 @@ def unit8(tmp):
     W.write_state(os.path.join(tmp, "unit8_state.bin"), st)
     subprocess.run([REF_HARNESS, "run", os.path.join(tmp, "unit8_state.bin"),
                     os.path.join(tmp, "single"), "1150", "1"], check=True,
-                   env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True)
+                   env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True, cwd=tmp)
     single = np.fromfile(os.path.join(tmp, "single_series.f64")).reshape(-1, 5)[:, :4]
     np.savez(os.path.join(HERE, "benchmark_unit8.npz"), procs=1, threads=8, unit_length=12.0,
              density=0.712, v0=0.9, n=n, L=np.array([24.0, 24.0, 24.0]),

@@ -45,7 +45,7 @@ def main():
             d = os.path.join(tmp, f"k{k}")
             os.mkdir(d)
             subprocess.run([HARNESS, "save", path, d, str(k)], check=True,
-                           env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True)
+                           env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True, cwd=tmp)
             for key, name in (("cd", "conf0000.cd"), ("cnf", "conf000.cnf")):
                 lines = open(os.path.join(d, name), "rb").read().splitlines()
                 out[f"{key}_k{k}"] = np.array(lines, dtype=bytes)
