@@ -167,6 +167,22 @@ int mdx_pair_number(mdx_system *sys, int64_t *count);
  * Writes min(cap, count) triples; *count receives the full count. */
 int mdx_pairs_canonical(mdx_system *sys, int32_t *out, int64_t cap, int64_t *count);
 
+/* Allocation-free variant of mdx_pairs_canonical.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as mdx_pairs_canonical (the
+ * canonical triples, sorted; min(cap, count) of them written; *count the
+ * full count; out = NULL or cap = 0 only counts), but the triples are
+ * staged and sorted in a fixed device array of MDX_NOALLOC_PAIRS pairs
+ * (12 MiB, held by every GPU the library runs on until the process ends)
+ * instead of a buffer allocated per call, and nothing is allocated on the
+ * device or the host.  A pair set larger than MDX_NOALLOC_PAIRS returns
+ * MDX_ERR_CAPACITY (with *count set).  Every system on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (two streams, two
+ * threads) race; calls ordered one after another do not. */
+#define MDX_NOALLOC_PAIRS (1 << 20)
+int mdx_pairs_canonical_noalloc(mdx_system *sys, int32_t *out, int64_t cap, int64_t *count);
+
 /* Differentiable form (no counterpart in the reference, whose epsilon and
  * sigma are hard-wired to 1, variables.cc:16-33): V(r) = eps v(r / sigma)
  * with v the reference's shifted LJ (variables.cc:21-27) cut at 3 sigma.

@@ -28,6 +28,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1559,6 +1560,9 @@ __device__ __forceinline__ void partner_id(int j, int n, const int *tag, const i
 
 // each pair sits in both lists; emit it from the side with the smaller id.
 // LDS offsets map back to global slots through the tile's segments.
+// Counts and offsets are indexed by particle id, so after an exclusive scan
+// in id order the triples of id_lo = t occupy one run starting at offset[t],
+// and the runs are in ascending id_lo.
 template <bool WRITE>
 __global__ void __launch_bounds__(kTile) k_canonical_tiles(
     const TileDesc *__restrict__ tiles, int ntiles, Geom g, Cells c, int n, int kcap,
@@ -1577,7 +1581,7 @@ __global__ void __launch_bounds__(kTile) k_canonical_tiles(
   const int ti = tag[i];
   const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
   int cnum = 0;
-  int w = WRITE ? offset[i] : 0;
+  int w = WRITE ? offset[ti] : 0;
   for_each_partner(lst, nn[i], nb[i], sm, [&](int o3) {
     const int o = o3 / 3;
     int sgi = 0;
@@ -1595,7 +1599,32 @@ __global__ void __launch_bounds__(kTile) k_canonical_tiles(
       cnum++;
     }
   });
-  if (!WRITE) count[i] = cnum;
+  if (!WRITE) count[ti] = cnum;
+}
+
+// Sorts each id's run of triples by (id_hi, code), one thread per id, in
+// place (insertion sort; a run holds one particle's partners with a larger
+// id, a few dozen).  With the runs already in ascending id_lo this leaves
+// the whole array in the canonical (id_lo, id_hi, code) order.
+__global__ void k_sort_runs(const int *__restrict__ offset, const int *__restrict__ count, int n,
+                            int32_t *__restrict__ tri) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int32_t *run = tri + 3 * (size_t)offset[t];
+  const int len = count[t];
+  for (int a = 1; a < len; a++) {
+    const int32_t lo = run[3 * a], hi = run[3 * a + 1], code = run[3 * a + 2];
+    int b = a - 1;
+    while (b >= 0 && (run[3 * b + 1] > hi || (run[3 * b + 1] == hi && run[3 * b + 2] > code))) {
+      run[3 * (b + 1)] = run[3 * b];
+      run[3 * (b + 1) + 1] = run[3 * b + 1];
+      run[3 * (b + 1) + 2] = run[3 * b + 2];
+      b--;
+    }
+    run[3 * (b + 1)] = lo;
+    run[3 * (b + 1) + 1] = hi;
+    run[3 * (b + 1) + 2] = code;
+  }
 }
 
 __global__ void k_scatter_by_tag(const double4 *__restrict__ src, const int *__restrict__ tag, int n,
@@ -1627,6 +1656,36 @@ __global__ void k_iota(int *__restrict__ v, int n) {
 inline int nblk(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 thread_local std::string g_create_error;
+
+// Staging of mdx_pairs_canonical_noalloc: the canonical triples of at most
+// kNoallocPairs pairs (12 MiB), in a module-scope device array that every
+// GPU the library runs on holds until the process ends.  Host code never
+// passes the array's name to a kernel or a copy (on the host it is a
+// stand-in, not a device address): mdx_create looks up each device's own
+// address once, with that device current, and keeps it in g_scratch_dev.
+// All systems on one GPU share that GPU's array, so allocation-free
+// exports that overlap on one GPU race.
+constexpr long long kNoallocPairs = MDX_NOALLOC_PAIRS;
+__device__ __align__(256) int32_t scratch[3 * kNoallocPairs];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t *g_scratch_dev[kMaxDevices] = {};
+
+// this device's address of `scratch` (the device must be current)
+hipError_t scratch_address(int device, int32_t **out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[device] = static_cast<int32_t *>(p);
+  }
+  *out = g_scratch_dev[device];
+  return hipSuccess;
+}
 
 }  // namespace
 
@@ -1686,6 +1745,7 @@ struct mdx_system {
   long long n_build = 0, n_steps = 0, n_kick = 0;
   double t_kick_only = 0;
   long long n_kick_timed = 0;
+  int32_t *scratch_dev = nullptr;   // this device's `scratch` (mdx_pairs_canonical_noalloc)
   std::string err;
 };
 
@@ -2458,6 +2518,14 @@ int mdx_create(const mdx_params *prm, mdx_system **out) {
   mdx_system *s = new mdx_system();
   s->prm = *prm;
   s->SL = SL;
+  {
+    const hipError_t e = scratch_address(prm->device, &s->scratch_dev);
+    if (e != hipSuccess) {
+      delete s;
+      return fail(nullptr, MDX_ERR_HIP, "address of the export staging on device %d: %s",
+                  prm->device, hipGetErrorString(e));
+    }
+  }
   // quadratic-shift constants (variables.cc:21-27)
   {
     const double s2 = 1.0 / (prm->cutoff * prm->cutoff);
@@ -2843,6 +2911,38 @@ int mdx_pairs_canonical(mdx_system *s, int32_t *out, int64_t cap, int64_t *count
   const long long w = std::min<long long>(cap, total);
   for (long long k = 0; k < w; k++)
     for (int d = 0; d < 3; d++) out[3 * k + d] = h[3 * idx[k] + d];
+  return MDX_OK;
+}
+
+int mdx_pairs_canonical_noalloc(mdx_system *s, int32_t *out, int64_t cap, int64_t *count) {
+  if (!s) return MDX_ERR_ARG;
+  if (cap < 0) return fail(s, MDX_ERR_ARG, "negative capacity");
+  if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
+  if (!s->scratch_dev) return fail(s, MDX_ERR_STATE, "no export staging on this device");
+  const int n = s->n;
+  k_canonical_tiles<false><<<s->ntiles, kTile, 0, s->stream>>>(
+      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, s->tag,
+      s->gsrc, s->gcode, nullptr, s->i_buf, nullptr);
+  int rc = scan(s, s->i_buf, s->goff, n, &s->dstat->ghost_total);
+  if (rc) return rc;
+  if ((rc = sync_status(s))) return rc;
+  const long long total = s->hstat->ghost_total;
+  if (count) *count = total;
+  if (total > kNoallocPairs)
+    return fail(s, MDX_ERR_CAPACITY,
+                "canonical pair set of %lld pairs exceeds the %lld pairs the allocation-free "
+                "export stages (mdx_pairs_canonical has no limit)",
+                total, kNoallocPairs);
+  if (!out || cap == 0 || total == 0) return MDX_OK;
+  k_canonical_tiles<true><<<s->ntiles, kTile, 0, s->stream>>>(
+      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, s->tag,
+      s->gsrc, s->gcode, s->goff, nullptr, s->scratch_dev);
+  k_sort_runs<<<nblk(n), kBlock, 0, s->stream>>>(s->goff, s->i_buf, n, s->scratch_dev);
+  if ((rc = launch_check(s, "canonical export"))) return rc;
+  const long long w = std::min<long long>(cap, total);
+  HIPCHK(s, hipMemcpyAsync(out, s->scratch_dev, sizeof(int32_t) * 3 * (size_t)w,
+                           hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
   return MDX_OK;
 }
 

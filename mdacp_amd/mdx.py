@@ -19,7 +19,8 @@ EXPORTS = (
     "mdx_make_pairlist",
     "mdx_is_pairlist_expired",
     "mdx_update_position_half", "mdx_refresh_images", "mdx_calculate_force", "mdx_calculate",
-    "mdx_observe", "mdx_forces", "mdx_pair_number", "mdx_pairs_canonical", "mdx_timing",
+    "mdx_observe", "mdx_forces", "mdx_pair_number", "mdx_pairs_canonical",
+    "mdx_pairs_canonical_noalloc", "mdx_timing",
     "mdx_timing_reset", "mdx_kick_profile", "mdx_lj_trajectory",
     "mdx_set_heatbath", "mdx_get_heatbath", "mdx_heatbath_zeta", "mdx_heatbath_momenta",
     "mdx_langevin",
@@ -99,6 +100,8 @@ def load() -> ctypes.CDLL:
         "mdx_forces": (ctypes.c_int, [vp, dp]),
         "mdx_pair_number": (ctypes.c_int, [vp, i64p]),
         "mdx_pairs_canonical": (ctypes.c_int, [vp, P(ctypes.c_int32), ctypes.c_int64, i64p]),
+        "mdx_pairs_canonical_noalloc": (ctypes.c_int, [vp, P(ctypes.c_int32), ctypes.c_int64,
+                                                       i64p]),
         "mdx_timing": (ctypes.c_int, [vp, dp, dp, dp, i64p, i64p]),
         "mdx_timing_reset": (ctypes.c_int, [vp, ctypes.c_int]),
         "mdx_kick_profile": (ctypes.c_int, [vp, dp, dp]),
