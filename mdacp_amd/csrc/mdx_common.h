@@ -293,12 +293,19 @@ inline PairK make_pairk(double C2, double dt) { return PairK{24.0 * dt, -48.0 * 
 // Pair term of ForceCalculator::CalculateForceSorted (fcalculator.cc:203-217):
 // df = ((24 r^6 - 48) / (r^6 r^6 r^2) + 8 C2) dt, zero beyond the cutoff,
 // evaluated as r^-8 (24 dt - 48 dt r^-6) + 8 C2 dt from one reciprocal of
-// r^2 (recip()): the same quantity in fewer dependent operations.
+// r^2 (recip()): the same quantity in fewer dependent operations.  Beyond
+// the cutoff only the high word of df is cleared (one 32-bit select, not
+// two): what is left is a positive subnormal below 2^-1042 (2.3e-314), so a
+// term beyond the cutoff adds at most 2.3e-314 |dq| (2.3e-214 for the 1e100
+// sentinel), which any accumulator of size above 1e-198 absorbs exactly,
+// and never -0.0.
 __device__ __forceinline__ double pair_df(double r2, const PairK &k, double CL2) {
   const double ir2 = recip(r2);
   const double ir6 = ir2 * ir2 * ir2;
   const double df = __fma_rn(ir6 * ir2, __fma_rn(k.b, ir6, k.a), k.c);
-  return r2 > CL2 ? 0.0 : df;
+  const long long bits = __double_as_longlong(df);
+  const int hi = r2 > CL2 ? 0 : (int)(bits >> 32);
+  return __hiloint2double(hi, (int)(unsigned)bits);
 }
 
 
