@@ -167,6 +167,41 @@ int mdx_pair_number(mdx_system *sys, int64_t *count);
  * Writes min(cap, count) triples; *count receives the full count. */
 int mdx_pairs_canonical(mdx_system *sys, int32_t *out, int64_t cap, int64_t *count);
 
+/* MeshList::MakeList's CSR half list and its getters (meshlist.cc:68-104,
+ * 556-583; meshlist.h:72-88: GetKeyPartnerPairs, GetPairNumber,
+ * GetPartnerNumber, GetKeyPointer, GetSortedList, GetKeyPointerP,
+ * GetNumberOfPartners) in the reference's index space: real particles by id
+ * in [0, pn), periodic image k as pn + k in mdx_get_images order (the
+ * reference's ghosts beyond particle_number).  Each pair has key = its
+ * smaller index: a pair of reals within the search length once, a real and
+ * an image from the real's side, so a pair across the box appears once from
+ * each side with an image as partner, as the reference's real-ghost pairs do
+ * (its observers weight those by 1/2, observer.cc:36-38); image-image pairs
+ * none.  number_of_partners and key_pointer ([tn] each, may be NULL) are
+ * the per-key counts and their exclusive prefix sum (images: 0 partners);
+ * sorted_list receives the partners key by key, min(cap, count) of them,
+ * and *count the total, MeshList::GetPairNumber. */
+int mdx_pairs_csr(mdx_system *sys, int32_t *key_pointer, int32_t *number_of_partners,
+                  int32_t *sorted_list, int64_t cap, int64_t *count);
+/* Variables::AdjustPeriodicBoundary (variables.cc:64-84): q < 0 -> q + L,
+ * q > L -> q - L per axis; MDX_ERR_INVALID_POS ("Invalid position") if a
+ * coordinate is still outside.  The pairlist is void afterwards (the
+ * reference wraps only inside MDManager::MakePairList). */
+int mdx_adjust_periodic_boundary(mdx_system *sys);
+/* Variables::ChangeScale (variables.cc:138-145): q *= alpha for the real
+ * particles, momenta unchanged; the pairlist is void afterwards. */
+int mdx_scale_positions(mdx_system *sys, double alpha);
+/* MeshList::ChangeScale (meshlist.cc:41-65) with SimulationInfo::L: a new
+ * box L3[3], its cell grid (floor(L / search length) cells per axis) and
+ * arrays; MDX_ERR_BOX if an edge is below 2 search lengths.  The pairlist is
+ * void afterwards.  MDUnit::ChangeScale (mdunit.cc:218-222) is
+ * mdx_scale_positions(alpha) then mdx_set_box(alpha L). */
+int mdx_set_box(mdx_system *sys, const double *L3);
+/* The ids of the real particles in the order the device keeps them (cell
+ * order of the last pairlist build), pn entries: the permutation
+ * MeshList::Sort applies to Variables (meshlist.cc:140-173). */
+int mdx_get_cell_order(mdx_system *sys, int32_t *ids);
+
 /* Allocation-free variant of mdx_pairs_canonical.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

@@ -48,6 +48,8 @@ BENCH_SRC = os.path.join(HERE, "host", "benchmark.cpp")
 BENCH_EXE = os.path.join(HERE, "mdx_benchmark")
 DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "shim_driver.cc")
 DRIVER = os.path.join(ROOT, "tests", "cpp", "_build", "shim_driver")
+MIRROR_SRC = os.path.join(ROOT, "tests", "cpp", "mirror_driver.cc")
+MIRROR = os.path.join(ROOT, "tests", "cpp", "_build", "mirror_driver")
 
 # The reference's floating-point flags (its CMakeLists.txt with GCC and
 # USE_AVX2): the Benchmark-mode initial conditions contract a + b c the same
@@ -66,6 +68,8 @@ def build_host(verbose: bool = False) -> None:
         ["g++", "-O2", "-std=c++17", "-Wall", "-o", BENCH_EXE, BENCH_SRC, f"-L{HERE}",
          "-lmdacp_gpu", "-lmdx", "-Wl,-rpath,$ORIGIN"],
         ["g++", "-O2", "-std=c++17", "-Wall", "-o", DRIVER, DRIVER_SRC, f"-L{HERE}",
+         "-lmdacp_gpu", "-lmdx", "-Wl,-rpath,$ORIGIN/../../../mdacp_amd"],
+        ["g++", "-O2", "-std=c++17", "-Wall", "-o", MIRROR, MIRROR_SRC, f"-L{HERE}",
          "-lmdacp_gpu", "-lmdx", "-Wl,-rpath,$ORIGIN/../../../mdacp_amd"],
     ]
     os.makedirs(os.path.dirname(DRIVER), exist_ok=True)

@@ -384,6 +384,37 @@ __global__ void k_wrap_bin_count(double4 *__restrict__ pos, int n, Geom g, doubl
 }
 
 
+// Variables::AdjustPeriodicBoundary (variables.cc:64-84) alone: q < 0 ->
+// q + L, q > L -> q - L per axis, then "Invalid position" if still outside.
+__global__ void k_wrap_reals(double4 *__restrict__ pos, int n, Geom g, int *__restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double4 q = pos[i];
+  double x[3] = {q.x, q.y, q.z};
+  bool bad = false;
+  for (int d = 0; d < 3; d++) {
+    if (x[d] < 0.0) x[d] += g.L[d];
+    else if (x[d] > g.L[d]) x[d] -= g.L[d];
+    if (!(x[d] >= 0.0 && x[d] <= g.L[d])) bad = true;
+  }
+  if (bad) atomicOr(flags, ST_INVALID_POS);
+  q.x = x[0];
+  q.y = x[1];
+  q.z = x[2];
+  pos[i] = q;
+}
+
+// Variables::ChangeScale (variables.cc:138-145): q *= alpha for the reals
+__global__ void k_scale_reals(double4 *__restrict__ pos, int n, double alpha) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double4 q = pos[i];
+  q.x *= alpha;
+  q.y *= alpha;
+  q.z *= alpha;
+  pos[i] = q;
+}
+
 // Images of each real, in a fixed (sz, sy, sx) enumeration; their cells are
 // fixed by the shift (outer layer) so no image can land in an interior cell.
 __global__ void k_make_images(const double4 *__restrict__ pos, const int *__restrict__ cellid,
@@ -1563,7 +1594,12 @@ __device__ __forceinline__ void partner_id(int j, int n, const int *tag, const i
 // Counts and offsets are indexed by particle id, so after an exclusive scan
 // in id order the triples of id_lo = t occupy one run starting at offset[t],
 // and the runs are in ascending id_lo.
-template <bool WRITE>
+// CSR (the reference's half list, meshlist.cc:68-104, 556-583): indices in
+// the reference's space, reals by id and image k as n + k; a real-real pair
+// from the side with the smaller id, a real-image pair always from the
+// real's side (a pair across the box then appears once from each side, as
+// the reference's real-ghost pairs do).  The partner index is written.
+template <bool WRITE, bool CSR = false>
 __global__ void __launch_bounds__(kTile) k_canonical_tiles(
     const TileDesc *__restrict__ tiles, int ntiles, Geom g, Cells c, int n, int kcap,
     const unsigned short *__restrict__ list, const int *__restrict__ nn, const int *__restrict__ nb,
@@ -1587,6 +1623,14 @@ __global__ void __launch_bounds__(kTile) k_canonical_tiles(
     int sgi = 0;
     while (o >= sm.seg_base[sgi + 1]) sgi++;
     const int j = sm.seg_start[sgi] + (o - sm.seg_base[sgi]);
+    if (CSR) {
+      const int hj = j < n ? tag[j] : j;
+      if (j >= n || ti < hj) {
+        if (WRITE) out[w++] = hj;
+        cnum++;
+      }
+      return;
+    }
     int tj, code;
     partner_id(j, n, tag, gsrc, gcode, &tj, &code);
     if (ti < tj) {
@@ -1696,7 +1740,7 @@ struct mdx_system {
   bool own_stream = false;
   Geom geom{};
   double SL = 0, C0 = 0, C2 = 0;
-  int ncell = 0;
+  int ncell = 0, cell_cap = 0, row_cap = 0;
   // sizes
   int n = 0, g = 0, cap = 0, gcap = 0, kcap = 0;
   size_t tot_elems = 0;     // length of pos / ref_pos / cellid / ... arrays
@@ -1896,6 +1940,49 @@ float ev_ms(mdx_system *s, int a, int b) {
   float ms = 0.f;
   if (s->timing) (void)hipEventElapsedTime(&ms, s->ev[a], s->ev[b]);
   return ms;
+}
+
+// The cell grid of MeshList::ChangeScale (meshlist.cc:41-65) for box L:
+// ms = floor(L / SL) interior cells per axis of width L / ms, plus one image
+// layer each side; the arrays sized by it grow when it does.  The pairlist
+// is void afterwards.
+int set_geometry(mdx_system *s, const double L[3]) {
+  const double SL = s->SL;
+  for (int d = 0; d < 3; d++)
+    if (!(L[d] >= 2.0 * SL))
+      return fail(s, MDX_ERR_BOX, "box edge %g < 2 x search length %g", L[d], SL);
+  Geom g{};
+  for (int d = 0; d < 3; d++) {
+    g.L[d] = L[d];
+    g.ms[d] = (int)(L[d] / SL);
+    const double w = L[d] / (double)g.ms[d];
+    g.inv_w[d] = 1.0 / w;
+    g.m[d] = g.ms[d] + 2;
+  }
+  g.SL2 = SL * SL;
+  g.CL2 = s->prm.cutoff * s->prm.cutoff;
+  const int ncell = g.m[0] * g.m[1] * g.m[2], rows = g.ms[1] * g.ms[2];
+  int rc;
+  if (ncell > s->cell_cap) {
+    if ((rc = dev_alloc(s, &s->cell_count, ncell)) || (rc = dev_alloc(s, &s->cell_start, ncell)) ||
+        (rc = dev_alloc(s, &s->cnt_in, ncell)) || (rc = dev_alloc(s, &s->cnt_out, ncell)) ||
+        (rc = dev_alloc(s, &s->scan_in, ncell)) || (rc = dev_alloc(s, &s->scan_out, ncell)))
+      return rc;
+    s->cell_cap = ncell;
+    if (s->cap > 0) {
+      const int tiles = (std::max(s->cap, ncell) + kScanTile - 1) / kScanTile + 1;
+      if ((rc = dev_alloc(s, &s->scan_aux, tiles))) return rc;
+    }
+  }
+  if (rows > s->row_cap) {
+    if ((rc = dev_alloc(s, &s->row_tiles, rows)) || (rc = dev_alloc(s, &s->row_off, rows))) return rc;
+    s->row_cap = rows;
+  }
+  s->geom = g;
+  s->ncell = ncell;
+  for (int d = 0; d < 3; d++) s->prm.L[d] = L[d];
+  s->list_valid = false;
+  return MDX_OK;
 }
 
 int status_error(mdx_system *s) {
@@ -2533,17 +2620,6 @@ int mdx_create(const mdx_params *prm, mdx_system **out) {
     s->C2 = 6.0 * s14 - 3.0 * s8;
     s->C0 = -s12 + s6 - s->C2 / s2;
   }
-  // cell grid of MeshList::ChangeScale (meshlist.cc:41-65)
-  for (int d = 0; d < 3; d++) {
-    s->geom.L[d] = prm->L[d];
-    s->geom.ms[d] = (int)(prm->L[d] / SL);
-    const double w = prm->L[d] / (double)s->geom.ms[d];
-    s->geom.inv_w[d] = 1.0 / w;
-    s->geom.m[d] = s->geom.ms[d] + 2;
-  }
-  s->geom.SL2 = SL * SL;
-  s->geom.CL2 = prm->cutoff * prm->cutoff;
-  s->ncell = s->geom.m[0] * s->geom.m[1] * s->geom.m[2];
   int rc = MDX_OK;
   if (prm->stream) {
     s->stream = (hipStream_t)prm->stream;
@@ -2555,12 +2631,7 @@ int mdx_create(const mdx_params *prm, mdx_system **out) {
     for (auto &e : s->ev)
       if (hipEventCreate(&e) != hipSuccess) rc = MDX_ERR_HIP;
   }
-  if (!rc && ((rc = dev_alloc(s, &s->cell_count, s->ncell)) || (rc = dev_alloc(s, &s->cell_start, s->ncell)) ||
-              (rc = dev_alloc(s, &s->cnt_in, s->ncell)) || (rc = dev_alloc(s, &s->cnt_out, s->ncell)) ||
-              (rc = dev_alloc(s, &s->scan_in, s->ncell)) || (rc = dev_alloc(s, &s->scan_out, s->ncell)) ||
-              (rc = dev_alloc(s, &s->dstat, 1)) ||
-              (rc = dev_alloc(s, &s->row_tiles, s->geom.ms[1] * s->geom.ms[2])) ||
-              (rc = dev_alloc(s, &s->row_off, s->geom.ms[1] * s->geom.ms[2]))))
+  if (!rc && ((rc = set_geometry(s, prm->L)) || (rc = dev_alloc(s, &s->dstat, 1))))
     ;
   if (!rc) {
     hipDeviceProp_t prop;
@@ -2911,6 +2982,87 @@ int mdx_pairs_canonical(mdx_system *s, int32_t *out, int64_t cap, int64_t *count
   const long long w = std::min<long long>(cap, total);
   for (long long k = 0; k < w; k++)
     for (int d = 0; d < 3; d++) out[3 * k + d] = h[3 * idx[k] + d];
+  return MDX_OK;
+}
+
+int mdx_pairs_csr(mdx_system *s, int32_t *key_pointer, int32_t *number_of_partners,
+                  int32_t *sorted_list, int64_t cap, int64_t *count) {
+  if (!s) return MDX_ERR_ARG;
+  if (cap < 0) return fail(s, MDX_ERR_ARG, "negative capacity");
+  if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
+  const int n = s->n, tn = n + s->g;
+  k_canonical_tiles<false, true><<<s->ntiles, kTile, 0, s->stream>>>(
+      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, s->tag,
+      s->gsrc, s->gcode, nullptr, s->i_buf, nullptr);
+  int rc = scan(s, s->i_buf, s->goff, n, &s->dstat->ghost_total);
+  if (rc) return rc;
+  if ((rc = sync_status(s))) return rc;
+  const long long total = s->hstat->ghost_total;
+  if (count) *count = total;
+  // counts and offsets by id; the images key no pair (0 partners, their
+  // offset the total, as the reference's prefix sum over all indices)
+  if (number_of_partners || key_pointer) {
+    if (number_of_partners)
+      HIPCHK(s, hipMemcpyAsync(number_of_partners, s->i_buf, sizeof(int32_t) * n,
+                               hipMemcpyDeviceToHost, s->stream));
+    if (key_pointer)
+      HIPCHK(s, hipMemcpyAsync(key_pointer, s->goff, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
+                               s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    for (int k = n; k < tn; k++) {
+      if (number_of_partners) number_of_partners[k] = 0;
+      if (key_pointer) key_pointer[k] = (int32_t)total;
+    }
+  }
+  if (!sorted_list || cap == 0 || total == 0) return launch_check(s, "csr export");
+  int32_t *dout = nullptr;
+  HIPCHK(s, hipMalloc((void **)&dout, sizeof(int32_t) * (size_t)total));
+  k_canonical_tiles<true, true><<<s->ntiles, kTile, 0, s->stream>>>(
+      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, s->tag,
+      s->gsrc, s->gcode, s->goff, nullptr, dout);
+  const long long w = std::min<long long>(cap, total);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(sorted_list, dout, sizeof(int32_t) * (size_t)w, hipMemcpyDeviceToHost,
+                       s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return fail(s, MDX_ERR_HIP, "csr export: %s", hipGetErrorString(e));
+  return MDX_OK;
+}
+
+int mdx_adjust_periodic_boundary(mdx_system *s) {
+  if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no particles loaded");
+  HIPCHK(s, hipMemsetAsync(&s->dstat->flags, 0, sizeof(int), s->stream));
+  k_wrap_reals<<<nblk(s->n), kBlock, 0, s->stream>>>(s->pos, s->n, s->geom, &s->dstat->flags);
+  s->list_valid = false;   // reals may have moved across the box
+  int rc = launch_check(s, "wrap");
+  if (rc || (rc = sync_status(s))) return rc;
+  return status_error(s);
+}
+
+int mdx_scale_positions(mdx_system *s, double alpha) {
+  if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no particles loaded");
+  if (!(alpha > 0.0) || !std::isfinite(alpha)) return fail(s, MDX_ERR_ARG, "scale %g", alpha);
+  k_scale_reals<<<nblk(s->n), kBlock, 0, s->stream>>>(s->pos, s->n, alpha);
+  s->list_valid = false;
+  int rc = launch_check(s, "scale");
+  if (rc) return rc;
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return MDX_OK;
+}
+
+int mdx_set_box(mdx_system *s, const double *L3) {
+  if (!s || !L3) return fail(s, MDX_ERR_ARG, "null argument");
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return set_geometry(s, L3);
+}
+
+int mdx_get_cell_order(mdx_system *s, int32_t *ids) {
+  if (!s || !ids) return fail(s, MDX_ERR_ARG, "null argument");
+  if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
+  HIPCHK(s, hipMemcpyAsync(ids, s->tag, sizeof(int32_t) * s->n, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
   return MDX_OK;
 }
 

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iostream>
 #include <random>
 
 namespace mdx {
@@ -54,17 +55,64 @@ void Variables::Grow(int need) {
   while (cap < need) cap *= 2;
   qbuf.resize(static_cast<size_t>(cap) * D, 0.0);
   pbuf.resize(static_cast<size_t>(cap) * D, 0.0);
-  type.resize(cap, 0);
+  typebuf.resize(cap, 0);
+  type = typebuf.data();
   q = reinterpret_cast<double(*)[D]>(qbuf.data());
   p = reinterpret_cast<double(*)[D]>(pbuf.data());
   capacity = cap;
 }
 
 int Variables::GetTotalParticleNumber() const {
-  if (!sys) return particle_number;
+  if (!sys || host_dirty || images_dropped) return particle_number;
   int pn = 0, tn = 0;
   mdx_particle_number(sys, &pn, &tn);
   return tn;
+}
+
+void Variables::SetParticleNumber(int pn) {
+  if (pn < 0) throw Error(MDX_ERR_ARG, "negative particle number");
+  SyncToHost();
+  Grow(pn);
+  for (int i = particle_number; i < pn; i++) type[i] = 0;
+  particle_number = pn;
+  host_dirty = true;
+}
+
+void Variables::SetTotalParticleNumber(int tn) {
+  if (tn == particle_number) {
+    images_dropped = true;   // until the next pairlist build
+    return;
+  }
+  if (tn != GetTotalParticleNumber())
+    throw Error(MDX_ERR_ARG, "the images are made on the device: the total is the particle "
+                             "number (images dropped) or the device's own total");
+}
+
+void Variables::AdjustPeriodicBoundary(SimulationInfo *si) {
+  if (sys && !host_dirty) {
+    check(sys, mdx_adjust_periodic_boundary(sys));
+    return;
+  }
+  // on the newer host values, the rule of variables.cc:64-84
+  for (int i = 0; i < particle_number; i++) {
+    for (int d = 0; d < 3; d++) {
+      if (q[i][d] < 0.0) q[i][d] += si->L[d];
+      else if (q[i][d] > si->L[d]) q[i][d] -= si->L[d];
+      if (q[i][d] < 0.0 || q[i][d] > si->L[d] || q[i][d] != q[i][d])
+        throw Error(MDX_ERR_INVALID_POS, "Invalid position");
+    }
+  }
+  host_dirty = true;
+}
+
+void Variables::ChangeScale(double alpha) {
+  if (sys && !host_dirty) {
+    check(sys, mdx_scale_positions(sys, alpha));
+    return;
+  }
+  for (int i = 0; i < particle_number; i++)
+    for (int d = 0; d < 3; d++) q[i][d] *= alpha;
+  host_dirty = true;
 }
 
 void Variables::AddParticle(double x[D], double v[D], int t) {
@@ -121,8 +169,9 @@ void Variables::SyncToDevice() {
     }
   check(sys, mdx_set_state(sys, particle_number, q3.data(), p3.data()));
   // the device numbers particles by their position in this upload
-  host_ids.assign(type.begin(), type.begin() + particle_number);
+  host_ids.assign(type, type + particle_number);
   host_dirty = false;
+  images_dropped = false;
 }
 
 void Variables::SyncToHost() {
@@ -135,6 +184,24 @@ void Variables::SyncToHost() {
       p[i][d] = p3[3 * i + d];
     }
     type[i] = host_ids[i];
+  }
+  // the images (the reference's ghosts), while a pairlist exists
+  if (images_dropped) return;
+  int pn = 0, tn = 0;
+  check(sys, mdx_particle_number(sys, &pn, &tn));
+  const int g = tn - pn;
+  if (g <= 0) return;
+  std::vector<double> gq(3 * static_cast<size_t>(g)), gp(gq.size());
+  std::vector<int32_t> gid(g);
+  if (mdx_get_images(sys, gq.data(), gp.data(), gid.data(), nullptr) != MDX_OK) return;
+  Grow(tn);
+  for (int k = 0; k < g; k++) {
+    for (int d = 0; d < 3; d++) {
+      q[pn + k][d] = gq[3 * k + d];
+      p[pn + k][d] = gp[3 * k + d];
+    }
+    q[pn + k][3] = p[pn + k][3] = 0.0;
+    type[pn + k] = host_ids[gid[k]];
   }
 }
 
@@ -170,15 +237,128 @@ mdx_system *Variables::Device() {
 // ------------------------------------------------------------ MeshList
 MeshList::MeshList(SimulationInfo *sinfo, MDRect &r) {
   (void)sinfo;
-  (void)r;
+  for (int d = 0; d < 3; d++) box[d] = r.GetWidth(d);
+}
+
+void MeshList::ChangeScale(SimulationInfo *sinfo, MDRect &myrect) {
+  (void)sinfo;
+  for (int d = 0; d < 3; d++) box[d] = myrect.GetWidth(d);
+  box_pending = true;
+  fetched = false;
+  if (bound && bound->sys && !bound->host_dirty) ApplyBox(bound);
+}
+
+void MeshList::ApplyBox(Variables *vars) {
+  if (!box_pending) return;
+  mdx_system *s = vars->Device();
+  check(s, mdx_set_box(s, box));
+  box_pending = false;
 }
 
 void MeshList::MakeList(Variables *vars, SimulationInfo *sinfo, MDRect &myrect) {
   (void)sinfo;
   (void)myrect;
+  bound = vars;
+  ApplyBox(vars);
   mdx_system *s = vars->Device();
   check(s, mdx_make_pairlist(s));
+  vars->images_dropped = false;
+  fetched = false;
   number_of_constructions++;
+}
+
+void MeshList::Sort(Variables *vars, SimulationInfo *sinfo, MDRect &myrect) {
+  (void)myrect;
+  if (!sinfo->SortParticle) return;
+  if (number_of_constructions % sort_interval != 0) return;
+  if (!vars->sys || vars->host_dirty) return;   // nothing binned on the device yet
+  const int pn = vars->GetParticleNumber();
+  std::vector<int32_t> order(pn);
+  if (mdx_get_cell_order(vars->sys, order.data()) != MDX_OK) return;   // no list yet
+  vars->SyncToHost();
+  std::vector<double> q2(static_cast<size_t>(pn) * D), p2(q2.size());
+  std::vector<int> t2(pn);
+  for (int i = 0; i < pn; i++) {
+    const int j = order[i];
+    for (int d = 0; d < D; d++) {
+      q2[static_cast<size_t>(i) * D + d] = vars->q[j][d];
+      p2[static_cast<size_t>(i) * D + d] = vars->p[j][d];
+    }
+    t2[i] = vars->type[j];
+  }
+  for (int i = 0; i < pn; i++) {
+    for (int d = 0; d < D; d++) {
+      vars->q[i][d] = q2[static_cast<size_t>(i) * D + d];
+      vars->p[i][d] = p2[static_cast<size_t>(i) * D + d];
+    }
+    vars->type[i] = t2[i];
+  }
+  vars->host_dirty = true;   // uploaded in the new order by the next device call
+  std::cout << "# Sorted!" << std::endl;
+}
+
+void MeshList::Fetch() {
+  if (fetched) return;
+  if (!bound) throw Error(MDX_ERR_STATE, "no pairlist: MakeList first");
+  mdx_system *s = bound->Device();
+  int pn = 0, tn = 0;
+  check(s, mdx_particle_number(s, &pn, &tn));
+  key_pointer.assign(tn, 0);
+  number_of_partners.assign(tn, 0);
+  int64_t c = 0;
+  check(s, mdx_pairs_csr(s, key_pointer.data(), number_of_partners.data(), nullptr, 0, &c));
+  sorted_list.assign(static_cast<size_t>(c), 0);
+  check(s, mdx_pairs_csr(s, nullptr, nullptr, sorted_list.data(), c, &c));
+  key_partner_pairs.resize(2 * static_cast<size_t>(c));
+  for (int i = 0; i < tn; i++)
+    for (int k = key_pointer[i]; k < key_pointer[i] + number_of_partners[i]; k++) {
+      key_partner_pairs[2 * static_cast<size_t>(k)] = i;
+      key_partner_pairs[2 * static_cast<size_t>(k) + 1] = sorted_list[k];
+    }
+  fetched = true;
+}
+
+int (*MeshList::GetKeyPartnerPairs())[2] {
+  Fetch();
+  return reinterpret_cast<int(*)[2]>(key_partner_pairs.data());
+}
+int MeshList::GetPairNumber() {
+  Fetch();
+  return static_cast<int>(sorted_list.size());
+}
+int MeshList::GetPartnerNumber(int i) {
+  Fetch();
+  return number_of_partners.at(i);
+}
+int MeshList::GetKeyPointer(int i) {
+  Fetch();
+  return key_pointer.at(i);
+}
+int *MeshList::GetSortedList() {
+  Fetch();
+  return sorted_list.data();
+}
+int *MeshList::GetKeyPointerP() {
+  Fetch();
+  return key_pointer.data();
+}
+int *MeshList::GetNumberOfPartners() {
+  Fetch();
+  return number_of_partners.data();
+}
+
+void MeshList::ShowPairs() {
+  Fetch();
+  for (size_t k = 0; k < sorted_list.size(); k++)
+    std::printf("(%05d,%05d)\n", key_partner_pairs[2 * k], key_partner_pairs[2 * k + 1]);
+}
+
+void MeshList::ShowSortedList(Variables *vars) {
+  Fetch();
+  const int tn = vars->GetTotalParticleNumber();
+  for (int i = 0; i < tn && i < static_cast<int>(key_pointer.size()); i++)
+    for (int k = 0; k < number_of_partners[i]; k++)
+      std::printf("(%05d,%05d)\n", i, sorted_list[key_pointer[i] + k]);
 }
 
 void MeshList::RefreshImages(Variables *vars) {
@@ -186,7 +366,7 @@ void MeshList::RefreshImages(Variables *vars) {
   check(s, mdx_refresh_images(s));
 }
 
-long MeshList::GetPairNumber(Variables *vars) {
+long MeshList::GetCanonicalPairNumber(Variables *vars) {
   mdx_system *s = vars->Device();
   int64_t c = 0;
   check(s, mdx_pair_number(s, &c));
@@ -310,6 +490,24 @@ MDUnit::MDUnit(SimulationInfo *si) : sinfo(si) {
   mesh = new MeshList(si, myrect);
 }
 
+MDUnit::MDUnit(int id_, SimulationInfo *si, ParaInfo *pi) : sinfo(si), id(id_) {
+  int size[D], pos[D];
+  pi->GetGridSize(size);
+  pi->GetGridPosition(id_, pos);
+  double s[D] = {0, 0, 0, 0}, e[D] = {0, 0, 0, 0};
+  for (int d = 0; d < 3; d++) {   // mdunit.cc:21-26
+    const double ul = si->L[d] / static_cast<double>(size[d]);
+    s[d] = ul * static_cast<double>(pos[d]);
+    e[d] = s[d] + ul;
+  }
+  if (size[X] != 1 || size[Y] != 1 || size[Z] != 1)
+    throw Error(MDX_ERR_ARG, "a GPU system is one periodic domain: the grid must be one unit");
+  myrect = MDRect(s, e);
+  vars = new Variables(si);
+  plist = new PairList();
+  mesh = new MeshList(si, myrect);
+}
+
 MDUnit::~MDUnit() {
   // the timers write their files (rank 0), as the reference's static ones
   // do at exit
@@ -326,9 +524,21 @@ void MDUnit::AddParticle(double x[D], double v[D], int type) {
   if (myrect.IsInside(x)) vars->AddParticle(x, v, type);   // mdunit.cc:61-65
 }
 
+void MDUnit::AddParticle(double x[D], int type) {
+  double v[D] = {0.0, 0.0, 0.0, 0.0};
+  AddParticle(x, v, type);
+}
+
 void MDUnit::MakePairList() {   // MDUnit::MakePairList (mdunit.cc:211-215)
+  mesh->Sort(vars, sinfo, myrect);
   plist->Init(vars, sinfo);
   mesh->MakeList(vars, sinfo, myrect);
+}
+
+void MDUnit::ChangeScale(double alpha) {   // mdunit.cc:217-222
+  myrect.ChangeScale(alpha);
+  vars->ChangeScale(alpha);
+  mesh->ChangeScale(sinfo, myrect);
 }
 
 // MDManager::Calculate (mdmanager.cc:154-186) with CalculateNoseHoover

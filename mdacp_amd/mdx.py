@@ -20,7 +20,8 @@ EXPORTS = (
     "mdx_is_pairlist_expired",
     "mdx_update_position_half", "mdx_refresh_images", "mdx_calculate_force", "mdx_calculate",
     "mdx_observe", "mdx_forces", "mdx_pair_number", "mdx_pairs_canonical",
-    "mdx_pairs_canonical_noalloc", "mdx_timing",
+    "mdx_pairs_canonical_noalloc", "mdx_pairs_csr", "mdx_adjust_periodic_boundary",
+    "mdx_scale_positions", "mdx_set_box", "mdx_get_cell_order", "mdx_timing",
     "mdx_timing_reset", "mdx_kick_profile", "mdx_lj_trajectory",
     "mdx_set_heatbath", "mdx_get_heatbath", "mdx_heatbath_zeta", "mdx_heatbath_momenta",
     "mdx_langevin",
@@ -102,6 +103,12 @@ def load() -> ctypes.CDLL:
         "mdx_pairs_canonical": (ctypes.c_int, [vp, P(ctypes.c_int32), ctypes.c_int64, i64p]),
         "mdx_pairs_canonical_noalloc": (ctypes.c_int, [vp, P(ctypes.c_int32), ctypes.c_int64,
                                                        i64p]),
+        "mdx_pairs_csr": (ctypes.c_int, [vp, P(ctypes.c_int32), P(ctypes.c_int32),
+                                         P(ctypes.c_int32), ctypes.c_int64, i64p]),
+        "mdx_adjust_periodic_boundary": (ctypes.c_int, [vp]),
+        "mdx_scale_positions": (ctypes.c_int, [vp, ctypes.c_double]),
+        "mdx_set_box": (ctypes.c_int, [vp, dp]),
+        "mdx_get_cell_order": (ctypes.c_int, [vp, P(ctypes.c_int32)]),
         "mdx_timing": (ctypes.c_int, [vp, dp, dp, dp, i64p, i64p]),
         "mdx_timing_reset": (ctypes.c_int, [vp, ctypes.c_int]),
         "mdx_kick_profile": (ctypes.c_int, [vp, dp, dp]),
@@ -261,6 +268,37 @@ class System:
             self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), c.value, ctypes.byref(c2)))
         assert c2.value == c.value
         return out
+
+    def pairs_csr(self):
+        """MeshList's CSR half list: (key_pointer [tn], number_of_partners
+        [tn], sorted_list) over reals by id then images (mdx_pairs_csr)."""
+        _, tn = self.particle_number()
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        kp = np.zeros(tn, dtype=np.int32)
+        npart = np.zeros(tn, dtype=np.int32)
+        c = ctypes.c_int64()
+        self._chk(self.lib.mdx_pairs_csr(self.h, kp.ctypes.data_as(i32),
+                                         npart.ctypes.data_as(i32), None, 0, ctypes.byref(c)))
+        sl = np.zeros(max(c.value, 1), dtype=np.int32)
+        c2 = ctypes.c_int64()
+        self._chk(self.lib.mdx_pairs_csr(self.h, None, None, sl.ctypes.data_as(i32), c.value,
+                                         ctypes.byref(c2)))
+        return kp, npart, sl[:c.value]
+
+    def adjust_periodic_boundary(self):
+        self._chk(self.lib.mdx_adjust_periodic_boundary(self.h))
+
+    def scale_positions(self, alpha: float):
+        self._chk(self.lib.mdx_scale_positions(self.h, float(alpha)))
+
+    def set_box(self, L):
+        L3 = np.ascontiguousarray(L, dtype=np.float64)
+        self._chk(self.lib.mdx_set_box(self.h, _dp(L3)))
+
+    def cell_order(self):
+        ids = np.zeros(self.n, dtype=np.int32)
+        self._chk(self.lib.mdx_get_cell_order(self.h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return ids
 
     def timing_reset(self, enable=True):
         self._chk(self.lib.mdx_timing_reset(self.h, 1 if enable else 0))
