@@ -166,6 +166,43 @@ def test_full_size_pairs_vs_oracle():
         assert force_err(g.forces(), of) < FORCE_TOL
 
 
+def test_full_size_run_through_rebuilds_vs_oracle():
+    """N = 2,048,000 from the FCC start, stepped until the pairlist has been
+    rebuilt at least twice (about 60 steps).  The total energy stays within
+    3e-5 relative of its start: the bar the second-order test below sets at
+    N = 4000 over 2000 steps from the same lattice, density and temperature
+    (the integrator's relative error is intensive, so it carries over to the
+    larger system and the shorter run).  Then, on the evolved state, the
+    forces of the list built inside the run equal the oracle's from a fresh
+    list, and a fresh build equals the oracle's pair set (digest) and forces."""
+    s = W.plain_full()
+    with System.from_state(s) as g:
+        g.make_pairlist()
+        ke, pe, _ = g.observe()
+        e0, worst, rebuilds, steps = ke + pe, 0.0, 0, 0
+        while rebuilds < 2 and steps < 400:
+            rebuilds += g.calculate(10)
+            steps += 10
+            g.refresh_images()
+            ke, pe, _ = g.observe()
+            worst = max(worst, abs(ke + pe - e0) / abs(e0))
+        print(f"full size: {steps} steps, {rebuilds} rebuilds, energy error {worst:.2e}", flush=True)
+        assert rebuilds >= 2
+        assert worst < 3e-5
+        q, p = g.state()
+        t = W.State(q, p, s.L.copy(), s.dt)
+        o = O.Oracle(t)
+        o.make_pairlist()
+        of = o.forces()
+        assert force_err(g.forces(), of) < FORCE_TOL       # the run's own list
+        g.set_state(q, p)
+        g.make_pairlist()
+        gp, op = g.pairs(), o.pairs()
+        assert len(gp) == len(op)
+        assert digest(gp) == digest(op)
+        assert force_err(g.forces(), of) < FORCE_TOL
+
+
 def test_full_size_inhomogeneous_slab():
     """The inhomogeneous workload at full size: 200 x 100 x 100 box, liquid
     slab (rho 0.8, T 0.9) in vapour (rho 0.05), 850,000 particles, skewed
