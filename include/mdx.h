@@ -283,7 +283,9 @@ int mdx_ensemble_get_state(mdx_ensemble *e, double *q3, double *p3);
  * [nrep]) receives each replica's pairlist constructions since
  * mdx_ensemble_set_state.  A replica that fails (too fast, invalid
  * position, capacity) stops; the others go on; the first failure is
- * returned and mdx_ensemble_status lists all. */
+ * returned and mdx_ensemble_status lists all.  A replica left short of its
+ * steps after the capacity regrowth rounds also returns MDX_ERR_CAPACITY
+ * (never MDX_OK with steps missing). */
 int mdx_ensemble_calculate(mdx_ensemble *e, int nsteps, int *rebuilds);
 int mdx_ensemble_status(mdx_ensemble *e, int *status, int *steps);
 /* per replica KE, PE, virial ([nrep] each, any may be NULL), as

@@ -50,6 +50,7 @@ constexpr int kRebuildBlocks = 1024;   // grid of the per-particle rebuild kerne
 constexpr int kReplicaThreads = 1024;  // one workgroup per replica: scans, guard
 constexpr int kMaxCand = 1024;         // CheckByDisplacement candidates held in LDS
 constexpr int kMaxImages = 7;          // images per real
+constexpr int kScanPer = 8;            // items per thread in the per-replica scans
 
 enum EnsFlags : int { EF_INVALID_POS = 1, EF_STAGE = 2, EF_LIST = 4, EF_TOO_FAST = 8, EF_IMAGES = 16 };
 
@@ -83,7 +84,9 @@ struct RepCtl {
   int check;          // the budget is spent: a displacement check runs after this step
   int check_slot;     // its position in the check list
   int mid;            // mid-step buffer of the step just done
-  int pad[3];
+  int pad;
+  unsigned long long dmax_bits;   // max squared displacement of the running check (bits of a
+                                  // non-negative double, which order as integers)
 };
 
 struct EnsDev {
@@ -113,11 +116,35 @@ struct EnsDev {
   int *nb;              // [NR] where planes 1 and 2 start in each list (16 + 16 bits)
   double *partial;      // [NT] max |p|^2 per tile
   double *disp;         // [NR + NI] squared displacement since the build
-  double *dpart;        // [R][kRebuildBlocks] block maxima of disp
+  int *foff;            // [kOffKinds][R + 1] prefix sums over the flagged / checked replicas
   int *cand_buf;        // [R][kMaxCand] candidates of each check
   double *obs_partial;  // [3 NT] KE, PE, virial per tile
   int *tile_kmax;       // [NT]
 };
+
+// Index spaces of the flagged replicas (those rebuilding in this step) laid
+// end to end, so that one launch covers every flagged particle, image, cell
+// or tile with all its workgroups: foff[kind][f] is the first index of the
+// f-th flagged replica (a replica stopped by a failure counts zero), and
+// the last entry the total.  OFF_CHECK is the same over the replicas whose
+// displacement check runs in this step (reals and images).
+enum OffKind { OFF_REALS = 0, OFF_IMAGES = 1, OFF_CELLS = 2, OFF_TILES = 3, OFF_CHECK = 4, kOffKinds = 5 };
+
+__device__ __forceinline__ const int *offsets(const EnsDev &d, int kind) {
+  return d.foff + (size_t)kind * (d.R + 1);
+}
+
+// the replica (position f in its list) holding flat index t: the last f
+// with off[f] <= t (replicas of size zero are passed over)
+__device__ __forceinline__ int replica_at(const int *off, int nf, int t) {
+  int lo = 0, hi = nf;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
 
 // flagged for a rebuild in this step: expired and about to step (a replica
 // that has finished its steps keeps its state until it is asked for more)
@@ -286,15 +313,77 @@ __global__ void k_e_begin(EnsDev d) {
   }
 }
 
+// The offsets of the kinds in `mask` (one workgroup): a block scan of the
+// sizes in list order.  A replica whose rebuild has failed counts zero.
+__global__ void __launch_bounds__(kReplicaThreads) k_e_offsets(EnsDev d, int mask) {
+  for (int kind = 0; kind < kOffKinds; kind++) {
+    if (!(mask & (1 << kind))) continue;
+    const bool chk = kind == OFF_CHECK;
+    const int nf = chk ? d.meta[1] : d.meta[0];
+    const int *list = chk ? d.checklist : d.flagged;
+    int *off = d.foff + (size_t)kind * (d.R + 1);
+    int carry = 0;
+    for (int base = 0; base < nf; base += blockDim.x) {
+      const int f = base + threadIdx.x;
+      int v = 0;
+      if (f < nf) {
+        const int r = list[f];
+        const RepCtl *c = d.ctl + r;
+        const Rep &rp = d.rep[r];
+        if (chk) v = rp.n + c->g;
+        else if (!c->eflags)
+          v = kind == OFF_REALS ? rp.n : kind == OFF_IMAGES ? c->g : kind == OFF_CELLS ? rp.ncell : c->ntiles;
+      }
+      int tot;
+      const int ex = block_excl_scan(v, &tot);
+      if (f < nf) off[f] = carry + ex;
+      carry += tot;
+    }
+    if (threadIdx.x == 0) off[nf] = carry;
+    __syncthreads();
+  }
+}
+
+// The offsets of one kind as the workgroup reads them: copied into LDS when
+// they fit (the binary search of every item then reads LDS, not a chain of
+// global loads), else the global array.  Every thread calls it.
+constexpr int kOffLds = 256;
+struct OffView {
+  const int *off;
+  int nf, total;
+};
+__device__ __forceinline__ OffView load_offsets(const EnsDev &d, int kind, int *lds) {
+  const int nf = kind == OFF_CHECK ? d.meta[1] : d.meta[0];
+  const int *g = offsets(d, kind);
+  OffView v{g, nf, g[nf]};
+  if (nf + 1 <= kOffLds && v.total > 0) {
+    for (int f = threadIdx.x; f <= nf; f += blockDim.x) lds[f] = g[f];
+    __syncthreads();
+    v.off = lds;
+  }
+  return v;
+}
+
+// Loop over the flat index space of one offsets kind: body(r, k) for item k
+// of replica r, every workgroup taking a share.  Every thread calls it.
+template <class Body>
+__device__ __forceinline__ void for_flagged(const EnsDev &d, int kind, int *lds, Body body) {
+  const int *list = kind == OFF_CHECK ? d.checklist : d.flagged;
+  const OffView v = load_offsets(d, kind, lds);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < v.total; t += gridDim.x * blockDim.x) {
+    const int f = replica_at(v.off, v.nf, t);
+    body(list[f], t - v.off[f]);
+  }
+}
+
 // wrap into the box (variables.cc:64-84), bin, count images (mdrect.cc:24-47);
 // the pre-sort copies go to the *_tmp arrays
 __global__ void k_e_wrap(EnsDev d) {
-  const int nf = d.meta[0];
-  for (int f = 0; f < nf; f++) {
-    const int r = d.flagged[f];
-    const Rep &rp = d.rep[r];
-    const Geom &g = rp.g;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < rp.n; k += gridDim.x * blockDim.x) {
+  __shared__ int soff[kOffLds];
+  for_flagged(d, OFF_REALS, soff, [&](int r, int k) {
+    {
+      const Rep &rp = d.rep[r];
+      const Geom &g = rp.g;
       const int i = rp.rbase + k;
       double4 q = d.qe[i];
       double x[3] = {q.x, q.y, q.z};
@@ -321,7 +410,7 @@ __global__ void k_e_wrap(EnsDev d) {
       d.slot[i] = wave_grouped_add(d.cell_count, cl);
       d.nimg[i] = cnt - 1;
     }
-  }
+  });
 }
 
 // image offsets: exclusive scan of the image counts of each flagged replica
@@ -330,13 +419,23 @@ __global__ void __launch_bounds__(kReplicaThreads) k_e_img_scan(EnsDev d) {
   RepCtl *c = d.ctl + r;
   if (!rebuilding(c)) return;
   const Rep &rp = d.rep[r];
+  // kScanPer consecutive reals per thread: a block scan per kScanPer x 1024
   int carry = 0;
-  for (int base = 0; base < rp.n; base += blockDim.x) {
-    const int k = base + threadIdx.x;
-    const int v = k < rp.n ? d.nimg[rp.rbase + k] : 0;
+  for (int base = 0; base < rp.n; base += blockDim.x * kScanPer) {
+    const int k0 = base + threadIdx.x * kScanPer;
+    int v[kScanPer], sum = 0;
+#pragma unroll
+    for (int j = 0; j < kScanPer; j++) {
+      v[j] = k0 + j < rp.n ? d.nimg[rp.rbase + k0 + j] : 0;
+      sum += v[j];
+    }
     int tot;
-    const int ex = block_excl_scan(v, &tot);
-    if (k < rp.n) d.goff[rp.rbase + k] = carry + ex;
+    int run = carry + block_excl_scan(sum, &tot);
+#pragma unroll
+    for (int j = 0; j < kScanPer; j++) {
+      if (k0 + j < rp.n) d.goff[rp.rbase + k0 + j] = run;
+      run += v[j];
+    }
     carry += tot;
   }
   if (threadIdx.x == 0) {
@@ -348,15 +447,14 @@ __global__ void __launch_bounds__(kReplicaThreads) k_e_img_scan(EnsDev d) {
 // images of each real in the fixed (sz, sy, sx) enumeration, with their
 // outer-layer cells (MDUnit::FindBorderParticles, mdunit.cc:153-192)
 __global__ void k_e_images(EnsDev d) {
-  const int nf = d.meta[0];
-  for (int f = 0; f < nf; f++) {
-    const int r = d.flagged[f];
-    if (d.ctl[r].eflags) continue;
+  __shared__ int soff[kOffLds];
+  for_flagged(d, OFF_REALS, soff, [&](int r, int k) {
+    if (d.ctl[r].eflags) return;
     const Rep &rp = d.rep[r];
     const Geom &g = rp.g;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < rp.n; k += gridDim.x * blockDim.x) {
+    {
       const int i = rp.rbase + k;
-      if (d.nimg[i] == 0) continue;
+      if (d.nimg[i] == 0) return;
       const double4 q = d.pos_tmp[i];
       const double x[3] = {q.x, q.y, q.z};
       int sh[3];
@@ -390,7 +488,7 @@ __global__ void k_e_images(EnsDev d) {
             idx++;
           }
     }
-  }
+  });
 }
 
 // Per flagged replica: split the cell counts into reals (interior) and
@@ -405,28 +503,42 @@ __global__ void __launch_bounds__(kReplicaThreads) k_e_cells(EnsDev d) {
   const Rep &rp = d.rep[r];
   const Geom &g = rp.g;
   int carry_in = 0, carry_out = 0;
-  for (int base = 0; base < rp.ncell; base += blockDim.x) {
-    const int k = base + threadIdx.x;
-    int ci = 0, co = 0;
-    if (k < rp.ncell) {
-      const int x = k % g.m[0], y = (k / g.m[0]) % g.m[1], z = k / (g.m[0] * g.m[1]);
-      const bool inner = x >= 1 && x <= g.ms[0] && y >= 1 && y <= g.ms[1] && z >= 1 && z <= g.ms[2];
-      const int cnt = d.cell_count[rp.cbase + k];
-      ci = inner ? cnt : 0;
-      co = inner ? 0 : cnt;
+  // kScanPer consecutive cells per thread: two block scans per kScanPer x 1024
+  for (int base = 0; base < rp.ncell; base += blockDim.x * kScanPer) {
+    const int k0 = base + threadIdx.x * kScanPer;
+    int ci[kScanPer], co[kScanPer], sin_ = 0, sout = 0;
+#pragma unroll
+    for (int j = 0; j < kScanPer; j++) {
+      const int k = k0 + j;
+      ci[j] = co[j] = 0;
+      if (k < rp.ncell) {
+        const int x = k % g.m[0], y = (k / g.m[0]) % g.m[1], z = k / (g.m[0] * g.m[1]);
+        const bool inner = x >= 1 && x <= g.ms[0] && y >= 1 && y <= g.ms[1] && z >= 1 && z <= g.ms[2];
+        const int cnt = d.cell_count[rp.cbase + k];
+        ci[j] = inner ? cnt : 0;
+        co[j] = inner ? 0 : cnt;
+      }
+      sin_ += ci[j];
+      sout += co[j];
     }
     int tin, tout;
-    const int ex_in = block_excl_scan(ci, &tin);
-    const int ex_out = block_excl_scan(co, &tout);
-    if (k < rp.ncell) {
-      const int cl = rp.cbase + k;
-      const int si = rp.rbase + carry_in + ex_in;
-      const int so = d.NR + rp.ibase + carry_out + ex_out;
-      d.cnt_in[cl] = ci;
-      d.cnt_out[cl] = co;
-      d.scan_in[cl] = si;
-      d.scan_out[cl] = so;
-      d.cell_start[cl] = ci > 0 ? si : so;
+    int run_in = carry_in + block_excl_scan(sin_, &tin);
+    int run_out = carry_out + block_excl_scan(sout, &tout);
+#pragma unroll
+    for (int j = 0; j < kScanPer; j++) {
+      const int k = k0 + j;
+      if (k < rp.ncell) {
+        const int cl = rp.cbase + k;
+        const int si = rp.rbase + run_in;
+        const int so = d.NR + rp.ibase + run_out;
+        d.cnt_in[cl] = ci[j];
+        d.cnt_out[cl] = co[j];
+        d.scan_in[cl] = si;
+        d.scan_out[cl] = so;
+        d.cell_start[cl] = ci[j] > 0 ? si : so;
+      }
+      run_in += ci[j];
+      run_out += co[j];
     }
     carry_in += tin;
     carry_out += tout;
@@ -499,48 +611,39 @@ __global__ void __launch_bounds__(kReplicaThreads) k_e_cells(EnsDev d) {
 
 // place pre-sort key k (real slot, or NR + image index) into its cell
 __global__ void k_e_scatter(EnsDev d) {
-  const int nf = d.meta[0];
-  for (int f = 0; f < nf; f++) {
-    const int r = d.flagged[f];
-    if (d.ctl[r].eflags) continue;
-    const Rep &rp = d.rep[r];
-    const int g = d.ctl[r].g;
-    const int stride = gridDim.x * blockDim.x;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < rp.n + g; k += stride) {
-      const int key = k < rp.n ? rp.rbase + k : d.NR + rp.ibase + (k - rp.n);
-      d.order[d.cell_start[d.cellid[key]] + d.slot[key]] = key;
-    }
-  }
+  __shared__ int soff[kOffLds];
+  auto place = [&](int key) { d.order[d.cell_start[d.cellid[key]] + d.slot[key]] = key; };
+  for_flagged(d, OFF_REALS, soff, [&](int r, int k) {
+    if (!d.ctl[r].eflags) place(d.rep[r].rbase + k);
+  });
+  __syncthreads();   // soff is reloaded for the images
+  for_flagged(d, OFF_IMAGES, soff, [&](int r, int k) { place(d.NR + d.rep[r].ibase + k); });
 }
 
 // atomics fixed the cell, not the order in it: sort each cell by pre-sort
 // key, which makes the result a stable counting sort
 __global__ void k_e_cell_sort(EnsDev d) {
   __shared__ int buf[kBlock / 64][kSortBuf];
-  const int nf = d.meta[0];
+  __shared__ int soff[kOffLds];
+  const OffView v = load_offsets(d, OFF_CELLS, soff);
   const int w = threadIdx.x >> 6, nw = gridDim.x * (kBlock / 64);
-  for (int f = 0; f < nf; f++) {
-    const int r = d.flagged[f];
-    if (d.ctl[r].eflags) continue;
-    const Rep &rp = d.rep[r];
-    for (int k = blockIdx.x * (kBlock / 64) + w; k < rp.ncell; k += nw) {
-      const int cl = rp.cbase + k;
-      wave_rank_sort(d.order + d.cell_start[cl], d.cell_count[cl], buf[w]);
-    }
+  for (int t = blockIdx.x * (kBlock / 64) + w; t < v.total; t += nw) {
+    const int f = replica_at(v.off, v.nf, t);   // the same for the whole wavefront
+    const int cl = d.rep[d.flagged[f]].cbase + (t - v.off[f]);
+    wave_rank_sort(d.order + d.cell_start[cl], d.cell_count[cl], buf[w]);
   }
 }
 
 // reals into cell order, snapshot (PairList::Init, pairlist.cc:11-23), and
 // the first half drift of the coming step into the current mid-step buffer
 __global__ void k_e_gather_reals(EnsDev d) {
-  const int nf = d.meta[0];
+  __shared__ int soff[kOffLds];
   const double dt2 = d.dt * 0.5;
-  for (int f = 0; f < nf; f++) {
-    const int r = d.flagged[f];
-    if (d.ctl[r].eflags) continue;
+  for_flagged(d, OFF_REALS, soff, [&](int r, int k) {
+    if (d.ctl[r].eflags) return;
     const Rep &rp = d.rep[r];
     double4 *mid = d.ctl[r].par ? d.qh1 : d.qh0;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < rp.n; k += gridDim.x * blockDim.x) {
+    {
       const int s = rp.rbase + k;
       const int key = d.order[s];
       const double4 q = d.pos_tmp[key];
@@ -552,26 +655,20 @@ __global__ void k_e_gather_reals(EnsDev d) {
       d.inv[key] = s;
       mid[s] = drift_half(q, p, dt2);
     }
-  }
+  });
 }
 
 __global__ void k_e_gather_images(EnsDev d) {
-  const int nf = d.meta[0];
-  for (int f = 0; f < nf; f++) {
-    const int r = d.flagged[f];
-    if (d.ctl[r].eflags) continue;
-    const Rep &rp = d.rep[r];
-    const int g = d.ctl[r].g;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < g; k += gridDim.x * blockDim.x) {
-      const int s = d.NR + rp.ibase + k;
-      const int pre = d.order[s] - d.NR;
-      const double4 q = d.gpos_tmp[pre];
-      d.qe[s] = q;
-      d.ref[s] = q;
-      d.gsrc[s - d.NR] = d.inv[d.gsrc_tmp[pre]];
-      d.gcode[s - d.NR] = d.gcode_tmp[pre];
-    }
-  }
+  __shared__ int soff[kOffLds];
+  for_flagged(d, OFF_IMAGES, soff, [&](int r, int k) {
+    const int s = d.NR + d.rep[r].ibase + k;
+    const int pre = d.order[s] - d.NR;
+    const double4 q = d.gpos_tmp[pre];
+    d.qe[s] = q;
+    d.ref[s] = q;
+    d.gsrc[s - d.NR] = d.inv[d.gsrc_tmp[pre]];
+    d.gcode[s - d.NR] = d.gcode_tmp[pre];
+  });
 }
 
 // Full neighbour list per tile (as the single-system build): lane = real,
@@ -581,21 +678,12 @@ template <int T>
 __global__ void __launch_bounds__(T) k_e_build(EnsDev d) {
   extern __shared__ double stage_mem[];
   __shared__ TileSmem sm;
-  const int nf = d.meta[0];
-  int total = 0;
-  for (int f = 0; f < nf; f++) {
-    const RepCtl *c = d.ctl + d.flagged[f];
-    total += c->eflags ? 0 : c->ntiles;
-  }
+  __shared__ int soff[kOffLds];
+  const OffView v = load_offsets(d, OFF_TILES, soff);
   // work items: the tiles of every flagged replica, spread over all blocks
-  for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    int f = 0, s = item;
-    for (;; f++) {
-      const RepCtl *c = d.ctl + d.flagged[f];
-      const int nt = c->eflags ? 0 : c->ntiles;
-      if (s < nt) break;
-      s -= nt;
-    }
+  for (int item = blockIdx.x; item < v.total; item += gridDim.x) {
+    const int f = replica_at(v.off, v.nf, item);
+    const int s = item - v.off[f];
     const int r = d.flagged[f];
     const Rep &rp = d.rep[r];
     const Geom &g = rp.g;
@@ -810,6 +898,7 @@ __global__ void k_e_budget(EnsDev d) {
   if (budget < 0.0) {
     c->check = 1;
     c->cand = 0;
+    c->dmax_bits = 0ull;
     const int f = atomicAdd(d.meta + 1, 1);
     c->check_slot = f;
     d.checklist[f] = r;
@@ -825,61 +914,73 @@ __device__ __forceinline__ int check_slot_of(const EnsDev &d, const Rep &rp, int
   return k < rp.n ? rp.rbase + k : d.NR + rp.ibase + (k - rp.n);
 }
 
+// displacement of every real and image of the checked replicas; the maximum
+// per replica goes through one 64-bit atomic max per replica a wavefront
+// touches (a maximum does not depend on the order it is taken in)
 __global__ void k_e_disp(EnsDev d) {
-  const int nf = d.meta[1];
-  for (int f = 0; f < nf; f++) {
-    const int r = d.checklist[f];
-    const RepCtl *c = d.ctl + r;
-    const Rep &rp = d.rep[r];
-    const double4 *mid = c->mid ? d.qh1 : d.qh0;
-    const int tn = rp.n + c->g;
-    double m = 0.0;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < tn; k += gridDim.x * blockDim.x) {
-      const int sl = check_slot_of(d, rp, k);
-      const double d2 = disp2(d.ref[sl], check_pos(d, rp, mid, sl));
+  __shared__ int soff[kOffLds];
+  const OffView v = load_offsets(d, OFF_CHECK, soff);
+  const int *off = v.off;
+  const int nf = v.nf, total = v.total;
+  const int lane = threadIdx.x & 63;
+  const int stride = gridDim.x * blockDim.x;
+  // every lane of a wavefront runs the same number of passes (the ballots
+  // below need them all), idle lanes with r = -1
+  for (int t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); t0 < total; t0 += stride) {
+    const int t = t0 + lane;
+    int r = -1;
+    double d2 = 0.0;
+    if (t < total) {
+      const int f = replica_at(off, nf, t);
+      r = d.checklist[f];
+      const RepCtl *c = d.ctl + r;
+      const Rep &rp = d.rep[r];
+      const double4 *mid = c->mid ? d.qh1 : d.qh0;
+      const int sl = check_slot_of(d, rp, t - off[f]);
+      d2 = disp2(d.ref[sl], check_pos(d, rp, mid, sl));
       d.disp[sl] = d2;
-      m = fmax(m, d2);
     }
-    m = block_max(m);
-    if (threadIdx.x == 0) d.dpart[(size_t)f * gridDim.x + blockIdx.x] = m;
+    unsigned long long todo = __ballot(r >= 0);
+    while (todo) {
+      const int leader = __ffsll((long long)todo) - 1;
+      const int rl = __shfl(r, leader);
+      const unsigned long long same = __ballot(r == rl) & todo;
+      const double m = wave_max(r == rl ? d2 : 0.0);
+      if (lane == leader)
+        atomicMax(&d.ctl[rl].dmax_bits, (unsigned long long)__double_as_longlong(m));
+      todo &= ~same;
+    }
   }
 }
 
-__global__ void k_e_dfold(EnsDev d, int nblocks) {
+__global__ void k_e_dfold(EnsDev d) {
 #pragma clang fp contract(off)
-  const int r = blockIdx.x;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= d.R) return;
   RepCtl *c = d.ctl + r;
   if (c->error || !c->check) return;
-  const int f = c->check_slot;
-  double m = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) m = fmax(m, d.dpart[(size_t)f * nblocks + b]);
-  const double dmax2 = block_max(m);
-  if (threadIdx.x == 0) {
+  const double dmax2 = __longlong_as_double((long long)c->dmax_bits);
+  {
     const double c_distance = d.buffer - __dsqrt_rn(dmax2);
     c->cd2 = c_distance * c_distance;
   }
 }
 
 __global__ void k_e_cand(EnsDev d) {
-  const int nf = d.meta[1];
-  for (int f = 0; f < nf; f++) {
-    const int r = d.checklist[f];
+  __shared__ int soff[kOffLds];
+  for_flagged(d, OFF_CHECK, soff, [&](int r, int k) {
     RepCtl *c = d.ctl + r;
-    const Rep &rp = d.rep[r];
-    const int tn = rp.n + c->g;
-    const double cd2 = c->cd2;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < tn; k += gridDim.x * blockDim.x) {
-      const int sl = check_slot_of(d, rp, k);
-      if (d.disp[sl] > cd2) {
-        const int at = atomicAdd(&c->cand, 1);
-        if (at < kMaxCand) d.cand_buf[(size_t)f * kMaxCand + at] = sl;
-      }
+    const int sl = check_slot_of(d, d.rep[r], k);
+    if (d.disp[sl] > c->cd2) {
+      const int at = atomicAdd(&c->cand, 1);
+      if (at < kMaxCand) d.cand_buf[(size_t)c->check_slot * kMaxCand + at] = sl;
     }
-  }
+  });
 }
 
 __global__ void __launch_bounds__(kReplicaThreads) k_e_decide(EnsDev d) {
   __shared__ int hit;
+  __shared__ double old_q[3 * kMaxCand], now_q[3 * kMaxCand];
   if (blockIdx.x == 0 && threadIdx.x == 0) d.meta[1] = 0;
   const int r = blockIdx.x;
   RepCtl *c = d.ctl + r;
@@ -892,17 +993,34 @@ __global__ void __launch_bounds__(kReplicaThreads) k_e_decide(EnsDev d) {
   if (l >= Nl) {
     expired = 1;
   } else if (l > 1) {
+    // the candidates' build-time and present positions, staged in LDS for
+    // the pair test (l < Nl <= kMaxCand)
     const double4 *mid = c->mid ? d.qh1 : d.qh0;
     const int *cand = d.cand_buf + (size_t)c->check_slot * kMaxCand;
     if (threadIdx.x == 0) hit = 0;
-    __syncthreads();
-    for (long long p = threadIdx.x; p < (long long)l * l; p += blockDim.x) {
-      const int a = (int)(p / l), b = (int)(p % l);
-      if (b <= a) continue;
-      const int s1 = cand[a], s2 = cand[b];
-      if (disp2(d.ref[s1], d.ref[s2]) < rp.g.SL2) continue;
-      if (disp2(check_pos(d, rp, mid, s1), check_pos(d, rp, mid, s2)) < rp.g.CL2) hit = 1;
+    for (int a = threadIdx.x; a < l; a += blockDim.x) {
+      const int sl = cand[a];
+      const double4 o = d.ref[sl], w = check_pos(d, rp, mid, sl);
+      old_q[3 * a] = o.x;
+      old_q[3 * a + 1] = o.y;
+      old_q[3 * a + 2] = o.z;
+      now_q[3 * a] = w.x;
+      now_q[3 * a + 1] = w.y;
+      now_q[3 * a + 2] = w.z;
     }
+    __syncthreads();
+    auto at = [](const double *v, int a) { return make_double4(v[3 * a], v[3 * a + 1], v[3 * a + 2], 0.0); };
+    // pairs (a, b > a) in 32 x 32 blocks of the upper triangle, thread
+    // (ty, tx) of the 1024 taking pair (32 A + ty, 32 B + tx)
+    const int nblk32 = (l + 31) / 32;
+    const int ty = (int)threadIdx.x >> 5, tx = (int)threadIdx.x & 31;
+    for (int A = 0; A < nblk32; A++)
+      for (int B = A; B < nblk32; B++) {
+        const int a = 32 * A + ty, b = 32 * B + tx;
+        if (b <= a || b >= l) continue;
+        if (disp2(at(old_q, a), at(old_q, b)) < rp.g.SL2) continue;
+        if (disp2(at(now_q, a), at(now_q, b)) < rp.g.CL2) hit = 1;
+      }
     __syncthreads();
     expired = hit;
   }
@@ -1045,7 +1163,8 @@ struct mdx_ensemble {
   Rep *rep = nullptr;
   RepCtl *ctl = nullptr;
   int *meta = nullptr, *flagged = nullptr, *checklist = nullptr, *cand_buf = nullptr;
-  double *disp = nullptr, *dpart = nullptr;
+  double *disp = nullptr;
+  int *foff = nullptr;
   double4 *qe = nullptr, *qh0 = nullptr, *qh1 = nullptr, *mom = nullptr, *ref = nullptr,
           *pos_tmp = nullptr, *mom_tmp = nullptr, *gpos_tmp = nullptr, *init_q = nullptr,
           *init_p = nullptr;
@@ -1108,7 +1227,7 @@ EnsDev dev_view(const mdx_ensemble *e) {
   d.flagged = e->flagged;
   d.checklist = e->checklist;
   d.disp = e->disp;
-  d.dpart = e->dpart;
+  d.foff = e->foff;
   d.cand_buf = e->cand_buf;
   d.R = e->R;
   d.NR = e->NR;
@@ -1241,10 +1360,13 @@ template <int T>
 int enqueue_rebuild_t(mdx_ensemble *e) {
   const EnsDev d = dev_view(e);
   k_e_begin<<<e->R, kBlock, 0, e->stream>>>(d);
+  k_e_offsets<<<1, kReplicaThreads, 0, e->stream>>>(d, 1 << OFF_REALS);
   k_e_wrap<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
   k_e_img_scan<<<e->R, kReplicaThreads, 0, e->stream>>>(d);
+  k_e_offsets<<<1, kReplicaThreads, 0, e->stream>>>(d, 1 << OFF_IMAGES);
   k_e_images<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
   k_e_cells<<<e->R, kReplicaThreads, 0, e->stream>>>(d);
+  k_e_offsets<<<1, kReplicaThreads, 0, e->stream>>>(d, (1 << OFF_IMAGES) | (1 << OFF_CELLS) | (1 << OFF_TILES));
   k_e_scatter<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
   k_e_cell_sort<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
   k_e_gather_reals<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
@@ -1261,8 +1383,9 @@ int enqueue_step_t(mdx_ensemble *e) {
   const EnsDev d = dev_view(e);
   k_e_step<T><<<e->step_grid, T, stage_bytes(e), e->stream>>>(d);
   k_e_budget<<<e->R, kBlock, 0, e->stream>>>(d);
+  k_e_offsets<<<1, kReplicaThreads, 0, e->stream>>>(d, 1 << OFF_CHECK);
   k_e_disp<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
-  k_e_dfold<<<e->R, kBlock, 0, e->stream>>>(d, kRebuildBlocks);
+  k_e_dfold<<<(e->R + kBlock - 1) / kBlock, kBlock, 0, e->stream>>>(d);
   k_e_cand<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
   k_e_decide<<<e->R, kReplicaThreads, 0, e->stream>>>(d);
   return launch_check(e, "step");
@@ -1492,7 +1615,7 @@ int mdx_ensemble_create(const mdx_params *prm, int nrep, const int *n, const dou
   if (!rc && ((rc = ealloc(e, &e->rep, nrep)) || (rc = ealloc(e, &e->ctl, nrep)) ||
               (rc = ealloc(e, &e->meta, 4)) || (rc = ealloc(e, &e->flagged, nrep)) ||
               (rc = ealloc(e, &e->checklist, nrep)) || (rc = ealloc(e, &e->disp, tot)) ||
-              (rc = ealloc(e, &e->dpart, (size_t)nrep * kRebuildBlocks)) ||
+              (rc = ealloc(e, &e->foff, (size_t)kOffKinds * (nrep + 1))) ||
               (rc = ealloc(e, &e->cand_buf, (size_t)nrep * kMaxCand)) ||
               (rc = ealloc(e, &e->qe, tot)) || (rc = ealloc(e, &e->ref, tot)) ||
               (rc = ealloc(e, &e->qh0, NR)) || (rc = ealloc(e, &e->qh1, NR)) ||
@@ -1540,7 +1663,7 @@ int mdx_ensemble_destroy(mdx_ensemble *e) {
   if (!e) return MDX_OK;
   if (e->exec) (void)hipGraphExecDestroy(e->exec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
-  void *bufs[] = {e->rep, e->ctl, e->meta, e->flagged, e->checklist, e->disp, e->dpart, e->cand_buf, e->qe, e->qh0, e->qh1, e->mom, e->ref,
+  void *bufs[] = {e->rep, e->ctl, e->meta, e->flagged, e->checklist, e->disp, e->foff, e->cand_buf, e->qe, e->qh0, e->qh1, e->mom, e->ref,
                   e->pos_tmp, e->mom_tmp, e->gpos_tmp, e->init_q, e->init_p, e->tag, e->tag_tmp,
                   e->gsrc, e->gcode, e->gsrc_tmp, e->gcode_tmp, e->cellid, e->slot, e->nimg, e->goff,
                   e->order, e->inv, e->cell_count, e->cnt_in, e->cnt_out, e->scan_in, e->scan_out,
@@ -1635,6 +1758,16 @@ int mdx_ensemble_calculate(mdx_ensemble *e, int nsteps, int *rebuilds) {
       return efail(e, c.error, "replica %d: Too fast particles exists. max_velocity = %f", first_bad,
                    std::sqrt(c.vmax2));
     return efail(e, c.error, "replica %d stopped (status %d, flags %d)", first_bad, c.error, c.eflags);
+  }
+  // every running replica has done its steps, or the capacity fixes did not
+  // converge within the launch rounds: say so rather than return short
+  for (int r = 0; r < e->R; r++) {
+    const RepCtl &c = e->hctl[r];
+    if (c.steps < c.target)
+      return efail(e, MDX_ERR_CAPACITY,
+                   "replica %d completed %d of %d steps: its capacity was grown in every launch "
+                   "round without the rebuild fitting (mdx_ensemble_status gives the counts)",
+                   r, c.steps - (c.target - nsteps), nsteps);
   }
   return worst;
 }
