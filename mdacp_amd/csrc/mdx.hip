@@ -691,6 +691,52 @@ __global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict_
   block_max_to(pmax, partial + gridDim.x);
 }
 
+// The staging index where each cell of a tile's nine rows starts, reals
+// (interior cells, kind 0) and images (outer cells, kind 1): row r, cell k of
+// x = xf - 1 + k at tab[(2 r + kind) * kCellTab + k], and the row's end at
+// k = nx = xl - xf + 3.  Consecutive cells of a row are consecutive slots,
+// so a cell's range is [tab[k], tab[k + 1]) and a run of cells one range.
+// Filled by every thread; false (and nothing written) when a row has more
+// cells than the table holds.
+constexpr int kCellTab = 48;
+__device__ __forceinline__ bool tile_cell_table(const Geom &g, const Cells &c, int n,
+                                                const TileDesc &t, const TileSmem &sm,
+                                                unsigned short *tab) {
+  const int nx = t.xl - t.xf + 3;
+  if (nx + 1 > kCellTab) return false;
+  for (int e = threadIdx.x; e < 18 * (nx + 1); e += blockDim.x) {
+    const int row = e / (nx + 1), k = e % (nx + 1), r = row >> 1, kind = row & 1;
+    const int y = t.y + r % 3 - 1, z = t.z + r / 3 - 1;
+    const int cl = cell_linear(g, t.xf - 1 + min(k, nx - 1), y, z);
+    const int start = kind == 0 ? c.scan_in[cl] : n + c.scan_out[cl];
+    const int cnt = k == nx ? (kind == 0 ? c.cnt_in[cl] : c.cnt_out[cl]) : 0;
+    tab[row * kCellTab + k] = (unsigned short)(sm.seg_base[row] + (start + cnt - sm.seg_start[row]));
+  }
+  __syncthreads();
+  return true;
+}
+
+// Partners of a real at qi among candidates [lo, hi) of the plane staged at
+// s3 (candidate o at 3 (o - pb)): r^2 <= SL2 (list_r2) and not the real
+// itself, put in candidate order.  Four candidates are tested at a time so
+// that their LDS reads are in flight together.
+template <class Writer>
+__device__ __forceinline__ void range_partners(const double *s3, int pb, const double4 &qi,
+                                               double SL2, int lo, int hi, int self, Writer &gw) {
+  constexpr int kU = 4;
+  for (int o = lo; o < hi; o += kU) {
+    bool ok[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const double *q = s3 + 3 * (min(o + u, hi - 1) - pb);
+      ok[u] = o + u < hi && list_r2(q[0], q[1], q[2], qi.x, qi.y, qi.z) <= SL2 && o + u != self;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++)
+      if (ok[u]) gw.put(plane_entry(o + u - pb));
+  }
+}
+
 // Full neighbour list per tile: one list column per real, every partner
 // j != i (real or image) of its 27 cells with r^2 <= SL^2, r^2 formed as
 // the reference's AVX2 search, fma(dx,dx, fma(dy,dy, dz*dz)) with dq = q_j
@@ -733,12 +779,27 @@ __global__ void __launch_bounds__(kTile) k_build_tiles(
   const int self = act ? sm.seg_base[8] + (i - sm.seg_start[8]) : -1;   // in row (0, 0)
   const int cx = interior_coord(qi.x, g.inv_w[0], g.ms[0]);
   GroupWriter<kTile> gw(list + (size_t)tile * kcap * kTile + (size_t)col * kILP, kcap);
+  __shared__ unsigned short tab[18 * kCellTab];
+  const bool have_tab = tile_cell_table(g, c, n, t, sm, tab);
   int b1 = 0, b2 = 0;
   for (int p = 0; p < kPlanes; p++) {
     if (p > 0) __syncthreads();   // the previous plane is read
     stage_plane(sm, p, sx, [&](int, int slot) { return pos[slot]; });
     const int pb = sm.seg_base[6 * p];
-    if (act) {
+    if (act && have_tab) {
+      // the real's three cells of each row from the table: the image cell
+      // before them, the real cells, the image cells after (only the end
+      // cells of a real row can be image cells, all cells of an image row
+      // are): the same candidates in the same order as cell by cell
+      const int k0 = cx - t.xf;   // cell cx - 1
+      for (int r = 3 * p; r < 3 * p + 3; r++) {
+        const unsigned short *ti = tab + (2 * r) * kCellTab, *to = tab + (2 * r + 1) * kCellTab;
+        range_partners(stage_mem, pb, qi, g.SL2, to[k0], to[k0 + 1], self, gw);
+        range_partners(stage_mem, pb, qi, g.SL2, ti[k0], ti[k0 + 3], self, gw);
+        range_partners(stage_mem, pb, qi, g.SL2, to[k0 + 1], to[k0 + 3], self, gw);
+      }
+      gw.pad_plane(sm.seg_base[6 * p + 6] - pb);
+    } else if (act) {
       for (int r = 3 * p; r < 3 * p + 3; r++) {
         const int y = t.y + r % 3 - 1, z = t.z + r / 3 - 1;
         for (int dx = -1; dx <= 1; dx++) {
@@ -1854,7 +1915,7 @@ int ensure_capacity(mdx_system *s, int n) {
   int cap = std::max(n, s->prm.capacity);
   int rc;
   if ((rc = dev_alloc(s, &s->mom, cap)) || (rc = dev_alloc(s, &s->mom_tmp, cap)) ||
-      (rc = dev_alloc(s, &s->pos_tmp, cap)) || (rc = dev_alloc(s, &s->vec_out, cap)) ||
+      (rc = dev_alloc(s, &s->vec_out, cap)) ||
       (rc = dev_alloc(s, &s->tag, cap)) || (rc = dev_alloc(s, &s->tag_tmp, cap)) ||
       (rc = dev_alloc(s, &s->inv, cap)) || (rc = dev_alloc(s, &s->nimg, cap)) ||
       (rc = dev_alloc(s, &s->goff, cap)) || (rc = dev_alloc(s, &s->nn, cap)) ||
@@ -1885,6 +1946,7 @@ int ensure_image_capacity(mdx_system *s, int gcap) {
   int rc;
   if ((rc = ensure_partials(s, nblk((long long)tot))) ||
       (rc = dev_alloc(s, &s->ref_pos, tot)) || (rc = dev_alloc(s, &s->pos_alt, tot)) ||
+      (rc = dev_alloc(s, &s->pos_tmp, tot)) ||
       (rc = dev_alloc(s, &s->pos_mid, tot)) ||
       (rc = dev_alloc(s, &s->gpos_tmp, gcap)) || (rc = dev_alloc(s, &s->gsrc, gcap)) ||
       (rc = dev_alloc(s, &s->gcode, gcap)) || (rc = dev_alloc(s, &s->gsrc_tmp, gcap)) ||
@@ -2037,26 +2099,28 @@ int build_pairlist(mdx_system *s) {
                                                   s->order);
   k_cell_sort<<<std::min(4096, (s->ncell + 3) / 4), kBlock, 0, s->stream>>>(
       s->order, s->cell_start, s->cell_count, s->ncell);
-  // 4. gather into cell order (pos_tmp/mom_tmp/tag_tmp hold the pre-sort copy)
-  HIPCHK(s, hipMemcpyAsync(s->pos_tmp, s->pos, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream));
-  HIPCHK(s, hipMemcpyAsync(s->mom_tmp, s->mom, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream));
-  HIPCHK(s, hipMemcpyAsync(s->tag_tmp, s->tag, sizeof(int) * n, hipMemcpyDeviceToDevice, s->stream));
-  k_gather_reals<<<nblk(n), kBlock, 0, s->stream>>>(s->order, n, s->pos_tmp, s->mom_tmp, s->tag_tmp,
-                                                     s->pos, s->mom, s->tag, s->inv);
+  // 4. gather into cell order, from pos / mom / tag into their spare
+  // buffers, which then take their place (no copy of the pre-sort state)
+  k_gather_reals<<<nblk(n), kBlock, 0, s->stream>>>(s->order, n, s->pos, s->mom, s->tag,
+                                                     s->pos_tmp, s->mom_tmp, s->tag_tmp, s->inv);
   if (g > 0)
     k_gather_images<<<nblk(g), kBlock, 0, s->stream>>>(s->order, n, g, s->gpos_tmp, s->gsrc_tmp,
-                                                        s->gcode_tmp, s->inv, s->pos, s->gsrc,
+                                                        s->gcode_tmp, s->inv, s->pos_tmp, s->gsrc,
                                                         s->gcode);
+  std::swap(s->pos, s->pos_tmp);
+  std::swap(s->mom, s->mom_tmp);
+  std::swap(s->tag, s->tag_tmp);
   if ((rc = launch_check(s, "sort"))) return rc;
   // 5. tiles over the interior cell rows
   Cells cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out, s->tseg, s->tperm};
   const int rows = s->geom.ms[1] * s->geom.ms[2];
   k_row_tiles<<<nblk(rows), kBlock, 0, s->stream>>>(s->geom, cells, s->row_tiles);
   if ((rc = scan(s, s->row_tiles, s->row_off, rows, &s->dstat->ntiles))) return rc;
-  if ((rc = sync_status(s))) return rc;
-  s->ntiles = s->hstat->ntiles;
-  if (s->ntiles > s->tiles_cap) {
-    const size_t cap = (size_t)s->ntiles + s->ntiles / 8 + 64;
+  // a row of c reals has ceil(c / kTile) <= c / kTile + 1 tiles: sized for
+  // that bound, the tile arrays never need the count before they are written
+  const int tiles_bound = rows + (n + kTile - 1) / kTile;
+  if (tiles_bound > s->tiles_cap) {
+    const size_t cap = (size_t)tiles_bound;
     if ((rc = dev_alloc(s, &s->tiles, cap)) || (rc = dev_alloc(s, &s->tseg, cap * kTSeg)) ||
         (rc = dev_alloc(s, &s->tperm, cap * kTile)))
       return rc;
@@ -2065,12 +2129,13 @@ int build_pairlist(mdx_system *s) {
   cells.tseg = s->tseg;
   cells.tperm = s->tperm;
   const int wt_blocks = (int)(((size_t)rows * 64 + kBlock - 1) / kBlock);
-  if ((rc = ensure_partials(s, std::max(tile_grid(s->ntiles), 2 * wt_blocks)))) return rc;
+  if ((rc = ensure_partials(s, std::max(tile_grid(tiles_bound), 2 * wt_blocks)))) return rc;
   k_write_tiles<<<wt_blocks, kBlock, 0, s->stream>>>(s->geom, cells, n, s->pos, s->row_tiles,
                                                        s->row_off, s->tiles, s->tseg, s->pmax);
   k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, wt_blocks, &s->dstat->stage_max);
   k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax + wt_blocks, wt_blocks, &s->dstat->plane_max);
   if ((rc = sync_status(s))) return rc;
+  s->ntiles = s->hstat->ntiles;
   s->stage_max = (int)s->hstat->stage_max;
   s->plane_max = (int)s->hstat->plane_max;
   if (adj_stage_bytes(s) + 4096 > s->lds_limit)
