@@ -197,6 +197,18 @@ int mdx_scale_positions(mdx_system *sys, double alpha);
  * void afterwards.  MDUnit::ChangeScale (mdunit.cc:218-222) is
  * mdx_scale_positions(alpha) then mdx_set_box(alpha L). */
 int mdx_set_box(mdx_system *sys, const double *L3);
+/* The density grid of the Cavitation project's BubbleHist (cavitation.cc:
+ * 38-80, 117-141, one domain at the origin): g_d = (int)(L_d / grid_size +
+ * 0.5) boxes per axis, one box width w = L_x / g_x for every axis, each
+ * real particle counted in box (trunc(x / w), trunc(y / w), trunc(z / w))
+ * at index ix + iy g_x + iz g_x g_y, the count kept as the reference's
+ * unsigned char (modulo 256).  dims[3] receives g; data (may be NULL: only
+ * the dimensions) receives the g_x g_y g_z bytes that BubbleHist::
+ * SaveDensity writes (cavitation.cc:85-90).  Positions are taken as they
+ * stand (the reference's Analyse first calls MakePairList, which wraps
+ * them); a particle outside the grid is MDX_ERR_INVALID_POS (the
+ * reference's assert). */
+int mdx_density_grid(mdx_system *sys, double grid_size, int32_t *dims, uint8_t *data, int64_t cap);
 /* The ids of the real particles in the order the device keeps them (cell
  * order of the last pairlist build), pn entries: the permutation
  * MeshList::Sort applies to Variables (meshlist.cc:140-173). */

@@ -404,6 +404,31 @@ __global__ void k_wrap_reals(double4 *__restrict__ pos, int n, Geom g, int *__re
   pos[i] = q;
 }
 
+// BubbleHist::AnalyseSub (cavitation.cc:117-141) for one domain at the
+// origin: grid box (ix, iy, iz) = trunc(q * inv) of each real, counted at
+// ix + iy gx + iz gx gy; an index outside the grid is the reference's
+// failed assert (flagged here)
+__global__ void k_density_count(const double4 *__restrict__ pos, int n, double inv, int gx, int gy,
+                                int total, int *__restrict__ hist, int *__restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 q = pos[i];
+  const int ix = (int)(q.x * inv), iy = (int)(q.y * inv), iz = (int)(q.z * inv);
+  const long long k = (long long)ix + (long long)iy * gx + (long long)iz * gx * gy;
+  if (k < 0 || k >= total) {
+    atomicOr(flags, ST_INVALID_POS);
+    return;
+  }
+  atomicAdd(hist + k, 1);
+}
+
+// the counts as the reference's unsigned char cells hold them (modulo 256)
+__global__ void k_density_bytes(const int *__restrict__ hist, int total,
+                                unsigned char *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < total) out[k] = (unsigned char)(hist[k] & 0xff);
+}
+
 // Variables::ChangeScale (variables.cc:138-145): q *= alpha for the reals
 __global__ void k_scale_reals(double4 *__restrict__ pos, int n, double alpha) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3121,6 +3146,46 @@ int mdx_set_box(mdx_system *s, const double *L3) {
   if (!s || !L3) return fail(s, MDX_ERR_ARG, "null argument");
   HIPCHK(s, hipStreamSynchronize(s->stream));
   return set_geometry(s, L3);
+}
+
+int mdx_density_grid(mdx_system *s, double grid_size, int32_t *dims, uint8_t *data, int64_t cap) {
+  if (!s || !dims) return fail(s, MDX_ERR_ARG, "null argument");
+  if (!s->have_state) return fail(s, MDX_ERR_STATE, "no particles loaded");
+  if (!(grid_size > 0.0) || !std::isfinite(grid_size))
+    return fail(s, MDX_ERR_ARG, "grid size %g", grid_size);
+  // BubbleHist::BubbleHist (cavitation.cc:38-80): boxes per axis rounded
+  // from the widths, one box size from the x width
+  int g[3];
+  for (int d = 0; d < 3; d++) g[d] = (int)(s->geom.L[d] / grid_size + 0.5);
+  if (g[0] < 1 || g[1] < 1 || g[2] < 1) return fail(s, MDX_ERR_ARG, "grid size %g exceeds the box", grid_size);
+  const long long total = (long long)g[0] * g[1] * g[2];
+  for (int d = 0; d < 3; d++) dims[d] = g[d];
+  if (!data) return MDX_OK;
+  if (cap < total || total > (1LL << 31) - 1)
+    return fail(s, MDX_ERR_CAPACITY, "density grid of %lld boxes, buffer of %lld", total, (long long)cap);
+  const double box = s->geom.L[0] / (double)g[0];
+  const double inv = 1.0 / box;
+  int *hist = nullptr;
+  unsigned char *bytes = nullptr;
+  HIPCHK(s, hipMalloc((void **)&hist, sizeof(int) * (size_t)total));
+  hipError_t e = hipMalloc((void **)&bytes, (size_t)total);
+  if (e == hipSuccess) e = hipMemsetAsync(hist, 0, sizeof(int) * (size_t)total, s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(&s->dstat->flags, 0, sizeof(int), s->stream);
+  if (e == hipSuccess) {
+    k_density_count<<<nblk(s->n), kBlock, 0, s->stream>>>(s->pos, s->n, inv, g[0], g[1], (int)total,
+                                                          hist, &s->dstat->flags);
+    k_density_bytes<<<nblk(total), kBlock, 0, s->stream>>>(hist, (int)total, bytes);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(data, bytes, (size_t)total, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(s->hstat, s->dstat, sizeof(DevStatus), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(hist);
+  (void)hipFree(bytes);
+  if (e != hipSuccess) return fail(s, MDX_ERR_HIP, "density grid: %s", hipGetErrorString(e));
+  if (s->hstat->flags & ST_INVALID_POS)
+    return fail(s, MDX_ERR_INVALID_POS, "a particle lies outside the density grid");
+  return MDX_OK;
 }
 
 int mdx_get_cell_order(mdx_system *s, int32_t *ids) {

@@ -1,7 +1,17 @@
 // benchmark.cpp — the reference's Benchmark mode (Benchmark::Run,
-// benchmark.cc:12-52) on one GPU through libmdx.
+// benchmark.cc:12-52) and its Cavitation project (Cavitation::Run,
+// cavitation.cc:238-287) on one GPU through libmdx.
 //
 //   mdx_benchmark <input.cfg> [--procs P] [--threads T]
+//
+// Mode=Cavitation: the same initial conditions, ThermalizeLoop steps under
+// the heat bath (HeatbathType, AimedTemperature, HeatbathTau), the box and
+// positions scaled by ChangeScale (MDManager::ChangeScale), then TotalLoop
+// + 1 NVE steps with the BubbleHist density grid (GridSize) analysed and
+// written to conf%04d.dat every ObserveLoop steps, and the reference's
+// "# Thermalize" / "# Observe" lines (time, temperature, potential energy,
+// pressure, total energy per particle, seconds).  It runs through the C++
+// mirror of the reference interface (mdacp_amd/host/mdacp_gpu.h).
 //
 // The input file has the reference's format (Parameter: Key=Value lines,
 // '#' comments) and keys: Mode=Benchmark, Density (0.7), UnitLength or
@@ -25,6 +35,7 @@
 
 #include "../../include/mdx.h"
 #include "confmaker.h"
+#include "mdacp_gpu.h"
 
 namespace {
 
@@ -67,6 +78,83 @@ void Check(mdx_system *s, int rc, const char *what) {
   }
 }
 
+std::string GetString(const std::map<std::string, std::string> &m, const char *k, const char *def) {
+  auto it = m.find(k);
+  return it == m.end() ? std::string(def) : it->second;
+}
+
+// Cavitation::Run (cavitation.cc:238-287) for the units of P x T as one
+// periodic system, through the mirror's MDUnit and BubbleHist
+int RunCavitation(const std::map<std::string, std::string> &prm, int procs, int threads,
+                  double unit_length) {
+  const double density = GetDouble(prm, "Density", 0.7);
+  const double v0 = GetDouble(prm, "InitialVelocity", 1.0);
+  const double dt = GetDouble(prm, "TimeStep", 0.001);
+  const int t_loop = GetInt(prm, "ThermalizeLoop", 150);
+  const int loop = GetInt(prm, "TotalLoop", 1000);
+  const int observe_loop = GetInt(prm, "ObserveLoop", 100);
+  const double change_scale = GetDouble(prm, "ChangeScale", 1.05);
+  const double grid_size = GetDouble(prm, "GridSize", 3.0);
+  const mdx::BenchmarkState st = mdx::MakeBenchmarkState(procs, threads, unit_length, density, v0);
+  const int n = static_cast<int>(st.q.size() / 3);
+  try {
+    // SimulationInfo (simulationinfo.cc:5-30) with the heat bath keys
+    mdx::SimulationInfo sinfo(st.L[0], st.L[1], st.L[2], dt);
+    sinfo.AimedTemperature = GetDouble(prm, "AimedTemperature", 1.0);
+    sinfo.HeatbathTau = GetDouble(prm, "HeatbathTau", 0.1);
+    sinfo.HeatbathGamma = GetDouble(prm, "HeatbathGamma", 0.1);
+    sinfo.HeatbathType =
+        GetString(prm, "HeatbathType", "NoseHoover") == "Langevin" ? mdx::HT_LANGEVIN : mdx::HT_NOSEHOOVER;
+    mdx::MDUnit unit(&sinfo);
+    for (int i = 0; i < n; i++) {
+      double x[mdx::D] = {st.q[3 * i], st.q[3 * i + 1], st.q[3 * i + 2], 0.0};
+      double v[mdx::D] = {st.p[3 * i], st.p[3 * i + 1], st.p[3 * i + 2], 0.0};
+      unit.AddParticle(x, v, i);
+    }
+    std::cout << "# Mode = Cavitation" << std::endl;
+    std::cout << "# " << procs << " MPI Process(es), " << threads << " OpenMP Thread(s), Total "
+              << procs * threads << " Unit(s) on one GPU" << std::endl;
+    std::cout << "# Number of Particles = " << n << std::endl;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto seconds = [&]() {
+      return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    // MDManager::Temperature / PotentialEnergy / Pressure / TotalEnergy
+    // (mdmanager.cc:531-570), per particle
+    auto line = [&](const char *tag) {
+      const double T = unit.Temperature();
+      const double pe = unit.PotentialEnergy();
+      std::cout << unit.GetVariables()->SimulationTime << " " << T << " " << pe << " "
+                << unit.Pressure() << " " << (1.5 * T + pe) << " " << seconds() << " # " << tag
+                << std::endl;
+    };
+    sinfo.ControlTemperature = true;   // MDManager::SetControlTemperature
+    for (int i = 0; i < t_loop; i++) {
+      unit.Calculate();
+      if (i % observe_loop == 0) line("Thermalize");
+    }
+    // MDManager::ChangeScale (mdmanager.cc:572-584)
+    unit.MakePairList();
+    for (int d = 0; d < 3; d++) sinfo.L[d] *= change_scale;
+    unit.ChangeScale(change_scale);
+    unit.MakePairList();
+    std::cout << "# Change Scale " << change_scale << std::endl;
+    sinfo.ControlTemperature = false;
+    mdx::BubbleHist bhist(&unit, grid_size);
+    for (int i = 0; i <= loop; i++) {
+      unit.Calculate();
+      if (i % observe_loop == 0) {
+        bhist.Analyse(&unit);
+        line("Observe");
+      }
+    }
+  } catch (const mdx::Error &e) {
+    std::fprintf(stderr, "mdx error %d: %s\n", e.code, e.what());
+    return 1;
+  }
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -82,8 +170,8 @@ int main(int argc, char **argv) {
   }
   const auto prm = LoadParameters(argv[1]);
   auto mode = prm.find("Mode");
-  if (mode == prm.end() || mode->second != "Benchmark") {
-    std::fprintf(stderr, "only Mode=Benchmark is supported\n");
+  if (mode == prm.end() || (mode->second != "Benchmark" && mode->second != "Cavitation")) {
+    std::fprintf(stderr, "only Mode=Benchmark and Mode=Cavitation are supported\n");
     return 2;
   }
   const mdx::UnitGrid grid(procs, threads);
@@ -93,6 +181,7 @@ int main(int argc, char **argv) {
     std::fprintf(stderr, "need UnitLength, or SystemSize on a cubic unit grid\n");
     return 2;
   }
+  if (mode->second == "Cavitation") return RunCavitation(prm, procs, threads, unit_length);
   const double density = GetDouble(prm, "Density", 0.7);
   const double v0 = GetDouble(prm, "InitialVelocity", 1.0);
   const double dt = GetDouble(prm, "TimeStep", 0.001);

@@ -644,6 +644,35 @@ void SaveAsCdviewSequential(MDUnit &unit) {
   SaveAsCdview(unit, filename);
 }
 
+// ------------------------------------------------------------ BubbleHist
+BubbleHist::BubbleHist(MDUnit *unit, double gsize) : gsize_asked(gsize) {
+  MDRect *r = unit->GetRect();   // cavitation.cc:41-57
+  for (int d = 0; d < 3; d++) grid[d] = static_cast<int>(r->GetWidth(d) / gsize + 0.5);
+  grid_size = r->GetWidth(X) / static_cast<double>(grid[0]);
+  data.assign(static_cast<size_t>(grid[0]) * grid[1] * grid[2], 0);
+  std::cout << "# Grid Information ------" << std::endl;
+  std::cout << "# grid_size = " << grid_size << std::endl;
+  std::cout << "# grid: " << grid[0] << "," << grid[1] << "," << grid[2] << std::endl;
+  std::cout << "# grid_count = " << data.size() << std::endl;
+  std::cout << "# ------" << std::endl;
+}
+
+void BubbleHist::Analyse(MDUnit *unit) {   // cavitation.cc:142-166
+  unit->MakePairList();
+  mdx_system *s = unit->GetVariables()->Device();
+  int32_t dims[3];
+  check(s, mdx_density_grid(s, gsize_asked, dims, data.data(), static_cast<int64_t>(data.size())));
+  SaveDensity();
+  analyse_count++;
+}
+
+void BubbleHist::SaveDensity() const {   // cavitation.cc:85-90
+  char filename[256];
+  std::snprintf(filename, sizeof filename, "conf%04d.dat", analyse_count);
+  std::ofstream ofs(filename, std::ios::binary);
+  ofs.write(reinterpret_cast<const char *>(data.data()), static_cast<std::streamsize>(data.size()));
+}
+
 double MDUnit::Temperature() {
   KineticEnergyObserver obs;
   return obs.Observe(vars, mesh) / vars->GetParticleNumber() / 1.5;

@@ -367,6 +367,28 @@ class MDUnit {
   StopWatch *swAll = nullptr, *swForce = nullptr, *swComm = nullptr, *swPair = nullptr;
 };
 
+// BubbleHist of the Cavitation project (cavitation.cc:13-167) for one
+// domain: the particle count of each box of a grid of about gsize per side
+// (mdx_density_grid), kept as unsigned char; Analyse rebuilds the pairlist
+// first as the reference's does, then SaveDensity writes the counts, box
+// index ix + iy gx + iz gx gy, to conf%04d.dat (numbered by analysis).  The
+// constructor prints the reference's grid information lines.
+class BubbleHist {
+ public:
+  BubbleHist(MDUnit *unit, double gsize);
+  void Analyse(MDUnit *unit);
+  void SaveDensity() const;
+  const std::vector<unsigned char> &GetData() const { return data; }
+  int GetGridSize(int d) const { return grid[d]; }
+  int GetAnalyseCount() const { return analyse_count; }
+
+ private:
+  double gsize_asked, grid_size;
+  int grid[3];
+  int analyse_count = 0;
+  std::vector<unsigned char> data;
+};
+
 // MDManager::SaveAsCdview / SaveAsCdviewSequential (mdmanager.cc:118-143):
 // the CDView file of a unit, and the numbered conf%04d.cd series
 void SaveAsCdview(MDUnit &unit, const char *filename);

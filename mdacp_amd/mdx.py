@@ -21,7 +21,7 @@ EXPORTS = (
     "mdx_update_position_half", "mdx_refresh_images", "mdx_calculate_force", "mdx_calculate",
     "mdx_observe", "mdx_forces", "mdx_pair_number", "mdx_pairs_canonical",
     "mdx_pairs_canonical_noalloc", "mdx_pairs_csr", "mdx_adjust_periodic_boundary",
-    "mdx_scale_positions", "mdx_set_box", "mdx_get_cell_order", "mdx_timing",
+    "mdx_scale_positions", "mdx_set_box", "mdx_get_cell_order", "mdx_density_grid", "mdx_timing",
     "mdx_timing_reset", "mdx_kick_profile", "mdx_lj_trajectory",
     "mdx_set_heatbath", "mdx_get_heatbath", "mdx_heatbath_zeta", "mdx_heatbath_momenta",
     "mdx_langevin",
@@ -109,6 +109,8 @@ def load() -> ctypes.CDLL:
         "mdx_scale_positions": (ctypes.c_int, [vp, ctypes.c_double]),
         "mdx_set_box": (ctypes.c_int, [vp, dp]),
         "mdx_get_cell_order": (ctypes.c_int, [vp, P(ctypes.c_int32)]),
+        "mdx_density_grid": (ctypes.c_int, [vp, ctypes.c_double, P(ctypes.c_int32),
+                                            P(ctypes.c_uint8), ctypes.c_int64]),
         "mdx_timing": (ctypes.c_int, [vp, dp, dp, dp, i64p, i64p]),
         "mdx_timing_reset": (ctypes.c_int, [vp, ctypes.c_int]),
         "mdx_kick_profile": (ctypes.c_int, [vp, dp, dp]),
@@ -299,6 +301,18 @@ class System:
         ids = np.zeros(self.n, dtype=np.int32)
         self._chk(self.lib.mdx_get_cell_order(self.h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return ids
+
+    def density_grid(self, grid_size: float):
+        """BubbleHist's counts per grid box (uint8, modulo 256) as an
+        array [gz][gy][gx] (mdx_density_grid)."""
+        dims = np.zeros(3, dtype=np.int32)
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        self._chk(self.lib.mdx_density_grid(self.h, grid_size, dims.ctypes.data_as(i32), None, 0))
+        out = np.zeros(int(np.prod(dims)), dtype=np.uint8)
+        self._chk(self.lib.mdx_density_grid(self.h, grid_size, dims.ctypes.data_as(i32),
+                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                            out.size))
+        return out.reshape(dims[2], dims[1], dims[0])
 
     def timing_reset(self, enable=True):
         self._chk(self.lib.mdx_timing_reset(self.h, 1 if enable else 0))

@@ -242,3 +242,24 @@ def bruteforce_forces(state):
     lib().orc_bruteforce_forces(ctypes.byref(prm), state.n, _dp(q), _dp(f), ctypes.byref(pe),
                                 ctypes.byref(vir))
     return f, pe.value, vir.value
+
+
+def density_grid(q, L, grid_size: float):
+    """BubbleHist of the Cavitation project for one unit at the origin
+    (cavitation.cc:38-57 for the grid, 117-141 for the counts): g_d =
+    int(L_d / grid_size + 0.5) boxes per axis, box width w = L_x / g_x on
+    every axis, particle i in box int(x / w) + int(y / w) g_x + int(z / w)
+    g_x g_y, counted in an unsigned char.  Returns (g, the g_x g_y g_z
+    bytes as SaveDensity writes them, cavitation.cc:85-90).  A box index
+    outside the grid raises, as the reference's assert stops."""
+    g = [int(float(L[d]) / grid_size + 0.5) for d in range(3)]
+    w = float(L[0]) / float(g[0])
+    inv = 1.0 / w
+    q = np.asarray(q, dtype=np.float64)
+    ix = np.trunc(q * inv).astype(np.int64)
+    k = ix[:, 0] + ix[:, 1] * g[0] + ix[:, 2] * g[0] * g[1]
+    total = g[0] * g[1] * g[2]
+    if (k < 0).any() or (k >= total).any():
+        raise OracleError("a particle lies outside the density grid")
+    counts = np.bincount(k, minlength=total)
+    return g, (counts & 0xFF).astype(np.uint8)
