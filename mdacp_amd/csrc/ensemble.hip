@@ -1702,7 +1702,8 @@ int mdx_ensemble_set_state(mdx_ensemble *e, const double *q3, const double *p3) 
 }
 
 int mdx_ensemble_get_state(mdx_ensemble *e, double *q3, double *p3) {
-  if (!e || !e->have_state) return efail(e, MDX_ERR_STATE, "no state");
+  if (!e) return efail(nullptr, MDX_ERR_ARG, "null ensemble");
+  if (!e->have_state) return efail(e, MDX_ERR_STATE, "no state");
   if (q3) {
     k_e_scatter_by_tag<<<nblk(e->NR), kBlock, 0, e->stream>>>(e->qe, e->tag, e->NR, e->io);
     EHIP(e, hipMemcpyAsync(q3, e->io, sizeof(double) * 3 * (size_t)e->NR, hipMemcpyDeviceToHost, e->stream));
@@ -1784,7 +1785,8 @@ int mdx_ensemble_status(mdx_ensemble *e, int *status, int *steps) {
 }
 
 int mdx_ensemble_observe(mdx_ensemble *e, double *ke, double *pe, double *vir) {
-  if (!e || !e->have_state) return efail(e, MDX_ERR_STATE, "no state");
+  if (!e) return efail(nullptr, MDX_ERR_ARG, "null ensemble");
+  if (!e->have_state) return efail(e, MDX_ERR_STATE, "no state");
   const EnsDev d = dev_view(e);
   if (e->tile == 64)
     k_e_observe<64><<<e->NT, 64, stage_bytes(e), e->stream>>>(d, e->C0, e->C2);

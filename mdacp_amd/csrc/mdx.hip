@@ -2810,7 +2810,8 @@ int mdx_set_state(mdx_system *s, int n, const double *q3, const double *p3) {
 }
 
 int mdx_get_state(mdx_system *s, double *q3, double *p3) {
-  if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no state");
+  if (!s) return fail(nullptr, MDX_ERR_ARG, "null system");
+  if (!s->have_state) return fail(s, MDX_ERR_STATE, "no state");
   if (q3) {
     k_scatter_by_tag<<<nblk(s->n), kBlock, 0, s->stream>>>(s->pos, s->tag, s->n, s->io_buf);
     HIPCHK(s, hipMemcpyAsync(q3, s->io_buf, sizeof(double) * 3 * s->n, hipMemcpyDeviceToHost, s->stream));
@@ -2844,7 +2845,8 @@ __global__ void k_image_rows(const double4 *__restrict__ pos, const double4 *__r
 }
 
 int mdx_get_images(mdx_system *s, double *q3, double *p3, int32_t *id, int32_t *code) {
-  if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no state");
+  if (!s) return fail(nullptr, MDX_ERR_ARG, "null system");
+  if (!s->have_state) return fail(s, MDX_ERR_STATE, "no state");
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist (no images yet)");
   // in chunks of the host-transfer buffers (cap particles)
   for (int k0 = 0; k0 < s->g; k0 += s->cap) {
@@ -2889,17 +2891,20 @@ int mdx_is_pairlist_expired(mdx_system *s, int *out) {
 }
 
 int mdx_update_position_half(mdx_system *s) {
-  if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no state");
+  if (!s) return fail(nullptr, MDX_ERR_ARG, "null system");
+  if (!s->have_state) return fail(s, MDX_ERR_STATE, "no state");
   return drift(s);
 }
 
 int mdx_refresh_images(mdx_system *s) {
-  if (!s || !s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
+  if (!s) return fail(nullptr, MDX_ERR_ARG, "null system");
+  if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
   return refresh(s);
 }
 
 int mdx_calculate_force(mdx_system *s) {
-  if (!s || !s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
+  if (!s) return fail(nullptr, MDX_ERR_ARG, "null system");
+  if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
   int rc = kick(s);
   if (rc) return rc;
   HIPCHK(s, hipStreamSynchronize(s->stream));
@@ -2947,17 +2952,20 @@ int mdx_heatbath_zeta(mdx_system *s, double current_temperature) {
 }
 
 int mdx_heatbath_momenta(mdx_system *s) {
-  if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no state");
+  if (!s) return fail(nullptr, MDX_ERR_ARG, "null system");
+  if (!s->have_state) return fail(s, MDX_ERR_STATE, "no state");
   return nh_scale(s);
 }
 
 int mdx_langevin(mdx_system *s) {
-  if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no state");
+  if (!s) return fail(nullptr, MDX_ERR_ARG, "null system");
+  if (!s->have_state) return fail(s, MDX_ERR_STATE, "no state");
   return langevin(s);
 }
 
 int mdx_calculate(mdx_system *s, int nsteps, int *rebuilds) {
-  if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no state");
+  if (!s) return fail(nullptr, MDX_ERR_ARG, "null system");
+  if (!s->have_state) return fail(s, MDX_ERR_STATE, "no state");
   int nreb = 0, rc;
   int done = 0;
   bool device_expired = false;   // the last batch ended on a device check that expired the list
@@ -3122,7 +3130,8 @@ int mdx_pairs_csr(mdx_system *s, int32_t *key_pointer, int32_t *number_of_partne
 }
 
 int mdx_adjust_periodic_boundary(mdx_system *s) {
-  if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no particles loaded");
+  if (!s) return fail(nullptr, MDX_ERR_ARG, "null system");
+  if (!s->have_state) return fail(s, MDX_ERR_STATE, "no particles loaded");
   HIPCHK(s, hipMemsetAsync(&s->dstat->flags, 0, sizeof(int), s->stream));
   k_wrap_reals<<<nblk(s->n), kBlock, 0, s->stream>>>(s->pos, s->n, s->geom, &s->dstat->flags);
   s->list_valid = false;   // reals may have moved across the box
@@ -3132,7 +3141,8 @@ int mdx_adjust_periodic_boundary(mdx_system *s) {
 }
 
 int mdx_scale_positions(mdx_system *s, double alpha) {
-  if (!s || !s->have_state) return fail(s, MDX_ERR_STATE, "no particles loaded");
+  if (!s) return fail(nullptr, MDX_ERR_ARG, "null system");
+  if (!s->have_state) return fail(s, MDX_ERR_STATE, "no particles loaded");
   if (!(alpha > 0.0) || !std::isfinite(alpha)) return fail(s, MDX_ERR_ARG, "scale %g", alpha);
   k_scale_reals<<<nblk(s->n), kBlock, 0, s->stream>>>(s->pos, s->n, alpha);
   s->list_valid = false;
