@@ -3274,16 +3274,18 @@ int mdx_kick_profile(mdx_system *s, double *avg_ms, double *bytes) {
   if (avg_ms) *avg_ms = s->n_kick_timed > 0 ? s->t_kick_only / (double)s->n_kick_timed : 0.0;
   if (bytes) {
     // Algorithmic bytes of one fused kick + half-drift launch, per real:
-    // read q and p, write p and q (24 B each), the partner count (4 B), and
-    // 4 B per half-list partner (each pair once, as the reference's half
-    // list); partner positions are re-reads of q and images re-reads of
-    // their sources, counted once in q.
+    // read q and p, write p and q (24 B each), the list's offset and count
+    // (4 B each: the CSR key_pointer and number_of_partners of the
+    // reference), and 4 B per half-list partner (each pair once, as the
+    // reference's half list); partner positions are re-reads of q and
+    // images re-reads of their sources, counted once in q.  That is 104 B
+    // per real plus 4 B per pair of the current list.
     int64_t pairs = 0;
     if (s->list_valid) {
       int rc = mdx_pair_number(s, &pairs);
       if (rc) return rc;
     }
-    *bytes = (double)s->n * (4.0 * 24.0 + 4.0) + 4.0 * (double)pairs;
+    *bytes = (double)s->n * (4.0 * 24.0 + 8.0) + 4.0 * (double)pairs;
   }
   return MDX_OK;
 }
