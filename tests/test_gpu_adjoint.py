@@ -11,7 +11,7 @@ import pytest
 
 from adjoint_reference import loss_and_grads
 from mdacp_amd import workloads as W
-from mdacp_amd.mdx import System
+from mdacp_amd.mdx import MdxError, System
 
 pytestmark = pytest.mark.gpu
 
@@ -58,6 +58,30 @@ def test_adjoint_vs_finite_differences_n32000():
         assert abs(g["dloss_dq0"][i, d] - fd) <= 1e-6 * scale + 1e-6 * abs(fd), (i, d, fd)
     # cost of the reverse sweep relative to the forward pass (recompute included)
     print(f"forward {g['forward_ms']:.1f} ms, backward {g['backward_ms']:.1f} ms", flush=True)
+
+
+def test_adjoint_at_larger_sigma_with_wider_search():
+    """sigma = 1.3 puts the cut at 3.9, past the default search length 3.3:
+    the system is created with buffer_length 1.5 (search length 4.5) and
+    the run matches the dense autograd reference at eps = 1.2, sigma = 1.3
+    (rho sigma^3 = 0.70).  With the default search length the call is
+    refused."""
+    s = W.fcc_state(5, 0.32, 1.0, 6)         # N=500, L=11.6
+    print(f"adjoint at sigma 1.3 N={s.n}", flush=True)
+    with System.from_state(s, buffer_length=1.5) as g:
+        r = g.lj_trajectory(eps=1.2, sigma=1.3, grad=True, **OPTS)
+    loss, ge, gs, gq, Pm, Em = loss_and_grads(s.q, s.p, s.L, eps=1.2, sigma=1.3, nsteps=200,
+                                              nobs=100, segment=20)
+    assert r["loss"] == pytest.approx(loss, rel=1e-10)
+    assert r["pressure_mean"] == pytest.approx(Pm, rel=1e-10)
+    assert r["pe_mean"] == pytest.approx(Em, rel=1e-10)
+    assert r["dloss_deps"] == pytest.approx(ge, rel=1e-8)
+    assert r["dloss_dsigma"] == pytest.approx(gs, rel=1e-8)
+    assert np.abs(r["dloss_dq0"] - gq).max() / np.abs(gq).max() < 1e-8
+    with System.from_state(s) as g:
+        with pytest.raises(MdxError) as ei:
+            g.lj_trajectory(eps=1.0, sigma=1.2, grad=False, **OPTS)
+        assert ei.value.code == -1
 
 
 def test_plain_potential_matches_kick():
