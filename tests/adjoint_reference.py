@@ -125,7 +125,8 @@ def loss_and_grads(q0, p0, L, eps=1.0, sigma=1.0, dt=0.001, nsteps=200, nobs=100
     Em = Esum / nobs
     loss = (Pm - p_target) ** 2 + (Em - pe_target) ** 2
     gq, ge, gs = torch.autograd.grad(loss, (qstart, e, sg))
-    return (float(loss), float(ge), float(gs), gq.numpy(), float(Pm), float(Em))
+    return (float(loss.detach()), float(ge), float(gs), gq.numpy(), float(Pm.detach()),
+            float(Em.detach()))
 
 
 def loss_only(q0, p0, L, eps=1.0, sigma=1.0, dt=0.001, nsteps=200, nobs=100,
