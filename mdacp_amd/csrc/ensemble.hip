@@ -84,7 +84,7 @@ struct RepCtl {
   int check;          // the budget is spent: a displacement check runs after this step
   int check_slot;     // its position in the check list
   int mid;            // mid-step buffer of the step just done
-  int pad;
+  int phase;          // 1 if flagged for a rebuild at the start of this step
   unsigned long long dmax_bits;   // max squared displacement of the running check (bits of a
                                   // non-negative double, which order as integers)
 };
@@ -92,9 +92,12 @@ struct RepCtl {
 struct EnsDev {
   const Rep *rep;
   RepCtl *ctl;
-  int *meta;      // meta[0]: replicas flagged for a rebuild in this step; meta[1]: checks
+  int *meta;      // meta[0]: replicas flagged for a rebuild in this step; meta[1]: checks of
+                  // the others; meta[2]: the flagged count of the last rebuild; meta[3]:
+                  // checks of the rebuilt ones
   int *flagged;   // their ids
-  int *checklist; // replicas that run a displacement check after this step
+  int *checklist; // [2R] replicas that run a displacement check after this step: those not
+                  // rebuilt in it first, the rebuilt ones from R on
   int R, NR, NI, NT;
   int T;          // tile size (reals per tile)
   int stage_cap;  // particles one plane of a tile may stage (dynamic LDS / 24 B)
@@ -126,9 +129,13 @@ struct EnsDev {
 // end to end, so that one launch covers every flagged particle, image, cell
 // or tile with all its workgroups: foff[kind][f] is the first index of the
 // f-th flagged replica (a replica stopped by a failure counts zero), and
-// the last entry the total.  OFF_CHECK is the same over the replicas whose
+// the last entry the total.  OFF_CHECK (OFF_CHECK2) is the same over the
+// replicas not rebuilt (rebuilt) in this step whose
 // displacement check runs in this step (reals and images).
-enum OffKind { OFF_REALS = 0, OFF_IMAGES = 1, OFF_CELLS = 2, OFF_TILES = 3, OFF_CHECK = 4, kOffKinds = 5 };
+enum OffKind {
+  OFF_REALS = 0, OFF_IMAGES = 1, OFF_CELLS = 2, OFF_TILES = 3, OFF_CHECK = 4, OFF_CHECK2 = 5,
+  kOffKinds = 6
+};
 
 __device__ __forceinline__ const int *offsets(const EnsDev &d, int kind) {
   return d.foff + (size_t)kind * (d.R + 1);
@@ -152,7 +159,20 @@ __device__ __forceinline__ bool rebuild_due(const RepCtl *c) {
   return c->need_rebuild && !c->error && c->steps < c->target;
 }
 
-__device__ __forceinline__ bool rebuilding(const RepCtl *c) { return rebuild_due(c) && !c->eflags; }
+// The expiry check runs in two groups that may overlap in time: group 1
+// the replicas not flagged for a rebuild at the start of the step, group 2
+// those flagged (c->phase, set by k_e_begin).  Each group has its own
+// count, list and offsets.
+__device__ __forceinline__ bool in_check_group(const RepCtl *c, int grp) {
+  return (c->phase != 0) == (grp == 2);
+}
+__device__ __forceinline__ int check_kind(int grp) { return grp == 2 ? OFF_CHECK2 : OFF_CHECK; }
+__device__ __forceinline__ int *check_count(const EnsDev &d, int kind) {
+  return d.meta + (kind == OFF_CHECK2 ? 3 : 1);
+}
+__device__ __forceinline__ int *check_list(const EnsDev &d, int kind) {
+  return d.checklist + (kind == OFF_CHECK2 ? d.R : 0);
+}
 
 __device__ __forceinline__ bool stepping(const RepCtl *c) {
   return !c->error && !c->need_rebuild && c->steps < c->target;
@@ -304,7 +324,11 @@ __device__ __forceinline__ SlotWalk slot_walk(int NT) {
 __global__ void k_e_begin(EnsDev d) {
   const int r = blockIdx.x;
   RepCtl *c = d.ctl + r;
-  if (!rebuild_due(c)) return;
+  const bool due = rebuild_due(c);
+  // the step kernels split the replicas by this flag: those not rebuilding
+  // step while the rebuild runs, the rebuilt ones after it
+  if (threadIdx.x == 0) c->phase = due ? 1 : 0;
+  if (!due) return;
   const Rep &rp = d.rep[r];
   for (int k = threadIdx.x; k < rp.ncell; k += blockDim.x) d.cell_count[rp.cbase + k] = 0;
   if (threadIdx.x == 0) {
@@ -318,9 +342,9 @@ __global__ void k_e_begin(EnsDev d) {
 __global__ void __launch_bounds__(kReplicaThreads) k_e_offsets(EnsDev d, int mask) {
   for (int kind = 0; kind < kOffKinds; kind++) {
     if (!(mask & (1 << kind))) continue;
-    const bool chk = kind == OFF_CHECK;
-    const int nf = chk ? d.meta[1] : d.meta[0];
-    const int *list = chk ? d.checklist : d.flagged;
+    const bool chk = kind == OFF_CHECK || kind == OFF_CHECK2;
+    const int nf = chk ? *check_count(d, kind) : d.meta[0];
+    const int *list = chk ? check_list(d, kind) : d.flagged;
     int *off = d.foff + (size_t)kind * (d.R + 1);
     int carry = 0;
     for (int base = 0; base < nf; base += blockDim.x) {
@@ -353,7 +377,7 @@ struct OffView {
   int nf, total;
 };
 __device__ __forceinline__ OffView load_offsets(const EnsDev &d, int kind, int *lds) {
-  const int nf = kind == OFF_CHECK ? d.meta[1] : d.meta[0];
+  const int nf = kind == OFF_CHECK || kind == OFF_CHECK2 ? *check_count(d, kind) : d.meta[0];
   const int *g = offsets(d, kind);
   OffView v{g, nf, g[nf]};
   if (nf + 1 <= kOffLds && v.total > 0) {
@@ -368,7 +392,7 @@ __device__ __forceinline__ OffView load_offsets(const EnsDev &d, int kind, int *
 // of replica r, every workgroup taking a share.  Every thread calls it.
 template <class Body>
 __device__ __forceinline__ void for_flagged(const EnsDev &d, int kind, int *lds, Body body) {
-  const int *list = kind == OFF_CHECK ? d.checklist : d.flagged;
+  const int *list = kind == OFF_CHECK || kind == OFF_CHECK2 ? check_list(d, kind) : d.flagged;
   const OffView v = load_offsets(d, kind, lds);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < v.total; t += gridDim.x * blockDim.x) {
     const int f = replica_at(v.off, v.nf, t);
@@ -417,7 +441,9 @@ __global__ void k_e_wrap(EnsDev d) {
 __global__ void __launch_bounds__(kReplicaThreads) k_e_img_scan(EnsDev d) {
   const int r = blockIdx.x;
   RepCtl *c = d.ctl + r;
-  if (!rebuilding(c)) return;
+  // the flag of k_e_begin, not need_rebuild: the check of the replicas not
+  // rebuilding may run at the same time and set theirs
+  if (!c->phase || c->eflags) return;
   const Rep &rp = d.rep[r];
   // kScanPer consecutive reals per thread: a block scan per kScanPer x 1024
   int carry = 0;
@@ -499,7 +525,7 @@ __global__ void k_e_images(EnsDev d) {
 __global__ void __launch_bounds__(kReplicaThreads) k_e_cells(EnsDev d) {
   const int r = blockIdx.x;
   RepCtl *c = d.ctl + r;
-  if (!rebuilding(c)) return;
+  if (!c->phase || c->eflags) return;   // as k_e_img_scan
   const Rep &rp = d.rep[r];
   const Geom &g = rp.g;
   int carry_in = 0, carry_out = 0;
@@ -745,10 +771,13 @@ __global__ void __launch_bounds__(T) k_e_build(EnsDev d) {
 // close the rebuild: longest list against the capacity, then the reset of
 // the velocity budget (pairlist.cc:11-23)
 __global__ void k_e_end(EnsDev d) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) d.meta[0] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    d.meta[2] = d.meta[0];   // the flagged count, for the step of the rebuilt replicas
+    d.meta[0] = 0;
+  }
   const int r = blockIdx.x;
   RepCtl *c = d.ctl + r;
-  if (!rebuild_due(c)) return;
+  if (!c->phase) return;   // as k_e_img_scan
   const Rep &rp = d.rep[r];
   int km = 0;
   if (!c->eflags)
@@ -778,17 +807,20 @@ __global__ void k_e_end(EnsDev d) {
 // q_end + p dt/2 into the other mid-step buffer.  The arithmetic is the
 // single-system kick's (kick_term, kILP chains, drift_half).
 template <int T>
-__global__ void __launch_bounds__(T) k_e_step(EnsDev d) {
+__global__ void __launch_bounds__(T) k_e_step(EnsDev d, int mode) {
   extern __shared__ double stage_mem[];
   __shared__ TileSmem sm;
-  const SlotWalk w = slot_walk(d.NT);
   const double dt2 = d.dt * 0.5;
-  for (int slot = w.s; slot < w.end; slot += w.stride) {
+  auto tile_step = [&](int slot) {
     const TileDesc t = d.tiles[slot];
-    if (t.ni == 0) continue;
+    if (t.ni == 0) return;
     const int r = t.pad;
     const RepCtl *c = d.ctl + r;
-    if (!stepping(c)) continue;
+    // mode 1: only replicas not rebuilding in this step (they run beside
+    // the rebuild, whose kernels touch only the rebuilding replicas);
+    // mode 2: only the replicas rebuilt in this step
+    if (mode != 0 && (c->phase != 0) != (mode == 2)) return;
+    if (!stepping(c)) return;
     const Rep &rp = d.rep[r];
     const double4 *qh = c->par ? d.qh1 : d.qh0;
     double4 *qn = c->par ? d.qh0 : d.qh1;
@@ -843,6 +875,20 @@ __global__ void __launch_bounds__(T) k_e_step(EnsDev d) {
     v2 = T == 64 ? wave_max(v2) : block_max(v2);
     if (threadIdx.x == 0) d.partial[slot] = v2;
     __syncthreads();
+  };
+  if (mode == 2) {
+    // the tiles of the replicas rebuilt in this step, as the list build
+    // spreads them (meta[2]: how many were flagged)
+    const int nf = d.meta[2];
+    const int *off = offsets(d, OFF_TILES);
+    const int total = off[nf];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+      const int f = replica_at(off, nf, item);
+      tile_step(d.rep[d.flagged[f]].tbase + item - off[f]);
+    }
+  } else {
+    const SlotWalk w = slot_walk(d.NT);
+    for (int slot = w.s; slot < w.end; slot += w.stride) tile_step(slot);
   }
 }
 
@@ -860,10 +906,11 @@ __global__ void __launch_bounds__(T) k_e_step(EnsDev d) {
 // Images enter at their mid-step positions, as the reference's ghosts do
 // after a step.  The budget arithmetic is rounded operation by operation as
 // the host does it, and sqrt is the correctly rounded one.
-__global__ void k_e_budget(EnsDev d) {
+__global__ void k_e_budget(EnsDev d, int grp) {
 #pragma clang fp contract(off)
   const int r = blockIdx.x;
   RepCtl *c = d.ctl + r;
+  if (!in_check_group(c, grp)) return;
   if (!stepping(c)) {
     // no step, no check: the later kernels of this step leave it alone
     __syncthreads();
@@ -899,9 +946,10 @@ __global__ void k_e_budget(EnsDev d) {
     c->check = 1;
     c->cand = 0;
     c->dmax_bits = 0ull;
-    const int f = atomicAdd(d.meta + 1, 1);
+    const int kind = check_kind(grp);
+    const int f = atomicAdd(check_count(d, kind), 1);
     c->check_slot = f;
-    d.checklist[f] = r;
+    check_list(d, kind)[f] = r;
   }
 }
 
@@ -917,9 +965,11 @@ __device__ __forceinline__ int check_slot_of(const EnsDev &d, const Rep &rp, int
 // displacement of every real and image of the checked replicas; the maximum
 // per replica goes through one 64-bit atomic max per replica a wavefront
 // touches (a maximum does not depend on the order it is taken in)
-__global__ void k_e_disp(EnsDev d) {
+__global__ void k_e_disp(EnsDev d, int grp) {
   __shared__ int soff[kOffLds];
-  const OffView v = load_offsets(d, OFF_CHECK, soff);
+  const int kind = check_kind(grp);
+  const int *list = check_list(d, kind);
+  const OffView v = load_offsets(d, kind, soff);
   const int *off = v.off;
   const int nf = v.nf, total = v.total;
   const int lane = threadIdx.x & 63;
@@ -932,7 +982,7 @@ __global__ void k_e_disp(EnsDev d) {
     double d2 = 0.0;
     if (t < total) {
       const int f = replica_at(off, nf, t);
-      r = d.checklist[f];
+      r = list[f];
       const RepCtl *c = d.ctl + r;
       const Rep &rp = d.rep[r];
       const double4 *mid = c->mid ? d.qh1 : d.qh0;
@@ -953,12 +1003,12 @@ __global__ void k_e_disp(EnsDev d) {
   }
 }
 
-__global__ void k_e_dfold(EnsDev d) {
+__global__ void k_e_dfold(EnsDev d, int grp) {
 #pragma clang fp contract(off)
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= d.R) return;
   RepCtl *c = d.ctl + r;
-  if (c->error || !c->check) return;
+  if (!in_check_group(c, grp) || c->error || !c->check) return;
   const double dmax2 = __longlong_as_double((long long)c->dmax_bits);
   {
     const double c_distance = d.buffer - __dsqrt_rn(dmax2);
@@ -966,25 +1016,25 @@ __global__ void k_e_dfold(EnsDev d) {
   }
 }
 
-__global__ void k_e_cand(EnsDev d) {
+__global__ void k_e_cand(EnsDev d, int grp) {
   __shared__ int soff[kOffLds];
-  for_flagged(d, OFF_CHECK, soff, [&](int r, int k) {
+  for_flagged(d, check_kind(grp), soff, [&](int r, int k) {
     RepCtl *c = d.ctl + r;
     const int sl = check_slot_of(d, d.rep[r], k);
     if (d.disp[sl] > c->cd2) {
       const int at = atomicAdd(&c->cand, 1);
-      if (at < kMaxCand) d.cand_buf[(size_t)c->check_slot * kMaxCand + at] = sl;
+      if (at < kMaxCand) d.cand_buf[(size_t)r * kMaxCand + at] = sl;
     }
   });
 }
 
-__global__ void __launch_bounds__(kReplicaThreads) k_e_decide(EnsDev d) {
+__global__ void __launch_bounds__(kReplicaThreads) k_e_decide(EnsDev d, int grp) {
   __shared__ int hit;
   __shared__ double old_q[3 * kMaxCand], now_q[3 * kMaxCand];
-  if (blockIdx.x == 0 && threadIdx.x == 0) d.meta[1] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *check_count(d, check_kind(grp)) = 0;
   const int r = blockIdx.x;
   RepCtl *c = d.ctl + r;
-  if (c->error || !c->check) return;
+  if (!in_check_group(c, grp) || c->error || !c->check) return;
   const Rep &rp = d.rep[r];
   const int tn = rp.n + c->g;
   const int Nl = tn > d.check_list_length ? d.check_list_length : tn;
@@ -996,7 +1046,7 @@ __global__ void __launch_bounds__(kReplicaThreads) k_e_decide(EnsDev d) {
     // the candidates' build-time and present positions, staged in LDS for
     // the pair test (l < Nl <= kMaxCand)
     const double4 *mid = c->mid ? d.qh1 : d.qh0;
-    const int *cand = d.cand_buf + (size_t)c->check_slot * kMaxCand;
+    const int *cand = d.cand_buf + (size_t)r * kMaxCand;
     if (threadIdx.x == 0) hit = 0;
     for (int a = threadIdx.x; a < l; a += blockDim.x) {
       const int sl = cand[a];
@@ -1149,6 +1199,10 @@ struct mdx_ensemble {
   mdx_params prm{};
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // second stream of a step: the replicas not rebuilding step on it while
+  // the rebuild runs on `stream` (default priority, like every stream here)
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   double SL = 0, C0 = 0, C2 = 0;
   int R = 0, NR = 0, NI = 0, NC = 0, NT = 0;
   int tile = 128;          // reals per tile
@@ -1356,10 +1410,9 @@ int set_stage_cap(mdx_ensemble *e, int cap) {
   return MDX_OK;
 }
 
+// the masked rebuild after k_e_begin
 template <int T>
-int enqueue_rebuild_t(mdx_ensemble *e) {
-  const EnsDev d = dev_view(e);
-  k_e_begin<<<e->R, kBlock, 0, e->stream>>>(d);
+void enqueue_rebuild_body(mdx_ensemble *e, const EnsDev &d) {
   k_e_offsets<<<1, kReplicaThreads, 0, e->stream>>>(d, 1 << OFF_REALS);
   k_e_wrap<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
   k_e_img_scan<<<e->R, kReplicaThreads, 0, e->stream>>>(d);
@@ -1373,21 +1426,50 @@ int enqueue_rebuild_t(mdx_ensemble *e) {
   k_e_gather_images<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
   k_e_build<T><<<e->step_grid, T, stage_bytes(e), e->stream>>>(d);
   k_e_end<<<e->R, kBlock, 0, e->stream>>>(d);
-  return launch_check(e, "rebuild");
 }
 
 template <int T>
-int enqueue_step_t(mdx_ensemble *e) {
-  int rc = enqueue_rebuild_t<T>(e);
-  if (rc) return rc;
+int enqueue_rebuild_t(mdx_ensemble *e) {
   const EnsDev d = dev_view(e);
-  k_e_step<T><<<e->step_grid, T, stage_bytes(e), e->stream>>>(d);
-  k_e_budget<<<e->R, kBlock, 0, e->stream>>>(d);
-  k_e_offsets<<<1, kReplicaThreads, 0, e->stream>>>(d, 1 << OFF_CHECK);
-  k_e_disp<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
-  k_e_dfold<<<(e->R + kBlock - 1) / kBlock, kBlock, 0, e->stream>>>(d);
-  k_e_cand<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
-  k_e_decide<<<e->R, kReplicaThreads, 0, e->stream>>>(d);
+  k_e_begin<<<e->R, kBlock, 0, e->stream>>>(d);
+  enqueue_rebuild_body<T>(e, d);
+  return launch_check(e, "rebuild");
+}
+
+// One step: the masked rebuild on `stream`, the step of the replicas not
+// rebuilding on `side` at the same time (forked after k_e_begin has flagged
+// the replicas), the step of the rebuilt replicas after the rebuild, then
+// (joined) the expiry check of all.  Each replica's kernels run in the same
+// order as one after another, so its arithmetic is unchanged.
+template <int T>
+void enqueue_check(mdx_ensemble *e, const EnsDev &d, hipStream_t st, int grp) {
+  k_e_budget<<<e->R, kBlock, 0, st>>>(d, grp);
+  k_e_offsets<<<1, kReplicaThreads, 0, st>>>(d, 1 << (grp == 2 ? OFF_CHECK2 : OFF_CHECK));
+  k_e_disp<<<kRebuildBlocks, kBlock, 0, st>>>(d, grp);
+  k_e_dfold<<<(e->R + kBlock - 1) / kBlock, kBlock, 0, st>>>(d, grp);
+  k_e_cand<<<kRebuildBlocks, kBlock, 0, st>>>(d, grp);
+  k_e_decide<<<e->R, kReplicaThreads, 0, st>>>(d, grp);
+}
+
+// One step: k_e_begin flags the replicas to rebuild; then on `side` the
+// others step and run their expiry check, while on `stream` the flagged
+// ones rebuild, step and run theirs; the two join before the next step.
+// Each replica's kernels run in the same order as one after another, so
+// its arithmetic is unchanged; the two branches touch disjoint replicas and
+// keep their own check lists.
+template <int T>
+int enqueue_step_t(mdx_ensemble *e) {
+  const EnsDev d = dev_view(e);
+  k_e_begin<<<e->R, kBlock, 0, e->stream>>>(d);
+  EHIP(e, hipEventRecord(e->fork, e->stream));
+  EHIP(e, hipStreamWaitEvent(e->side, e->fork, 0));
+  k_e_step<T><<<e->step_grid, T, stage_bytes(e), e->side>>>(d, 1);
+  enqueue_check<T>(e, d, e->side, 1);
+  EHIP(e, hipEventRecord(e->join, e->side));
+  enqueue_rebuild_body<T>(e, d);
+  k_e_step<T><<<e->step_grid, T, stage_bytes(e), e->stream>>>(d, 2);
+  enqueue_check<T>(e, d, e->stream, 2);
+  EHIP(e, hipStreamWaitEvent(e->stream, e->join, 0));
   return launch_check(e, "step");
 }
 
@@ -1611,10 +1693,14 @@ int mdx_ensemble_create(const mdx_params *prm, int nrep, const int *n, const dou
   }
   for (auto &ev : e->ev)
     if (!rc && hipEventCreate(&ev) != hipSuccess) rc = efail(e, MDX_ERR_HIP, "hipEventCreate failed");
+  if (!rc && (hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
+              hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&e->join, hipEventDisableTiming) != hipSuccess))
+    rc = efail(e, MDX_ERR_HIP, "second stream or its events");
   const size_t tot = (size_t)NR + NI;
   if (!rc && ((rc = ealloc(e, &e->rep, nrep)) || (rc = ealloc(e, &e->ctl, nrep)) ||
               (rc = ealloc(e, &e->meta, 4)) || (rc = ealloc(e, &e->flagged, nrep)) ||
-              (rc = ealloc(e, &e->checklist, nrep)) || (rc = ealloc(e, &e->disp, tot)) ||
+              (rc = ealloc(e, &e->checklist, 2 * (size_t)nrep)) || (rc = ealloc(e, &e->disp, tot)) ||
               (rc = ealloc(e, &e->foff, (size_t)kOffKinds * (nrep + 1))) ||
               (rc = ealloc(e, &e->cand_buf, (size_t)nrep * kMaxCand)) ||
               (rc = ealloc(e, &e->qe, tot)) || (rc = ealloc(e, &e->ref, tot)) ||
@@ -1673,6 +1759,9 @@ int mdx_ensemble_destroy(mdx_ensemble *e) {
     if (b) (void)hipFree(b);
   for (auto &ev : e->ev)
     if (ev) (void)hipEventDestroy(ev);
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  if (e->join) (void)hipEventDestroy(e->join);
+  if (e->side) (void)hipStreamDestroy(e->side);
   if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return MDX_OK;
