@@ -210,8 +210,9 @@ int mdx_set_box(mdx_system *sys, const double *L3);
  * reference's assert). */
 int mdx_density_grid(mdx_system *sys, double grid_size, int32_t *dims, uint8_t *data, int64_t cap);
 /* The ids of the real particles in the order the device keeps them (cell
- * order of the last pairlist build), pn entries: the permutation
- * MeshList::Sort applies to Variables (meshlist.cc:140-173). */
+ * order of the last pairlist build, by x inside a cell where the reference
+ * keeps index order), pn entries: the permutation MeshList::Sort applies
+ * to Variables (meshlist.cc:140-173). */
 int mdx_get_cell_order(mdx_system *sys, int32_t *ids);
 
 /* Allocation-free variant of mdx_pairs_canonical.  No counterpart in the

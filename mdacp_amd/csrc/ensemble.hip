@@ -646,17 +646,19 @@ __global__ void k_e_scatter(EnsDev d) {
   for_flagged(d, OFF_IMAGES, soff, [&](int r, int k) { place(d.NR + d.rep[r].ibase + k); });
 }
 
-// atomics fixed the cell, not the order in it: sort each cell by pre-sort
-// key, which makes the result a stable counting sort
+// atomics fixed the cell, not the order in it: sort each cell by (x,
+// pre-sort key), the single-system order (rows sorted by x as a whole)
 __global__ void k_e_cell_sort(EnsDev d) {
   __shared__ int buf[kBlock / 64][kSortBuf];
+  __shared__ double xbuf[kBlock / 64][kSortBuf];
   __shared__ int soff[kOffLds];
   const OffView v = load_offsets(d, OFF_CELLS, soff);
   const int w = threadIdx.x >> 6, nw = gridDim.x * (kBlock / 64);
+  const auto xof = [&](int k) { return k < d.NR ? d.pos_tmp[k].x : d.gpos_tmp[k - d.NR].x; };
   for (int t = blockIdx.x * (kBlock / 64) + w; t < v.total; t += nw) {
     const int f = replica_at(v.off, v.nf, t);   // the same for the whole wavefront
     const int cl = d.rep[d.flagged[f]].cbase + (t - v.off[f]);
-    wave_rank_sort(d.order + d.cell_start[cl], d.cell_count[cl], buf[w]);
+    wave_rank_sort_x(d.order + d.cell_start[cl], d.cell_count[cl], buf[w], xbuf[w], xof);
   }
 }
 

@@ -518,14 +518,18 @@ __global__ void k_scatter(const int *__restrict__ cellid, const int *__restrict_
 }
 
 // atomics fixed the cell, not the order inside it: sort each cell's
-// segment by pre-sort index so the result equals a stable counting sort
-// (one wavefront per cell)
+// segment by (x, pre-sort index), which leaves every cell row sorted by x
+// (the list build searches it), deterministically (one wavefront per cell);
+// pre-sort index k < n is real k (pos), else image k - n (gpos)
 __global__ void k_cell_sort(int *__restrict__ order, const int *__restrict__ cell_start,
-                            const int *__restrict__ cell_count, int ncell) {
+                            const int *__restrict__ cell_count, int ncell, int n,
+                            const double4 *__restrict__ pos, const double4 *__restrict__ gpos) {
   __shared__ int buf[kBlock / 64][kSortBuf];
+  __shared__ double xbuf[kBlock / 64][kSortBuf];
   const int w = threadIdx.x >> 6;
+  const auto xof = [&](int k) { return k < n ? pos[k].x : gpos[k - n].x; };
   for (int c = blockIdx.x * (kBlock / 64) + w; c < ncell; c += gridDim.x * (kBlock / 64))
-    wave_rank_sort(order + cell_start[c], cell_count[c], buf[w]);
+    wave_rank_sort_x(order + cell_start[c], cell_count[c], buf[w], xbuf[w], xof);
 }
 
 __global__ void k_gather_reals(const int *__restrict__ order, int n,
@@ -762,6 +766,41 @@ __device__ __forceinline__ void range_partners(const double *s3, int pb, const d
   }
 }
 
+// First candidate in [lo, hi) of the plane staged at s3 (candidate o at
+// 3 (o - pb)) whose x is >= v (above: > v); the range is sorted by x.
+template <bool ABOVE>
+__device__ __forceinline__ int x_bound(const double *s3, int pb, int lo, int hi, double v) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const double x = s3[3 * (mid - pb)];
+    if (ABOVE ? x <= v : x < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// The x window of the cell row (dy, dz) around a real at qi of cell row
+// (y, z): every candidate of that row with list_r2 <= SL2 has |x - qi.x| <=
+// h.  gy, gz are lower bounds of its distance to the row in y and z (the
+// cell's faces (c - 1) w, c w, less eps for the rounding of the faces and of
+// the binning), and the slack 1e-10 SL2 covers the roundings of r^2, so h
+// is only ever too wide.  Returns false when no candidate of the row can be
+// in range.
+__device__ __forceinline__ bool row_window(const Geom &g, const double4 &qi, int y, int z,
+                                           int dy, int dz, double eps, double *h) {
+  double gy = 0.0, gz = 0.0;
+  if (dy < 0) gy = qi.y - (double)(y - 1) / g.inv_w[1];
+  if (dy > 0) gy = (double)y / g.inv_w[1] - qi.y;
+  if (dz < 0) gz = qi.z - (double)(z - 1) / g.inv_w[2];
+  if (dz > 0) gz = (double)z / g.inv_w[2] - qi.z;
+  gy = fmax(gy - eps, 0.0);
+  gz = fmax(gz - eps, 0.0);
+  const double h2 = g.SL2 * (1.0 + 1e-10) - gy * gy - gz * gz;
+  if (h2 < 0.0) return false;
+  *h = sqrt(h2) + eps;
+  return true;
+}
+
 // Full neighbour list per tile: one list column per real, every partner
 // j != i (real or image) of its 27 cells with r^2 <= SL^2, r^2 formed as
 // the reference's AVX2 search, fma(dx,dx, fma(dy,dy, dz*dz)) with dq = q_j
@@ -803,6 +842,8 @@ __global__ void __launch_bounds__(kTile) k_build_tiles(
   }
   const int self = act ? sm.seg_base[8] + (i - sm.seg_start[8]) : -1;   // in row (0, 0)
   const int cx = interior_coord(qi.x, g.inv_w[0], g.ms[0]);
+  // rounding allowance of the cell faces and the binning, far above it
+  const double eps = 1e-9 * (g.L[0] + g.L[1] + g.L[2]);
   GroupWriter<kTile> gw(list + (size_t)tile * kcap * kTile + (size_t)col * kILP, kcap);
   __shared__ unsigned short tab[18 * kCellTab];
   const bool have_tab = tile_cell_table(g, c, n, t, sm, tab);
@@ -815,13 +856,22 @@ __global__ void __launch_bounds__(kTile) k_build_tiles(
       // the real's three cells of each row from the table: the image cell
       // before them, the real cells, the image cells after (only the end
       // cells of a real row can be image cells, all cells of an image row
-      // are): the same candidates in the same order as cell by cell
+      // are), each range sorted by x and cut to the row's x window: the
+      // same partners in the same order as cell by cell
       const int k0 = cx - t.xf;   // cell cx - 1
       for (int r = 3 * p; r < 3 * p + 3; r++) {
+        double h;
+        if (!row_window(g, qi, t.y, t.z, r % 3 - 1, r / 3 - 1, eps, &h)) continue;
+        const double xlo = qi.x - h, xhi = qi.x + h;
         const unsigned short *ti = tab + (2 * r) * kCellTab, *to = tab + (2 * r + 1) * kCellTab;
-        range_partners(stage_mem, pb, qi, g.SL2, to[k0], to[k0 + 1], self, gw);
-        range_partners(stage_mem, pb, qi, g.SL2, ti[k0], ti[k0 + 3], self, gw);
-        range_partners(stage_mem, pb, qi, g.SL2, to[k0 + 1], to[k0 + 3], self, gw);
+        const auto window = [&](int a, int b) {
+          const int lo = x_bound<false>(stage_mem, pb, a, b, xlo);
+          range_partners(stage_mem, pb, qi, g.SL2, lo, x_bound<true>(stage_mem, pb, lo, b, xhi),
+                         self, gw);
+        };
+        window(to[k0], to[k0 + 1]);
+        window(ti[k0], ti[k0 + 3]);
+        window(to[k0 + 1], to[k0 + 3]);
       }
       gw.pad_plane(sm.seg_base[6 * p + 6] - pb);
     } else if (act) {
@@ -2123,7 +2173,7 @@ int build_pairlist(mdx_system *s) {
     k_scatter<<<nblk(g), kBlock, 0, s->stream>>>(s->cellid + n, s->slot + n, s->cell_start, g, n,
                                                   s->order);
   k_cell_sort<<<std::min(4096, (s->ncell + 3) / 4), kBlock, 0, s->stream>>>(
-      s->order, s->cell_start, s->cell_count, s->ncell);
+      s->order, s->cell_start, s->cell_count, s->ncell, n, s->pos, s->gpos_tmp);
   // 4. gather into cell order, from pos / mom / tag into their spare
   // buffers, which then take their place (no copy of the pre-sort state)
   k_gather_reals<<<nblk(n), kBlock, 0, s->stream>>>(s->order, n, s->pos, s->mom, s->tag,

@@ -225,7 +225,12 @@ template <int S, class Term>
 __device__ __forceinline__ void plane_groups(const unsigned short *__restrict__ lst, int kb, int ke,
                                              const double *s3, Term term) {
   if (kb >= ke) return;
-  auto load = [&](int k) { return *reinterpret_cast<const uint4 *>(lst + list_index<S>(k)); };
+  // every k loaded is a group start (a multiple of kILP, as kb and ke are),
+  // where list_index<S>(k) = k S: one unsigned 32-bit byte offset
+  auto load = [&](int k) {
+    const unsigned off = (unsigned)k * (unsigned)(S * sizeof(unsigned short));
+    return *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(lst) + off);
+  };
   auto entry = [&](const uint4 &w, int u) {
     const unsigned h = u < 2 ? w.x : (u < 4 ? w.y : (u < 6 ? w.z : w.w));
     return staged_at(s3, (h >> (16 * (u % 2))) & 0xffffu);
@@ -371,21 +376,32 @@ __device__ __forceinline__ void observe_term(double r2, double C0, double C2, do
   vir += ((24.0 * r6 - 48.0) / (r6 * r6 * r2) + C2 * 8.0) * r2;
 }
 
-// Sort one cell's distinct keys ascending in place, one wavefront per cell:
-// the keys go to LDS, and each key is written to its rank (the number of
-// smaller keys).  The wave's lanes run in lockstep, so the LDS copy is
-// complete before any lane reads it back; cells larger than the LDS buffer
-// (not met in practice) fall back to an insertion sort by one lane.
+// Sort one cell's entries in place by (x, entry), one wavefront per cell,
+// x = xof(entry) the x coordinate of the particle an entry names: entries
+// and their x go to LDS, and each entry is written to its rank (the number
+// of entries before it in that order).  Cells are monotonic in x (the cell
+// of x is floor(x / w), and floor of a rounded product is monotonic; an
+// image cell holds x - L < 0 or x + L > L), so a row of cells sorted this
+// way is sorted by x as a whole.  The wave's lanes run in lockstep, so the
+// LDS copy is complete before any lane reads it back; cells larger than the
+// LDS buffer (not met in practice) fall back to an insertion sort by one
+// lane.
 constexpr int kSortBuf = 256;
-__device__ __forceinline__ void wave_rank_sort(int *seg, int len, int *buf) {
+__device__ __forceinline__ bool x_before(double xa, int a, double xb, int b) {
+  return xa < xb || (xa == xb && a < b);
+}
+template <class X>
+__device__ __forceinline__ void wave_rank_sort_x(int *seg, int len, int *buf, double *xb,
+                                                 X xof) {
   const int lane = threadIdx.x & 63;
   if (len <= 1) return;
   if (len > kSortBuf) {
     if (lane == 0) {
       for (int a = 1; a < len; a++) {
         const int v = seg[a];
+        const double xv = xof(v);
         int b = a - 1;
-        while (b >= 0 && seg[b] > v) {
+        while (b >= 0 && x_before(xv, v, xof(seg[b]), seg[b])) {
           seg[b + 1] = seg[b];
           b--;
         }
@@ -394,14 +410,18 @@ __device__ __forceinline__ void wave_rank_sort(int *seg, int len, int *buf) {
     }
     return;
   }
-  for (int a = lane; a < len; a += 64) buf[a] = seg[a];
+  for (int a = lane; a < len; a += 64) {
+    buf[a] = seg[a];
+    xb[a] = xof(seg[a]);
+  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   for (int a = lane; a < len; a += 64) {
     const int v = buf[a];
+    const double xv = xb[a];
     int rank = 0;
-    for (int b = 0; b < len; b++) rank += buf[b] < v ? 1 : 0;
+    for (int b = 0; b < len; b++) rank += x_before(xb[b], buf[b], xv, v) ? 1 : 0;
     seg[rank] = v;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

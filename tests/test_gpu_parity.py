@@ -357,6 +357,98 @@ def test_edge_cases_vs_oracle(case):
         assert np.abs(d).max() < 1e-10 and np.abs(pg - po).max() < 1e-9
 
 
+def _lowest_in_next_cell(c, inv):
+    """The smallest double that bins into cell c + 1 (1-based cells, bin =
+    floor(x inv) + 1): it may lie just below the face c / inv."""
+    v = c / inv
+    while np.floor(v * inv) < c:
+        v = np.nextafter(v, np.inf)
+    while np.floor(np.nextafter(v, -np.inf) * inv) >= c:
+        v = np.nextafter(v, -np.inf)
+    return v
+
+
+def _face_pairs_state(seed=96, per_step=60):
+    """Pairs at the search length whose partner sits on the y and z faces of
+    its cell: partner j at (x0 +- c, yf, zf) with yf, zf the lowest positions
+    binned into the next cells, real i at (x0, yf - a, zf - b), c =
+    sqrt(SL^2 - dy^2 - dz^2) moved by -3 .. 3 ulp, so r^2 falls on both
+    sides of SL^2 and j lies in a row above i's (or the same row when a or b
+    is 0), where the build's x window around i is at its narrowest.  A box
+    whose cell width L / ms is not a binary fraction."""
+    rng = np.random.default_rng(seed)
+    L, SL = 25.0, 3.0 + 0.3
+    ms = int(np.floor(L / SL))
+    inv = ms / L
+    q = []
+    for k in range(-3, 4):
+        for _ in range(per_step):
+            cy, cz = rng.integers(1, ms, size=2)
+            yf, zf = _lowest_in_next_cell(cy, inv), _lowest_in_next_cell(cz, inv)
+            a, b = rng.uniform(0.0, 3.2, size=2) * rng.integers(0, 2, size=2)
+            if a * a + b * b > 0.95 * SL * SL:
+                a, b = 0.5 * a, 0.5 * b
+            yi, zi = yf - a, zf - b
+            c = np.sqrt(SL * SL - (yf - yi) ** 2 - (zf - zi) ** 2)
+            c = c + k * np.spacing(c)
+            x0 = rng.uniform(SL + 0.1, L - SL - 0.1)
+            q.append([x0, yi, zi])
+            q.append([x0 + rng.choice([-1.0, 1.0]) * c, yf, zf])
+    q = np.array(q)
+    p = rng.normal(0, 0.5, size=q.shape)
+    p -= p.mean(axis=0)
+    return W.State(q, p, np.array([L, L, L]), 0.001)
+
+
+def _windows_without_margins_miss(s, pairs):
+    """How many listed pairs (i, j) of the same image (code 13) an x window
+    with no rounding allowance would drop: |x_j - x_i| <= sqrt(SL^2 - gy^2 -
+    gz^2), gy, gz the distances from i to the faces of j's row.  Run on the
+    CPU, it shows that the case is sharp enough to catch a window that is
+    too narrow."""
+    SL = 3.0 + 0.3
+    ms = np.floor(s.L / SL).astype(int)
+    inv = ms / s.L
+
+    def cell(x, d):
+        return min(max(int(np.floor(x * inv[d])) + 1, 1), ms[d])
+
+    miss = 0
+    for lo, hi, code in pairs.tolist():
+        if code != 13:
+            continue
+        for i, j in ((lo, hi), (hi, lo)):
+            qi, qj = s.q[i], s.q[j]
+            g2 = 0.0
+            for d in (1, 2):
+                ci, cj = cell(qi[d], d), cell(qj[d], d)
+                if cj < ci:
+                    g2 += (qi[d] - (ci - 1) / inv[d]) ** 2
+                elif cj > ci:
+                    g2 += (ci / inv[d] - qi[d]) ** 2
+            h2 = SL * SL - g2
+            miss += not (h2 >= 0 and abs(qj[0] - qi[0]) <= np.sqrt(h2))
+    return miss
+
+
+def test_pairs_at_search_length_on_cell_faces():
+    """The build cuts each cell row of a real's 27 cells to the x window that
+    its y-z distance to the row allows, widened for rounding; pairs at the
+    search length against the cell faces are listed exactly as the oracle
+    lists them, including those a window without that allowance drops."""
+    s = _face_pairs_state()
+    o = O.Oracle(s)
+    o.make_pairlist()
+    op = o.pairs()
+    own = {(2 * m, 2 * m + 1) for m in range(s.n // 2)}
+    listed = sum(1 for lo, hi, _ in op.tolist() if (lo, hi) in own)
+    assert 0 < listed < s.n // 2        # built pairs on both sides of SL^2
+    assert _windows_without_margins_miss(s, op) > 0
+    with System.from_state(s) as g:
+        g.make_pairlist()
+        np.testing.assert_array_equal(g.pairs(), op)
+
+
 def test_rejected_inputs():
     """No particles, and a box narrower than two search lengths, are refused
     with an error (the reference needs a domain of at least that width for
