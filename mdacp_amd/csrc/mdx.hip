@@ -7,9 +7,11 @@
 //   gsrc  int[g], gcode   image -> its real (cell-ordered index) and shift
 //   scan_in/cnt_in, scan_out/cnt_out over the (ms+2)^3 padded cell grid:
 //         interior cells hold reals only, the outer layer images only
-//   tiles TileDesc[]      <= kTile consecutive reals of one cell row
-//   list16 u16[tile][K][kTile]  full neighbour list of every real (partners
-//         real or image), as 3 x the partner's index in its tile's LDS
+//   tiles BlkTile[]       <= kBlk reals of a block of 3 x 3 cell rows, with
+//         the slots of their staged partners in sidx and the real of each
+//         list column in treal
+//   list16 u16[tile][K][kBlk]  full neighbour list of every real (partners
+//         real or image), as 24 x the partner's index in its tile's LDS
 //         staging of interleaved (x, y, z); nn[n]
 //
 // The reference keeps q/p as AoS double[N][4] on the host, materialises
@@ -46,6 +48,8 @@ enum StatusBits : int {
   ST_GHOST_OVERFLOW = 2,
   ST_LIST_OVERFLOW = 4,
   ST_CELL_INDEX = 8,
+  ST_STAGE_OVERFLOW = 16,
+  ST_TILE_OVERFLOW = 32,
 };
 
 // Small block of device results read back by the host in one copy.
@@ -55,12 +59,11 @@ struct DevStatus {
   int ntiles;
   int cand_count;
   int pair_hit;
-  int pad0;
+  int stage_total;                 // staged particles of all tiles
   double vmax2;                    // max |p|^2 over the reals
   double dmax2;                    // max displacement^2 since the last build
   double kmax_d;                   // longest neighbour list (as a double)
   double stage_max;                // largest tile staging (particles)
-  double plane_max;                // largest staging of one plane of a tile
   long long canon_count;
   double sums[4];                  // KE, PE, virial, spare
 };
@@ -71,7 +74,7 @@ struct DevStatus {
 // partials (k_max_partials), so no address sees more than one writer.  It
 // ends with a barrier, so calls may follow one another.
 __device__ __forceinline__ void block_max_to(double v, double *__restrict__ partial) {
-  __shared__ double lds[kBlock / 64];
+  __shared__ double lds[16];   // up to 1024 threads
   v = wave_max(v);
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
   __syncthreads();
@@ -560,219 +563,419 @@ __global__ void k_gather_images(const int *__restrict__ order, int n, int g,
 }
 
 // ---------------------------------------------------------------- tiles
-// A tile is up to kTile consecutive reals of one interior cell row (y, z),
-// spanning cells xf..xl.  Its partners all lie in 9 cell-row ranges
-// (y+dy, z+dz, xf-1..xl+1), each of which is one contiguous run of reals
-// (interior cells, slots scan_in) and one of images (outer cells, slots
-// n + scan_out): 18 segments, staged once into LDS as interleaved (x, y, z)
-// triples.  Neighbour lists hold 16-bit entries e = 3 x (staged index), so a
-// partner is three LDS loads at fixed offsets from one address.
+// A tile is up to kBlk reals from a block of at most 3 x 3 interior cell
+// rows (y0 .. y0+ny-1, z0 .. z0+nz-1): a slice of each of the block's rows,
+// the block's reals split by rank into tiles of about equal size.  Every
+// partner of a tile's reals lies in the (ny+2) x (nz+2) cell rows around
+// the block, within the x window [xa - h, xb + h] of the tile's reals
+// (h a little over the search length).  Each cell row is sorted by x as a
+// whole (image cell 0, real cells 1..ms, image cell ms+1, or image cells
+// only), so its window is one run of that order, found by two binary
+// searches.  The runs are staged into LDS once per tile, row after row
+// (staged row gr = gy + 5 gz), as interleaved (x, y, z); sidx holds the
+// global slot of every staged particle.  A neighbour list entry is 24 x the
+// partner's staging index (its byte offset), so a tile stages at most
+// kMaxPlane particles and the sentinel after them.
+//
+// Against tiles of one cell row staged a z-plane at a time, a block tile
+// stages 4.5 instead of 11 particles per real at the same LDS per
+// wavefront, and each list is walked in one pass: the wavefront's lanes
+// (list columns, sorted by list length) run for the longest list of the
+// wavefront instead of the longest per plane.
+constexpr int kBlk = 512;                                 // reals (list columns) per tile
+constexpr int kBlkRows = 3;                               // most cell rows per block and axis
+constexpr int kSRow = kBlkRows + 2;                       // staged rows per axis, at most
+constexpr int kSRows = kSRow * kSRow;                     // staged rows per tile, at most
+constexpr int kRealRows = kBlkRows * kBlkRows;            // real rows per block, at most
 
-// per tile, written at the build: seg_start[18], seg_base[19] (prefix sums)
-constexpr int kTSeg = 40;
+// Block shapes and reals aimed at per tile, tried in this order while some
+// tile's staging overflows: 3 x 3 rows of cells suit the usual densities
+// (about 2,300 staged particles for 500 reals at rho 0.5); dense boxes of
+// wide cells need smaller blocks and tiles.
+struct BlkShape {
+  int rows, target;
+};
+constexpr BlkShape kBlkLadder[] = {{3, 503}, {3, 402}, {3, 321}, {3, 256}, {2, 224}, {2, 179},
+                                   {2, 143}, {2, 114}, {1, 96},  {1, 76},  {1, 61},  {1, 48},
+                                   {1, 38},  {1, 32}};
+constexpr int kBlkLevels = (int)(sizeof kBlkLadder / sizeof kBlkLadder[0]);
+
+// per tile, as every list walker reads it
+struct BlkTile {
+  int nreal;   // reals, list columns 0 .. nreal-1
+  int S;       // staged particles; the sentinel sits at S
+  int s0;      // first entry of the tile in sidx
+  int pad;
+};
+
+// per tile, for the list build and the staging index
+struct BlkInfo {
+  int y0, z0, ny, nz;
+  double xa, xb;              // x range of the tile's reals
+  int sp[kRealRows + 1];      // natural column of the first real of real row r = ry + 3 rz
+  int sfirst[kRealRows];      // slot of that real
+  int selfoff[kRealRows];     // staging index of real i of row r is i + selfoff[r]
+  int rs[kSRows + 1];         // staging index of the first particle of staged row gr
+  int rlo[kSRows];            // that particle's index in its row's x-sorted run
+};
 
 struct Cells {
   const int *scan_in, *cnt_in, *scan_out, *cnt_out;
-  const int *tseg;   // [tile][kTSeg] segments of every tile
-  const unsigned char *tperm;   // [tile][kTile] real (offset in the tile) of each list column
 };
 
-
-// [start, end) of segment s (0..17) of a tile, as global slots
-__device__ __forceinline__ void tile_segment(const Geom &g, const Cells &c, int n, int y, int z,
-                                             int xf, int xl, int s, int *start, int *len) {
-  const int r = s >> 1, dy = r % 3 - 1, dz = r / 3 - 1;
-  const int a = cell_linear(g, xf - 1, y + dy, z + dz);
-  const int b = cell_linear(g, xl + 1, y + dy, z + dz);
-  if ((s & 1) == 0) {
-    *start = c.scan_in[a];
-    *len = c.scan_in[b] + c.cnt_in[b] - *start;
+// The cell row (y, z) of the padded grid in x order: slot ranges a (image
+// cell x = 0, or every cell of an image row), b (real cells 1..ms), c
+// (image cell ms + 1).
+struct RowRun {
+  int a0, na, b0, nb, c0, nc;
+};
+__device__ __forceinline__ RowRun row_run(const Geom &g, const Cells &c, int n, int y, int z) {
+  const int cl0 = cell_linear(g, 0, y, z), cl1 = cell_linear(g, g.m[0] - 1, y, z);
+  RowRun r;
+  if (y >= 1 && y <= g.ms[1] && z >= 1 && z <= g.ms[2]) {
+    r.a0 = n + c.scan_out[cl0];
+    r.na = c.cnt_out[cl0];
+    r.b0 = c.scan_in[cl0 + 1];
+    r.nb = c.scan_in[cl1 - 1] + c.cnt_in[cl1 - 1] - r.b0;
+    r.c0 = n + c.scan_out[cl1];
+    r.nc = c.cnt_out[cl1];
   } else {
-    *start = n + c.scan_out[a];
-    *len = n + c.scan_out[b] + c.cnt_out[b] - *start;
+    r.a0 = n + c.scan_out[cl0];
+    r.na = c.scan_out[cl1] + c.cnt_out[cl1] - c.scan_out[cl0];
+    r.b0 = r.c0 = r.a0;
+    r.nb = r.nc = 0;
   }
+  return r;
+}
+__device__ __forceinline__ int run_len(const RowRun &r) { return r.na + r.nb + r.nc; }
+__device__ __forceinline__ int run_slot(const RowRun &r, int t) {
+  return t < r.na ? r.a0 + t : (t < r.na + r.nb ? r.b0 + (t - r.na) : r.c0 + (t - r.na - r.nb));
+}
+// first index of the run whose x is >= v (ABOVE: > v)
+template <bool ABOVE>
+__device__ __forceinline__ int run_bound(const RowRun &r, const double4 *__restrict__ pos, double v) {
+  int lo = 0, hi = run_len(r);
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const double x = pos[run_slot(r, mid)].x;
+    if (ABOVE ? x <= v : x < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
 }
 
-// XCD-aware tile order: the 8 XCDs each take one contiguous block of tiles
-// (workgroup b runs on XCD b % 8), so neighbouring rows share an L2.
+// blocks of br x br cell rows: by fastest, bz-major
+__host__ __device__ inline int blk_count(const Geom &g, int axis, int br) {
+  return (g.ms[axis] + br - 1) / br;
+}
+
+// XCD-aware tile order: the 8 XCDs each take one contiguous range of tiles
+// (workgroup b runs on XCD b % 8), so tiles of neighbouring blocks, which
+// stage the same rows, share an L2.
 __device__ __forceinline__ int tile_of_block(int b, int ntiles) {
   const int chunk = (ntiles + 7) >> 3;
   return (b & 7) * chunk + (b >> 3);
 }
 __host__ __device__ inline int tile_grid(int ntiles) { return ((ntiles + 7) >> 3) << 3; }
 
+// reals of real row r of block (by, bz) of br x br rows (ry = r % 3, rz =
+// r / 3; none outside the block or the grid)
+__device__ __forceinline__ void blk_row(const Geom &g, const Cells &c, int by, int bz, int br,
+                                        int r, int *first, int *cnt) {
+  const int ry = r % kBlkRows, rz = r / kBlkRows;
+  const int y = br * by + ry + 1, z = br * bz + rz + 1;
+  *first = 0;
+  *cnt = 0;
+  if (ry >= br || rz >= br || y > g.ms[1] || z > g.ms[2]) return;
+  const int a = cell_linear(g, 1, y, z), b = cell_linear(g, g.ms[0], y, z);
+  *first = c.scan_in[a];
+  *cnt = c.scan_in[b] + c.cnt_in[b] - *first;
+}
 
-// the tile's segments (precomputed at the build) into sm; every thread of
-// the block calls it
-__device__ __forceinline__ void tile_load_segments(const Cells &c, int tile, TileSmem &sm) {
-  const int *ts = c.tseg + (size_t)tile * kTSeg;
-  const int t = threadIdx.x;
-  if (t < kSeg) sm.seg_start[t] = ts[t];
-  else if (t < 2 * kSeg + 1) sm.seg_base[t - kSeg] = ts[t];
+// Cumulative count of a block's reals over its x cells by one workgroup into
+// LDS: hist[c] = reals of the block in x cells 1..c, hist[0] = 0.
+constexpr int kMaxCellsX = 4096;
+__device__ void blk_hist(const Geom &g, const Cells &c, int by, int bz, int br, int *hist) {
+  const int ms = g.ms[0];
+  for (int cx = (int)threadIdx.x + 1; cx <= ms; cx += (int)blockDim.x) {
+    int s = 0;
+    for (int ry = 0; ry < br; ry++)
+      for (int rz = 0; rz < br; rz++) {
+        const int y = br * by + ry + 1, z = br * bz + rz + 1;
+        if (y <= g.ms[1] && z <= g.ms[2]) s += c.cnt_in[cell_linear(g, cx, y, z)];
+      }
+    hist[cx] = s;
+  }
+  if (threadIdx.x == 0) hist[0] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int cx = 1; cx <= ms; cx++) hist[cx] += hist[cx - 1];
   __syncthreads();
 }
 
-// Partners of one list column in list order, for kernels that stage the
-// whole tile: f(o) with o = 3 x the partner's index in that staging (an
-// entry is 24 x the index in its plane, so o = 3 x the plane's first index
-// + entry / 8); sentinel entries are skipped.  One 16-byte load per group.
-template <class F>
-__device__ __forceinline__ void for_each_partner(const unsigned short *__restrict__ lst, int m,
-                                                 int nb, const TileSmem &sm, F f) {
-  for (int p = 0; p < kPlanes; p++) {
-    const int pb = sm.seg_base[6 * p];
-    const unsigned lim = (unsigned)(kEntryBytes * (sm.seg_base[6 * p + 6] - pb));
-    const int ke = plane_end(p, nb, m);
-    for (int k = plane_begin(p, nb); k < ke; k += kILP) {
-      const uint4 w = *reinterpret_cast<const uint4 *>(lst + list_index<kTile>(k));
-      const unsigned h[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-      for (int u = 0; u < kILP; u++) {
-        const unsigned e = (h[u / 2] >> (16 * (u % 2))) & 0xffffu;
-        if (e < lim) f(3 * pb + (int)(e >> 3));
-      }
-    }
+// Where tile k of nt starts in real row r of the block: the first real of
+// the row (slots [first, first + cnt), sorted by x) at or past cut k, the x
+// at which the block's cumulative count, taken as linear within each x
+// cell, reaches k M / nt.  Cut 0 is the row's start, cut nt its end, so the
+// tiles of a block share their x cuts across its rows and cover every real
+// once.
+__device__ int blk_cut_slot(const Geom &g, const int *hist, const double4 *__restrict__ pos,
+                            int first, int cnt, int k, int nt) {
+  if (k <= 0) return first;
+  if (k >= nt) return first + cnt;
+  const int ms = g.ms[0];
+  const double cum = (double)k * (double)hist[ms] / (double)nt;
+  int lo = 1, hi = ms;   // first x cell whose cumulative count exceeds cum
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((double)hist[mid] > cum) hi = mid;
+    else lo = mid + 1;
   }
-}
-
-// For kernels that walk the lists a plane at a time (the LDS of one plane):
-// every thread stages plane p and runs prep(p) (per-plane LDS of the
-// kernel's own, ending with a barrier if it writes any); then each active
-// lane calls f(o) for its list's partners in that plane, in list order,
-// with o = 3 x the partner's index in the plane's staging (= entry / 8);
-// sentinel entries are skipped.  One 16-byte list load per group.
-template <class Load, class Prep, class F>
-__device__ __forceinline__ void walk_planes(const TileSmem &sm, double *s3, Load load, Prep prep,
-                                            bool act, const unsigned short *__restrict__ lst, int m,
-                                            int nb, F f) {
-  for (int p = 0; p < kPlanes; p++) {
-    if (p > 0) __syncthreads();   // the previous plane is read
-    stage_plane(sm, p, s3, load);
-    prep(p);
-    if (!act) continue;
-    const unsigned lim = (unsigned)(kEntryBytes * (sm.seg_base[6 * p + 6] - sm.seg_base[6 * p]));
-    const int ke = plane_end(p, nb, m);
-    for (int k = plane_begin(p, nb); k < ke; k += kILP) {
-      const uint4 w = *reinterpret_cast<const uint4 *>(lst + list_index<kTile>(k));
-      const unsigned h[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-      for (int u = 0; u < kILP; u++) {
-        const unsigned e = (h[u / 2] >> (16 * (u % 2))) & 0xffffu;
-        if (e < lim) f((int)(e >> 3));
-      }
-    }
+  const double f = (cum - (double)hist[lo - 1]) / (double)max(hist[lo] - hist[lo - 1], 1);
+  const double X = ((double)(lo - 1) + f) / g.inv_w[0];
+  int a = 0, b = cnt;
+  while (a < b) {
+    const int mid = (a + b) >> 1;
+    if (pos[first + mid].x < X) a = mid + 1;
+    else b = mid;
   }
+  return first + a;
 }
 
-// rows (in row_of order) -> number of tiles
-__global__ void k_row_tiles(Geom g, Cells c, int *__restrict__ row_tiles) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int rows = g.ms[1] * g.ms[2];
-  if (r >= rows) return;
-  int y, z;
-  row_of(g, r, &y, &z);
-  const int a = cell_linear(g, 1, y, z), b = cell_linear(g, g.ms[0], y, z);
-  const int cnt = c.scan_in[b] + c.cnt_in[b] - c.scan_in[a];
-  row_tiles[r] = (cnt + kTile - 1) / kTile;
-}
-
-// Tile descriptors and segments: one wavefront per cell row, one lane per
-// tile of the row; block maxima of the staging sizes to partial.
-__global__ void k_write_tiles(Geom g, Cells c, int n, const double4 *__restrict__ pos,
-                              const int *__restrict__ row_tiles, const int *__restrict__ row_off,
-                              TileDesc *__restrict__ tiles, int *__restrict__ tseg,
-                              double *__restrict__ partial) {
-  const int r = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
-  const int rows = g.ms[1] * g.ms[2];
-  double smax = 0.0, pmax = 0.0;
-  if (r < rows) {
-    int y, z;
-    row_of(g, r, &y, &z);
-    const int a = cell_linear(g, 1, y, z), b = cell_linear(g, g.ms[0], y, z);
-    const int first = c.scan_in[a], cnt = c.scan_in[b] + c.cnt_in[b] - first;
-    for (int k = lane; k < row_tiles[r]; k += 64) {
-      TileDesc t;
-      t.i0 = first + k * kTile;
-      t.ni = min(kTile, cnt - k * kTile);
-      t.y = y;
-      t.z = z;
-      t.xf = interior_coord(pos[t.i0].x, g.inv_w[0], g.ms[0]);
-      t.xl = interior_coord(pos[t.i0 + t.ni - 1].x, g.inv_w[0], g.ms[0]);
-      int *ts = tseg + (size_t)(row_off[r] + k) * kTSeg;
-      int stage = 0;
-      ts[kSeg] = 0;
-      for (int sgi = 0; sgi < kSeg; sgi++) {
-        int st, ln;
-        tile_segment(g, c, n, y, z, t.xf, t.xl, sgi, &st, &ln);
-        ts[sgi] = st;
-        stage += ln;
-        ts[kSeg + 1 + sgi] = stage;
+// Tiles of each block, one workgroup per block: nt = ceil(reals / target),
+// raised until no tile holds more than kBlk reals (the shared x cuts follow
+// the block's x histogram, so a tile's count is near M / nt); none for an
+// empty block.
+__global__ void k_blk_count(const double4 *__restrict__ pos, Geom g, Cells c, int br, int target,
+                            int *__restrict__ blk_nt) {
+  __shared__ int hist[kMaxCellsX + 1];
+  __shared__ int rfirst[kRealRows], rcnt[kRealRows];
+  constexpr int kChunk = kBlock / kRealRows;   // tiles counted per round
+  __shared__ int len[kChunk][kRealRows];
+  __shared__ int over;
+  const int b = blockIdx.x, BY = blk_count(g, 1, br), by = b % BY, bz = b / BY;
+  const int tid = threadIdx.x;
+  if (tid < kRealRows) blk_row(g, c, by, bz, br, tid, &rfirst[tid], &rcnt[tid]);
+  blk_hist(g, c, by, bz, br, hist);
+  const int M = hist[g.ms[0]];
+  int nt = (M + target - 1) / target;
+  for (int pass = 0; pass < 64 && nt > 0; pass++) {
+    if (tid == 0) over = 0;
+    __syncthreads();
+    for (int k0 = 0; k0 < nt; k0 += kChunk) {
+      const int kk = tid / kRealRows, r = tid % kRealRows, k = k0 + kk;
+      if (kk < kChunk && k < nt)
+        len[kk][r] = blk_cut_slot(g, hist, pos, rfirst[r], rcnt[r], k + 1, nt) -
+                     blk_cut_slot(g, hist, pos, rfirst[r], rcnt[r], k, nt);
+      __syncthreads();
+      if (tid < kChunk && k0 + tid < nt) {
+        int s = 0;
+        for (int q = 0; q < kRealRows; q++) s += len[tid][q];
+        if (s > kBlk) atomicOr(&over, 1);
       }
-      t.stage = stage;
-      t.pad = 0;
-      tiles[row_off[r] + k] = t;
-      smax = fmax(smax, (double)stage);
-      for (int p = 0; p < kPlanes; p++)
-        pmax = fmax(pmax, (double)(ts[kSeg + 6 * p + 6] - ts[kSeg + 6 * p]));
+      __syncthreads();
     }
+    if (!over) break;
+    nt += max(1, nt / 8);
+    __syncthreads();
+  }
+  if (tid == 0) blk_nt[b] = nt;
+}
+
+// Tiles of one block per workgroup, one wavefront per tile: real row r
+// (lanes 0..8) gives the tile the slots between its cuts.  Lanes 0..24 then
+// cut the staged rows to the window [xa - h, xb + h] of the tile's reals, h
+// = SL (1 + 1e-10) + eps, wider than any |dx| of a pair with r^2 <= SL^2 as
+// rounded, eps covering the rounding of positions near the box (the
+// images' shift).  tile_S gets each tile's staging size; a staging above
+// cap raises ST_STAGE_OVERFLOW.
+__global__ void k_blk_tiles(const double4 *__restrict__ pos, Geom g, Cells c, int n, int br,
+                            const int *__restrict__ blk_nt, const int *__restrict__ blk_toff,
+                            int cap, BlkTile *__restrict__ tiles, BlkInfo *__restrict__ info,
+                            int *__restrict__ tile_S, double *__restrict__ partial,
+                            int *__restrict__ flags) {
+  __shared__ int hist[kMaxCellsX + 1];
+  const int b = blockIdx.x, BY = blk_count(g, 1, br);
+  const int by = b % BY, bz = b / BY;
+  const int nt = blk_nt[b], lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int y0 = br * by + 1, z0 = br * bz + 1;
+  const int ny = min(br, g.ms[1] - y0 + 1), nz = min(br, g.ms[2] - z0 + 1);
+  const double h = sqrt(g.SL2) * (1.0 + 1e-10) + 1e-9 * (g.L[0] + g.L[1] + g.L[2]);
+  blk_hist(g, c, by, bz, br, hist);
+  double smax = 0.0;
+  for (int k = w; k < nt; k += (int)(blockDim.x >> 6)) {
+    const int t = blk_toff[b] + k;
+    // slices of the real rows (lanes 0..8)
+    int lo = 0, len = 0;
+    if (lane < kRealRows) {
+      int rf, rm;
+      blk_row(g, c, by, bz, br, lane, &rf, &rm);
+      lo = blk_cut_slot(g, hist, pos, rf, rm, k, nt);
+      len = blk_cut_slot(g, hist, pos, rf, rm, k + 1, nt) - lo;
+    }
+    double xa = kFar, xb = -kFar;
+    if (len > 0) {
+      xa = pos[lo].x;
+      xb = pos[lo + len - 1].x;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      xa = fmin(xa, __shfl_xor(xa, off));
+      xb = fmax(xb, __shfl_xor(xb, off));
+    }
+    // exclusive prefix of the slice lengths over lanes 0..8
+    int sp = 0;
+    for (int r = 0; r < kRealRows; r++) {
+      const int lr = __shfl(len, r);
+      if (r < lane) sp += lr;
+    }
+    const int nreal = __shfl(sp, kRealRows);   // lane 9 holds the total
+    // the window of staged row gr = lane (0..24)
+    int wlo = 0, wlen = 0;
+    RowRun run{};
+    const int gy = lane % kSRow, gz = lane / kSRow;
+    if (lane < kSRows && gy < ny + 2 && gz < nz + 2 && nreal > 0) {
+      run = row_run(g, c, n, y0 - 1 + gy, z0 - 1 + gz);
+      wlo = run_bound<false>(run, pos, xa - h);
+      wlen = run_bound<true>(run, pos, xb + h) - wlo;
+    }
+    int rs = 0;
+    for (int r = 0; r < kSRows; r++) {
+      const int lr = __shfl(wlen, r);
+      if (r < lane) rs += lr;
+    }
+    const int S = __shfl(rs, kSRows);   // lane 25 holds the total
+    BlkInfo *I = info + t;
+    if (lane == 0) {
+      tiles[t] = BlkTile{nreal, S, 0, 0};
+      tile_S[t] = S;
+      I->y0 = y0;
+      I->z0 = z0;
+      I->ny = ny;
+      I->nz = nz;
+      I->xa = xa;
+      I->xb = xb;
+      if (S > cap) atomicOr(flags, ST_STAGE_OVERFLOW);
+      if (nreal > kBlk) atomicOr(flags, ST_TILE_OVERFLOW);
+    }
+    if (lane <= kRealRows) I->sp[lane] = sp;
+    if (lane < kRealRows) I->sfirst[lane] = lo;
+    if (lane <= kSRows) I->rs[lane] = rs;
+    if (lane < kSRows) I->rlo[lane] = wlo;
+    // the real rows' self offsets: real row r sits in staged row (ry + 1) + 5 (rz + 1)
+    if (lane < kSRows && gy >= 1 && gy <= ny && gz >= 1 && gz <= nz) {
+      const int r = (gy - 1) + kBlkRows * (gz - 1);
+      I->selfoff[r] = rs + run.na - run.b0 - wlo;
+    }
+    smax = fmax(smax, (double)S);
   }
   block_max_to(smax, partial);
-  block_max_to(pmax, partial + gridDim.x);
 }
 
-// The staging index where each cell of a tile's nine rows starts, reals
-// (interior cells, kind 0) and images (outer cells, kind 1): row r, cell k of
-// x = xf - 1 + k at tab[(2 r + kind) * kCellTab + k], and the row's end at
-// k = nx = xl - xf + 3.  Consecutive cells of a row are consecutive slots,
-// so a cell's range is [tab[k], tab[k + 1]) and a run of cells one range.
-// Filled by every thread; false (and nothing written) when a row has more
-// cells than the table holds.
-constexpr int kCellTab = 48;
-__device__ __forceinline__ bool tile_cell_table(const Geom &g, const Cells &c, int n,
-                                                const TileDesc &t, const TileSmem &sm,
-                                                unsigned short *tab) {
-  const int nx = t.xl - t.xf + 3;
-  if (nx + 1 > kCellTab) return false;
-  for (int e = threadIdx.x; e < 18 * (nx + 1); e += blockDim.x) {
-    const int row = e / (nx + 1), k = e % (nx + 1), r = row >> 1, kind = row & 1;
-    const int y = t.y + r % 3 - 1, z = t.z + r / 3 - 1;
-    const int cl = cell_linear(g, t.xf - 1 + min(k, nx - 1), y, z);
-    const int start = kind == 0 ? c.scan_in[cl] : n + c.scan_out[cl];
-    const int cnt = k == nx ? (kind == 0 ? c.cnt_in[cl] : c.cnt_out[cl]) : 0;
-    tab[row * kCellTab + k] = (unsigned short)(sm.seg_base[row] + (start + cnt - sm.seg_start[row]));
+// sidx of every tile (one workgroup per tile) and the tile's first entry in it
+__global__ void k_blk_sidx(Geom g, Cells c, int n, int ntiles, const int *__restrict__ s0,
+                           BlkTile *__restrict__ tiles, const BlkInfo *__restrict__ info,
+                           int *__restrict__ sidx) {
+  __shared__ RowRun runs[kSRows];
+  __shared__ int rs[kSRows + 1], rlo[kSRows];
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  const BlkInfo *I = info + t;
+  const int tid = threadIdx.x;
+  if (tid < kSRows) {
+    const int gy = tid % kSRow, gz = tid / kSRow;
+    RowRun r{};
+    if (gy < I->ny + 2 && gz < I->nz + 2) r = row_run(g, c, n, I->y0 - 1 + gy, I->z0 - 1 + gz);
+    runs[tid] = r;
+    rlo[tid] = I->rlo[tid];
   }
+  if (tid <= kSRows) rs[tid] = I->rs[tid];
   __syncthreads();
-  return true;
+  const int S = rs[kSRows], base = s0[t];
+  if (tid == 0) tiles[t].s0 = base;
+  for (int u = tid; u < S; u += blockDim.x) {
+    int gr = 0;
+    while (u >= rs[gr + 1]) gr++;
+    sidx[base + u] = run_slot(runs[gr], rlo[gr] + (u - rs[gr]));
+  }
 }
 
-// Partners of a real at qi among candidates [lo, hi) of the plane staged at
-// s3 (candidate o at 3 (o - pb)): r^2 <= SL2 (list_r2) and not the real
-// itself, put in candidate order.  Four candidates are tested at a time so
-// that their LDS reads are in flight together.
+// Stage a tile: its S particles, load(sidx[u]) for staged index u, into s3
+// as interleaved (x, y, z), and the sentinel after them.  kB loads per
+// thread go out together; kB NT covers the largest staging.  Every thread
+// calls it; it ends with a barrier.
+template <int NT, class Load>
+__device__ __forceinline__ void stage_tile(int S, const int *__restrict__ sidx, double *s3,
+                                           Load load) {
+  constexpr int kB = (kMaxPlane + NT) / NT;
+  for (int u0 = 0; u0 < S; u0 += kB * NT) {
+    int sl[kB];
+#pragma unroll
+    for (int j = 0; j < kB; j++) {
+      const int u = u0 + j * NT + (int)threadIdx.x;
+      sl[j] = sidx[u < S ? u : 0];
+    }
+    double4 q[kB];
+#pragma unroll
+    for (int j = 0; j < kB; j++) q[j] = load(sl[j]);
+#pragma unroll
+    for (int j = 0; j < kB; j++) {
+      const int u = u0 + j * NT + (int)threadIdx.x;
+      if (u < S) {
+        s3[3 * u] = q[j].x;
+        s3[3 * u + 1] = q[j].y;
+        s3[3 * u + 2] = q[j].z;
+      }
+    }
+  }
+  if (threadIdx.x == 0) s3[3 * S] = s3[3 * S + 1] = s3[3 * S + 2] = kFar;
+  __syncthreads();
+}
+
+// Partners of one list column (entries [0, m), column base lst, kBlk
+// columns) in list order: f(o) with o = 3 x the partner's staging index;
+// sentinel entries (>= lim = 24 S) are skipped.  One 16-byte load per group.
+template <class F>
+__device__ __forceinline__ void for_each_partner(const unsigned short *__restrict__ lst, int m,
+                                                 unsigned lim, F f) {
+  for (int k = 0; k < m; k += kILP) {
+    const uint4 w = *reinterpret_cast<const uint4 *>(lst + list_index<kBlk>(k));
+    const unsigned h[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int u = 0; u < kILP; u++) {
+      const unsigned e = (h[u / 2] >> (16 * (u % 2))) & 0xffffu;
+      if (e < lim) f((int)(e >> 3));
+    }
+  }
+}
+
+// Partners of a real at qi among staged candidates [lo, hi) (candidate o
+// at s3 + 3 o): r^2 <= SL2 (list_r2) and not the real itself, put in
+// candidate order.  Four candidates are tested at a time so that their LDS
+// reads are in flight together.
 template <class Writer>
-__device__ __forceinline__ void range_partners(const double *s3, int pb, const double4 &qi,
-                                               double SL2, int lo, int hi, int self, Writer &gw) {
+__device__ __forceinline__ void range_partners(const double *s3, const double4 &qi, double SL2,
+                                               int lo, int hi, int self, Writer &gw) {
   constexpr int kU = 4;
   for (int o = lo; o < hi; o += kU) {
     bool ok[kU];
 #pragma unroll
     for (int u = 0; u < kU; u++) {
-      const double *q = s3 + 3 * (min(o + u, hi - 1) - pb);
+      const double *q = s3 + 3 * min(o + u, hi - 1);
       ok[u] = o + u < hi && list_r2(q[0], q[1], q[2], qi.x, qi.y, qi.z) <= SL2 && o + u != self;
     }
 #pragma unroll
     for (int u = 0; u < kU; u++)
-      if (ok[u]) gw.put(plane_entry(o + u - pb));
+      if (ok[u]) gw.put(plane_entry(o + u));
   }
 }
 
-// First candidate in [lo, hi) of the plane staged at s3 (candidate o at
-// 3 (o - pb)) whose x is >= v (above: > v); the range is sorted by x.
+// First staged candidate in [lo, hi) (candidate o at s3 + 3 o) whose x is
+// >= v (ABOVE: > v); the range is sorted by x.
 template <bool ABOVE>
-__device__ __forceinline__ int x_bound(const double *s3, int pb, int lo, int hi, double v) {
+__device__ __forceinline__ int x_bound(const double *s3, int lo, int hi, double v) {
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
-    const double x = s3[3 * (mid - pb)];
+    const double x = s3[3 * mid];
     if (ABOVE ? x <= v : x < v) lo = mid + 1;
     else hi = mid;
   }
@@ -801,110 +1004,82 @@ __device__ __forceinline__ bool row_window(const Geom &g, const double4 &qi, int
   return true;
 }
 
-// Full neighbour list per tile: one list column per real, every partner
-// j != i (real or image) of its 27 cells with r^2 <= SL^2, r^2 formed as
-// the reference's AVX2 search, fma(dx,dx, fma(dy,dy, dz*dz)) with dq = q_j
-// - q_i (meshlist.cc:354-362).  Entry k of column l sits at
-// list[tile * kcap * kTile + list_index<kTile>(k) + l kILP] in the plane-list form: the
-// cells are visited plane by plane (dz = -1, 0, +1), each plane staged
-// alone, its entries byte offsets in that staging, padded with its
-// sentinel to a multiple of kILP; nb[i] holds where planes 1 and 2 start
-// (low and high 16 bits) and nn[i] the padded length.
-__global__ void __launch_bounds__(kTile) k_build_tiles(
-    const double4 *__restrict__ pos, const TileDesc *__restrict__ tiles, int ntiles, Geom g,
-    Cells c, int n, int kcap, unsigned short *__restrict__ list, int *__restrict__ nn,
-    int *__restrict__ nb, unsigned char *__restrict__ tperm, double *__restrict__ partial,
-    int *__restrict__ flags) {
+// Full neighbour list of every real of a tile: every partner j != i (real
+// or image) of its 27 cells with r^2 <= SL^2, r^2 formed as the reference's
+// AVX2 search, fma(dx,dx, fma(dy,dy, dz*dz)) with dq = q_j - q_i
+// (meshlist.cc:354-362).  The partners go plane by plane (dz = -1, 0, +1),
+// row by row (dy = -1, 0, +1), in x order within a row; each plane's entries
+// are padded with the sentinel to a multiple of kILP.  nb[i] holds where
+// planes 1 and 2 start (low and high 16 bits), nn[i] the padded length.
+// Each real's list goes first to column l (its place in the tile's real
+// rows) of the build buffer; then the columns are ranked by list length
+// (longest first, ties by column) and copied to that rank in the tile's
+// region of the list, and treal gets the real of each ranked column.
+__global__ void __launch_bounds__(kBlk) k_build_tiles(
+    const double4 *__restrict__ pos, const BlkTile *__restrict__ tiles,
+    const BlkInfo *__restrict__ info, const int *__restrict__ sidx, int ntiles, Geom g, int kcap,
+    unsigned short *__restrict__ list_nat, unsigned short *__restrict__ list,
+    int *__restrict__ nn, int *__restrict__ nb, int *__restrict__ treal,
+    double *__restrict__ partial, int *__restrict__ flags) {
   extern __shared__ double stage_mem[];
-  __shared__ TileSmem sm;
+  __shared__ int keys[kBlk];
+  __shared__ int rs[kSRows + 1];
   const int tile = tile_of_block(blockIdx.x, ntiles);
   if (tile >= ntiles) {
-    if (threadIdx.x == 0) partial[blockIdx.x] = 0.0;
+    block_max_to(0.0, partial);
     return;
   }
-  const TileDesc t = tiles[tile];
-  double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
+  const BlkTile t = tiles[tile];
+  const BlkInfo *I = info + tile;
   const int lane = threadIdx.x;
-  const bool act = lane < t.ni;
-  const int i = t.i0 + lane;
+  if (lane <= kSRows) rs[lane] = I->rs[lane];
+  const bool act = lane < t.nreal;
+  int r = 0;
+  while (r < kRealRows - 1 && lane >= I->sp[r + 1]) r++;
+  const int i = act ? I->sfirst[r] + (lane - I->sp[r]) : 0;
   const double4 qi = act ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
-  // List columns in the order of z (ties by position in the tile): reals
-  // near the bottom of their cells have many partners in the plane below,
-  // so a wavefront's lanes reach each plane boundary at similar entries.
-  __shared__ double zs[kTile];
-  zs[lane] = act ? qi.z : kFar;
-  tile_load_segments(c, tile, sm);
-  int col = lane;
-  if (act) {
-    col = 0;
-    for (int j = 0; j < t.ni; j++) col += (zs[j] < qi.z || (zs[j] == qi.z && j < lane)) ? 1 : 0;
-    tperm[(size_t)tile * kTile + col] = (unsigned char)lane;
-  }
-  const int self = act ? sm.seg_base[8] + (i - sm.seg_start[8]) : -1;   // in row (0, 0)
-  const int cx = interior_coord(qi.x, g.inv_w[0], g.ms[0]);
+  const int self = act ? i + I->selfoff[r] : -1;
+  const int ry = r % kBlkRows, rz = r / kBlkRows, y = I->y0 + ry, z = I->z0 + rz;
+  stage_tile<kBlk>(t.S, sidx + t.s0, stage_mem, [&](int slot) { return pos[slot]; });
   // rounding allowance of the cell faces and the binning, far above it
   const double eps = 1e-9 * (g.L[0] + g.L[1] + g.L[2]);
-  GroupWriter<kTile> gw(list + (size_t)tile * kcap * kTile + (size_t)col * kILP, kcap);
-  __shared__ unsigned short tab[18 * kCellTab];
-  const bool have_tab = tile_cell_table(g, c, n, t, sm, tab);
+  const size_t region = (size_t)tile * kcap * kBlk;
+  GroupWriter<kBlk> gw(list_nat + region + (size_t)lane * kILP, kcap);
   int b1 = 0, b2 = 0;
-  for (int p = 0; p < kPlanes; p++) {
-    if (p > 0) __syncthreads();   // the previous plane is read
-    stage_plane(sm, p, sx, [&](int, int slot) { return pos[slot]; });
-    const int pb = sm.seg_base[6 * p];
-    if (act && have_tab) {
-      // the real's three cells of each row from the table: the image cell
-      // before them, the real cells, the image cells after (only the end
-      // cells of a real row can be image cells, all cells of an image row
-      // are), each range sorted by x and cut to the row's x window: the
-      // same partners in the same order as cell by cell
-      const int k0 = cx - t.xf;   // cell cx - 1
-      for (int r = 3 * p; r < 3 * p + 3; r++) {
-        double h;
-        if (!row_window(g, qi, t.y, t.z, r % 3 - 1, r / 3 - 1, eps, &h)) continue;
-        const double xlo = qi.x - h, xhi = qi.x + h;
-        const unsigned short *ti = tab + (2 * r) * kCellTab, *to = tab + (2 * r + 1) * kCellTab;
-        const auto window = [&](int a, int b) {
-          const int lo = x_bound<false>(stage_mem, pb, a, b, xlo);
-          range_partners(stage_mem, pb, qi, g.SL2, lo, x_bound<true>(stage_mem, pb, lo, b, xhi),
-                         self, gw);
-        };
-        window(to[k0], to[k0 + 1]);
-        window(ti[k0], ti[k0 + 3]);
-        window(to[k0 + 1], to[k0 + 3]);
-      }
-      gw.pad_plane(sm.seg_base[6 * p + 6] - pb);
-    } else if (act) {
-      for (int r = 3 * p; r < 3 * p + 3; r++) {
-        const int y = t.y + r % 3 - 1, z = t.z + r / 3 - 1;
-        for (int dx = -1; dx <= 1; dx++) {
-          const int x = cx + dx;
-          const int cl = cell_linear(g, x, y, z);
-          int lo, len;
-          if (is_outer(g, x, y, z)) {
-            lo = sm.seg_base[2 * r + 1] + (n + c.scan_out[cl] - sm.seg_start[2 * r + 1]);
-            len = c.cnt_out[cl];
-          } else {
-            lo = sm.seg_base[2 * r] + (c.scan_in[cl] - sm.seg_start[2 * r]);
-            len = c.cnt_in[cl];
-          }
-          for (int o = lo; o < lo + len; o++) {
-            const int e = 3 * (o - pb);
-            const double r2 = list_r2(sx[e], sy[e], sz[e], qi.x, qi.y, qi.z);
-            if (r2 <= g.SL2 && o != self) gw.put(plane_entry(o - pb));
-          }
-        }
-      }
-      gw.pad_plane(sm.seg_base[6 * p + 6] - pb);
-    }
-    if (p == 0) b1 = gw.cnt;
-    if (p == 1) b2 = gw.cnt;
-  }
-  const int cnt = gw.cnt;
   if (act) {
-    nn[i] = cnt < kcap ? cnt : kcap;
+    for (int p = 0; p < kPlanes; p++) {
+      for (int dy = -1; dy <= 1; dy++) {
+        const int gr = (ry + 1 + dy) + kSRow * (rz + p);
+        double h;
+        if (!row_window(g, qi, y, z, dy, p - 1, eps, &h)) continue;
+        const int lo = x_bound<false>(stage_mem, rs[gr], rs[gr + 1], qi.x - h);
+        range_partners(stage_mem, qi, g.SL2, lo, x_bound<true>(stage_mem, lo, rs[gr + 1], qi.x + h),
+                       self, gw);
+      }
+      gw.pad_plane(t.S);
+      if (p == 0) b1 = gw.cnt;
+      if (p == 1) b2 = gw.cnt;
+    }
+  }
+  const int cnt = gw.cnt, m = min(cnt, kcap);
+  if (act) {
+    nn[i] = m;
     nb[i] = min(b1, kcap) | (min(b2, kcap) << 16);
     if (cnt > kcap) atomicOr(flags, ST_LIST_OVERFLOW);
+  }
+  keys[lane] = act ? m : -1;
+  __syncthreads();
+  if (act) {
+    int rank = 0;
+    for (int l = 0; l < kBlk; l++) {
+      const int kl = keys[l];
+      rank += (kl > m || (kl == m && l < lane)) ? 1 : 0;
+    }
+    treal[(size_t)tile * kBlk + rank] = i;
+    // this thread's own column, written above, to its rank
+    const uint4 *src = reinterpret_cast<const uint4 *>(list_nat + region + (size_t)lane * kILP);
+    uint4 *dst = reinterpret_cast<uint4 *>(list + region + (size_t)rank * kILP);
+    for (int k = 0; k < m; k += kILP) dst[(size_t)(k / kILP) * kBlk] = src[(size_t)(k / kILP) * kBlk];
   }
   block_max_to((double)cnt, partial);
 }
@@ -1078,29 +1253,48 @@ struct HeatKick {
   double gamma = 0.0, D = 0.0;
 };
 
-// Fixed-order sum over the block (blockDim.x <= kTile threads, a power of
-// two) to partial[blockIdx.x]; every thread calls it
-__device__ __forceinline__ void tile_sum_to(double v, double *__restrict__ partial) {
-  __shared__ double lds[kTile];
-  lds[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
-    __syncthreads();
+// Fixed-order sum over the block of K values per thread into partial[block
+// * K + k]: a butterfly within each wavefront, then the wavefronts' sums in
+// order by thread 0 (up to 1024 threads).  Every thread calls it; it ends
+// with a barrier.
+template <int K>
+__device__ __forceinline__ void block_sums_to(const double (&v)[K], double *__restrict__ partial) {
+  __shared__ double red[K][16];
+  double w[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    w[k] = v[k];
+    for (int off = 32; off > 0; off >>= 1) w[k] += __shfl_xor(w[k], off);
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < K; k++) red[k][threadIdx.x >> 6] = w[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < K; k++) {
+      double s = red[k][0];
+      for (int q = 1; q < (int)(blockDim.x >> 6); q++) s += red[k][q];
+      partial[(size_t)blockIdx.x * K + k] = s;
+    }
+  }
+  __syncthreads();
 }
 
+// The force kick over a tile: one lane per list column, the ranked reals of
+// the tile; the tile's partners are staged once, and each lane walks its
+// whole list (plane_groups over [0, nn)).
+//   MODE 0: p += F dt;  MODE 1: out = F (dt = 1);  MODE 2: p += F dt and
+//   out = q + p dt/2;  MODE 3 (batched step): as 2, plus the next mid-step
+//   position qn = out + p dt/2 and the heat bath's epilogue.
+// Per-block maxima of |p|^2 go to partial (MODE != 1).
 template <int MODE, bool SCALED = false>
-__global__ void __launch_bounds__(kTile) k_kick_tiles(
-    const double4 *__restrict__ pos, double4 *__restrict__ mom, const TileDesc *__restrict__ tiles,
-    int ntiles, Geom g, Cells c, int n, int kcap, const unsigned short *__restrict__ list,
-    const int *__restrict__ nn, const int *__restrict__ nb, PairK pk, double dt,
-    double4 *__restrict__ out, double *__restrict__ partial, LJ lj = LJ{},
+__global__ void __launch_bounds__(kBlk) k_kick_tiles(
+    const double4 *__restrict__ pos, double4 *__restrict__ mom, const BlkTile *__restrict__ tiles,
+    int ntiles, const int *__restrict__ treal, const int *__restrict__ sidx, int kcap,
+    const unsigned short *__restrict__ list, const int *__restrict__ nn, PairK pk, double dt,
+    double CL2, double4 *__restrict__ out, double *__restrict__ partial, LJ lj = LJ{},
     const int *__restrict__ halt = nullptr, double4 *__restrict__ qn = nullptr,
     HeatKick hk = HeatKick{}) {
   extern __shared__ double stage_mem[];
-  __shared__ TileSmem sm;
   if (halt && *halt) return;
   const bool nh = MODE == 3 && hk.type == MDX_HEATBATH_NOSE_HOOVER;
   const bool lgv = MODE == 3 && hk.type == MDX_HEATBATH_LANGEVIN;
@@ -1110,35 +1304,29 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
     if (nh && threadIdx.x == 0) hk.ke_partial[blockIdx.x] = 0.0;
     return;
   }
-  const TileDesc t = tiles[tile];
-  double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
+  const BlkTile t = tiles[tile];
   const int lane = threadIdx.x;
-  const bool act = lane < t.ni;
-  const int i = t.i0 + (act ? c.tperm[(size_t)tile * kTile + lane] : 0);   // list column -> real
+  const bool act = lane < t.nreal;
+  const int i = act ? treal[(size_t)tile * kBlk + lane] : 0;   // list column -> real
   // the lane's own loads go out before the staging, which hides them
   double4 qi = make_double4(0.0, 0.0, 0.0, 0.0), pm = qi;
-  int m = 0, bnd = 0;
+  int m = 0;
   if (act) {
     qi = pos[i];
     m = nn[i];
-    bnd = nb[i];
     if (MODE != 1) pm = mom[i];
   }
-  tile_load_segments(c, tile, sm);
-  const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
+  stage_tile<kBlk>(t.S, sidx + t.s0, stage_mem, [&](int slot) { return pos[slot]; });
+  const unsigned short *lst = list + (size_t)tile * kcap * kBlk + (size_t)lane * kILP;
   // kILP independent partner chains per lane: entry k of the list feeds
   // chain k % kILP, and the chains are summed in a fixed order at the end.
   // A sentinel entry adds exactly +0.0 (the accumulators are never -0.0).
   double ax[kILP], ay[kILP], az[kILP];
 #pragma unroll
   for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
-  const double xi = qi.x, yi = qi.y, zi = qi.z, CL2 = g.CL2;
-  for (int p = 0; p < kPlanes; p++) {
-    if (p > 0) __syncthreads();   // the previous plane is read
-    stage_plane(sm, p, sx, [&](int, int slot) { return pos[slot]; });
-    if (!act) continue;
-    plane_groups<kTile>(lst, plane_begin(p, bnd), plane_end(p, bnd, m), stage_mem,
-                        [&](int u, double xj, double yj, double zj) {
+  const double xi = qi.x, yi = qi.y, zi = qi.z;
+  if (act) {
+    plane_groups<kBlk>(lst, 0, m, stage_mem, [&](int u, double xj, double yj, double zj) {
       if (SCALED) {
         const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
         const double df = pair_df_lj(dx * dx + dy * dy + dz * dz, lj, dt);
@@ -1200,7 +1388,10 @@ __global__ void __launch_bounds__(kTile) k_kick_tiles(
     }
   }
   if (MODE != 1) block_max_to(v2, partial);
-  if (nh) tile_sum_to(ke, hk.ke_partial);
+  if (nh) {
+    const double kv[1] = {ke};
+    block_sums_to<1>(kv, hk.ke_partial);
+  }
 }
 
 // The guard ending a step (block 0) and the image refresh of the next
@@ -1372,21 +1563,6 @@ __global__ void k_langevin(double4 *__restrict__ mom, const int *__restrict__ ta
 }
 
 // ------------------------------------------------------------ adjoint
-// Fixed-order block sum of K values per thread into partial[block * K + k].
-template <int K>
-__device__ __forceinline__ void block_sums_to(const double (&v)[K], double *__restrict__ partial) {
-  __shared__ double red[K][kTile];
-  const int t = threadIdx.x;
-  for (int k = 0; k < K; k++) red[k][t] = v[k];
-  __syncthreads();
-  for (int off = kTile / 2; off > 0; off >>= 1) {
-    if (t < off)
-      for (int k = 0; k < K; k++) red[k][t] += red[k][t + off];
-    __syncthreads();
-  }
-  if (t < K) partial[(size_t)blockIdx.x * K + t] = red[t][0];
-}
-
 template <int K>
 __global__ void k_sum_partials_k(const double *__restrict__ partial, int nblocks,
                                  double *__restrict__ out, double scale) {
@@ -1414,36 +1590,34 @@ __global__ void k_sum_partials_k(const double *__restrict__ partial, int nblocks
 //     ap_i += ck p_i,
 //   and block sums (dPE/deps, dPE/dsigma, dW/deps, dW/dsigma).
 template <bool GRAD>
-__global__ void __launch_bounds__(kTile) k_lj_observe_tiles(
+__global__ void __launch_bounds__(kBlk) k_lj_observe_tiles(
     const double4 *__restrict__ pos, const double4 *__restrict__ mom,
-    const TileDesc *__restrict__ tiles, int ntiles, Geom g, Cells c, int n, int kcap,
-    const unsigned short *__restrict__ list, const int *__restrict__ nn, const int *__restrict__ nb,
-    LJ lj, double cw, double ce, double ck, double4 *__restrict__ aq, double4 *__restrict__ ap,
-    double *__restrict__ partial) {
+    const BlkTile *__restrict__ tiles, int ntiles, const int *__restrict__ treal,
+    const int *__restrict__ sidx, int kcap, const unsigned short *__restrict__ list,
+    const int *__restrict__ nn, LJ lj, double cw, double ce, double ck, double4 *__restrict__ aq,
+    double4 *__restrict__ ap, double *__restrict__ partial) {
   extern __shared__ double stage_mem[];
-  __shared__ TileSmem sm;
   constexpr int K = GRAD ? 4 : 3;
   const int tile = tile_of_block(blockIdx.x, ntiles);
   if (tile >= ntiles) {
     if (threadIdx.x < K) partial[(size_t)blockIdx.x * K + threadIdx.x] = 0.0;
     return;
   }
-  const TileDesc t = tiles[tile];
+  const BlkTile t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-  tile_load_segments(c, tile, sm);
   const int lane = threadIdx.x;
-  const bool act = lane < t.ni;
-  const int i = t.i0 + (act ? c.tperm[(size_t)tile * kTile + lane] : 0);
+  const bool act = lane < t.nreal;
+  const int i = act ? treal[(size_t)tile * kBlk + lane] : 0;
   const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
   const double4 qi = act ? pos[i] : z4;
   const double4 p = act ? mom[i] : z4;
-  const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
+  const unsigned short *lst = list + (size_t)tile * kcap * kBlk + (size_t)lane * kILP;
   double acc[K];
   for (int k = 0; k < K; k++) acc[k] = 0.0;
   double pe = 0.0, w = 0.0, dpe_s = 0.0, dw_s = 0.0;
   double gwx = 0.0, gwy = 0.0, gwz = 0.0, gex = 0.0, gey = 0.0, gez = 0.0;
-  walk_planes(sm, sx, [&](int, int slot) { return pos[slot]; }, [](int) {}, act, lst,
-              act ? nn[i] : 0, act ? nb[i] : 0, [&](int o) {
+  stage_tile<kBlk>(t.S, sidx + t.s0, stage_mem, [&](int slot) { return pos[slot]; });
+  for_each_partner(lst, act ? nn[i] : 0, (unsigned)(kEntryBytes * t.S), [&](int o) {
       const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
       const double s2 = dx * dx + dy * dy + dz * dz;
       double phi, dphi, V;
@@ -1464,7 +1638,7 @@ __global__ void __launch_bounds__(kTile) k_lj_observe_tiles(
       }
   });
   if (act) {
-    if (!GRAD) {
+    if constexpr (!GRAD) {
       acc[0] = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z;
       acc[1] = 0.5 * pe;
       acc[2] = 0.5 * w / 3.0;
@@ -1513,52 +1687,37 @@ __global__ void k_adj_pre(const double4 *__restrict__ ap, const double4 *__restr
 //   ap_i += aq_i dt/2,
 //   block sums dt sum_i ap_i . dF_i/d(eps, sigma).
 // Partner adjoints come from ext[] through the staged slot index.
-__global__ void __launch_bounds__(kTile) k_adj_hvp_tiles(
-    const double4 *__restrict__ pos, const TileDesc *__restrict__ tiles, int ntiles, Geom g,
-    Cells c, int n, int kcap, const unsigned short *__restrict__ list, const int *__restrict__ nn,
-    const int *__restrict__ nb, LJ lj, double dt, const double4 *__restrict__ ext,
-    double4 *__restrict__ aq,
-    double4 *__restrict__ ap, double *__restrict__ partial) {
+__global__ void __launch_bounds__(kBlk) k_adj_hvp_tiles(
+    const double4 *__restrict__ pos, const BlkTile *__restrict__ tiles, int ntiles,
+    const int *__restrict__ treal, const int *__restrict__ sidx, int kcap,
+    const unsigned short *__restrict__ list, const int *__restrict__ nn, LJ lj, double dt,
+    const double4 *__restrict__ ext, double4 *__restrict__ aq, double4 *__restrict__ ap,
+    double *__restrict__ partial) {
   extern __shared__ double stage_mem[];
-  __shared__ TileSmem sm;
   const int tile = tile_of_block(blockIdx.x, ntiles);
   if (tile >= ntiles) {
     if (threadIdx.x < 2) partial[(size_t)blockIdx.x * 2 + threadIdx.x] = 0.0;
     return;
   }
-  const TileDesc t = tiles[tile];
+  const BlkTile t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-  tile_load_segments(c, tile, sm);
-  // slot of the plane's staged particle o / 3, after the largest plane of
-  // the tile and its sentinel
-  int lmax = 0;
-  for (int p = 0; p < kPlanes; p++) lmax = max(lmax, sm.seg_base[6 * p + 6] - sm.seg_base[6 * p]);
-  int *sidx = reinterpret_cast<int *>(stage_mem + 3 * (lmax + 1));
+  const int *tsidx = sidx + t.s0;   // global slot of each staged particle
   const int lane = threadIdx.x;
-  const bool act = lane < t.ni;
-  const int i = t.i0 + (act ? c.tperm[(size_t)tile * kTile + lane] : 0);
+  const bool act = lane < t.nreal;
+  const int i = act ? treal[(size_t)tile * kBlk + lane] : 0;
   const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
   const double4 qi = act ? pos[i] : z4;
   const double4 ai = act ? ext[i] : z4;
-  const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
+  const unsigned short *lst = list + (size_t)tile * kcap * kBlk + (size_t)lane * kILP;
   double acc[2] = {0.0, 0.0};
   double jx = 0.0, jy = 0.0, jz = 0.0, te = 0.0, ts = 0.0;
-  auto slots = [&](int p) {
-    const int p0 = sm.seg_base[6 * p];
-    for (int k = 6 * p; k < 6 * p + 6; k++) {
-      const int b = sm.seg_base[k] - p0, len = sm.seg_base[k + 1] - sm.seg_base[k];
-      const int st = sm.seg_start[k];
-      for (int u = threadIdx.x; u < len; u += blockDim.x) sidx[b + u] = st + u;
-    }
-    __syncthreads();
-  };
-  walk_planes(sm, sx, [&](int, int slot) { return pos[slot]; }, slots, act, lst,
-              act ? nn[i] : 0, act ? nb[i] : 0, [&](int o) {
+  stage_tile<kBlk>(t.S, tsidx, stage_mem, [&](int slot) { return pos[slot]; });
+  for_each_partner(lst, act ? nn[i] : 0, (unsigned)(kEntryBytes * t.S), [&](int o) {
       const double dx = sx[o] - qi.x, dy = sy[o] - qi.y, dz = sz[o] - qi.z;
       const double s2 = dx * dx + dy * dy + dz * dz;
       double phi, dphi, V;
       lj_terms(s2, lj, &phi, &dphi, &V);
-      const double4 aj = ext[sidx[o / 3]];
+      const double4 aj = ext[tsidx[o / 3]];
       const double wx = aj.x - ai.x, wy = aj.y - ai.y, wz = aj.z - ai.z;
       const double dw = 2.0 * dphi * (dx * wx + dy * wy + dz * wz);
       jx += phi * wx + dw * dx;
@@ -1604,33 +1763,30 @@ __global__ void k_gather4_by_tag(const double4 *__restrict__ by_id, const int *_
 // PE = sum 0.5 * 4 (r^-12 - r^-6 + C2 r^2 + C0) (observer.cc:18-47) and
 // virial = sum 0.5 * r V'(r) (observer.cc:50-79); every pair sits in two
 // lists, hence the halves.  Fixed-order tree within the block.
-__global__ void __launch_bounds__(kTile) k_observe_tiles(
+__global__ void __launch_bounds__(kBlk) k_observe_tiles(
     const double4 *__restrict__ pos, const double4 *__restrict__ mom,
-    const TileDesc *__restrict__ tiles, int ntiles, Geom g, Cells c, int n, int kcap,
-    const unsigned short *__restrict__ list, const int *__restrict__ nn, const int *__restrict__ nb,
-    double C0, double C2, double *__restrict__ partial) {
+    const BlkTile *__restrict__ tiles, int ntiles, const int *__restrict__ treal,
+    const int *__restrict__ sidx, int kcap, const unsigned short *__restrict__ list,
+    const int *__restrict__ nn, double CL2, double C0, double C2, double *__restrict__ partial) {
   extern __shared__ double stage_mem[];
-  __shared__ TileSmem sm;
-  __shared__ double red[3][kTile];
   const int tile = tile_of_block(blockIdx.x, ntiles);
   if (tile >= ntiles) {
     if (threadIdx.x < 3) partial[(size_t)blockIdx.x * 3 + threadIdx.x] = 0.0;
     return;
   }
-  const TileDesc t = tiles[tile];
+  const BlkTile t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-  tile_load_segments(c, tile, sm);
   const int lane = threadIdx.x;
-  const bool act = lane < t.ni;
-  const int i = t.i0 + (act ? c.tperm[(size_t)tile * kTile + lane] : 0);
+  const bool act = lane < t.nreal;
+  const int i = act ? treal[(size_t)tile * kBlk + lane] : 0;
   const double4 qi = act ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
-  const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
+  const unsigned short *lst = list + (size_t)tile * kcap * kBlk + (size_t)lane * kILP;
   double ke = 0.0, pe = 0.0, vir = 0.0;
-  walk_planes(sm, sx, [&](int, int slot) { return pos[slot]; }, [](int) {}, act, lst,
-              act ? nn[i] : 0, act ? nb[i] : 0, [&](int o) {
+  stage_tile<kBlk>(t.S, sidx + t.s0, stage_mem, [&](int slot) { return pos[slot]; });
+  for_each_partner(lst, act ? nn[i] : 0, (unsigned)(kEntryBytes * t.S), [&](int o) {
     const double dx = qi.x - sx[o], dy = qi.y - sy[o], dz = qi.z - sz[o];
     const double r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 <= g.CL2) observe_term(r2, C0, C2, pe, vir);
+    if (r2 <= CL2) observe_term(r2, C0, C2, pe, vir);
   });
   if (act) {
     // all four lanes, as the reference's observer (the fourth is zero
@@ -1640,19 +1796,8 @@ __global__ void __launch_bounds__(kTile) k_observe_tiles(
     pe *= 0.5;
     vir *= 0.5;
   }
-  red[0][lane] = ke;
-  red[1][lane] = pe;
-  red[2][lane] = vir;
-  __syncthreads();
-  for (int off = kTile / 2; off > 0; off >>= 1) {
-    if (lane < off) {
-      red[0][lane] += red[0][lane + off];
-      red[1][lane] += red[1][lane + off];
-      red[2][lane] += red[2][lane + off];
-    }
-    __syncthreads();
-  }
-  if (lane < 3) partial[(size_t)blockIdx.x * 3 + lane] = red[lane][0];
+  const double v[3] = {ke, pe, vir};
+  block_sums_to<3>(v, partial);
 }
 
 __global__ void k_sum_partials(const double *__restrict__ partial, int nblocks,
@@ -1726,7 +1871,7 @@ __device__ __forceinline__ void partner_id(int j, int n, const int *tag, const i
 }
 
 // each pair sits in both lists; emit it from the side with the smaller id.
-// LDS offsets map back to global slots through the tile's segments.
+// List entries map back to global slots through the tile's sidx.
 // Counts and offsets are indexed by particle id, so after an exclusive scan
 // in id order the triples of id_lo = t occupy one run starting at offset[t],
 // and the runs are in ascending id_lo.
@@ -1736,29 +1881,25 @@ __device__ __forceinline__ void partner_id(int j, int n, const int *tag, const i
 // real's side (a pair across the box then appears once from each side, as
 // the reference's real-ghost pairs do).  The partner index is written.
 template <bool WRITE, bool CSR = false>
-__global__ void __launch_bounds__(kTile) k_canonical_tiles(
-    const TileDesc *__restrict__ tiles, int ntiles, Geom g, Cells c, int n, int kcap,
-    const unsigned short *__restrict__ list, const int *__restrict__ nn, const int *__restrict__ nb,
-    const int *__restrict__ tag, const int *__restrict__ gsrc, const int *__restrict__ gcode,
-    const int *__restrict__ offset,
-    int *__restrict__ count, int *__restrict__ out) {
-  __shared__ TileSmem sm;
+__global__ void __launch_bounds__(kBlk) k_canonical_tiles(
+    const BlkTile *__restrict__ tiles, int ntiles, const int *__restrict__ treal,
+    const int *__restrict__ sidx, int n, int kcap, const unsigned short *__restrict__ list,
+    const int *__restrict__ nn, const int *__restrict__ tag, const int *__restrict__ gsrc,
+    const int *__restrict__ gcode, const int *__restrict__ offset, int *__restrict__ count,
+    int *__restrict__ out) {
   const int tile = blockIdx.x;
   if (tile >= ntiles) return;
-  const TileDesc t = tiles[tile];
-  tile_load_segments(c, tile, sm);
+  const BlkTile t = tiles[tile];
   const int lane = threadIdx.x;
-  if (lane >= t.ni) return;
-  const int i = t.i0 + c.tperm[(size_t)tile * kTile + lane];
+  if (lane >= t.nreal) return;
+  const int i = treal[(size_t)tile * kBlk + lane];
   const int ti = tag[i];
-  const unsigned short *lst = list + (size_t)tile * kcap * kTile + (size_t)lane * kILP;
+  const int *tsidx = sidx + t.s0;
+  const unsigned short *lst = list + (size_t)tile * kcap * kBlk + (size_t)lane * kILP;
   int cnum = 0;
   int w = WRITE ? offset[ti] : 0;
-  for_each_partner(lst, nn[i], nb[i], sm, [&](int o3) {
-    const int o = o3 / 3;
-    int sgi = 0;
-    while (o >= sm.seg_base[sgi + 1]) sgi++;
-    const int j = sm.seg_start[sgi] + (o - sm.seg_base[sgi]);
+  for_each_partner(lst, nn[i], (unsigned)(kEntryBytes * t.S), [&](int o3) {
+    const int j = tsidx[o3 / 3];
     if (CSR) {
       const int hj = j < n ? tag[j] : j;
       if (j >= n || ti < hj) {
@@ -1876,7 +2017,7 @@ struct mdx_system {
   bool own_stream = false;
   Geom geom{};
   double SL = 0, C0 = 0, C2 = 0;
-  int ncell = 0, cell_cap = 0, row_cap = 0;
+  int ncell = 0, cell_cap = 0;
   // sizes
   int n = 0, g = 0, cap = 0, gcap = 0, kcap = 0;
   size_t tot_elems = 0;     // length of pos / ref_pos / cellid / ... arrays
@@ -1890,13 +2031,18 @@ struct mdx_system {
   int *cell_count = nullptr, *cell_start = nullptr, *cnt_in = nullptr, *cnt_out = nullptr,
       *scan_in = nullptr, *scan_out = nullptr, *scan_aux = nullptr;
   int *nn = nullptr, *nb = nullptr;   // list length; plane starts in it
-  // tiles and their 16-bit lists
-  TileDesc *tiles = nullptr;
-  int *tseg = nullptr;        // [tiles_cap][kTSeg]
-  unsigned char *tperm = nullptr;   // [tiles_cap][kTile] real of each list column
-  int ntiles = 0, tiles_cap = 0, stage_max = 0, plane_max = 0;
-  int *row_tiles = nullptr, *row_off = nullptr;
-  unsigned short *list16 = nullptr;
+  // tiles over blocks of cell rows and their 16-bit lists
+  BlkTile *tiles = nullptr;
+  BlkInfo *tinfo = nullptr;
+  int *treal = nullptr;                        // [tiles_cap][kBlk] real of each ranked column
+  int *tile_S = nullptr, *tile_s0 = nullptr;   // [tiles_cap] staging sizes, their prefix sums
+  int *sidx = nullptr;                         // global slot of every staged particle
+  size_t sidx_cap = 0;
+  int *blk_nt = nullptr, *blk_toff = nullptr;  // [blk_cap] tiles of each block, their prefix
+  int blk_cap = 0, blk_level = 0;   // block shape in use (kBlkLadder)
+  int ntiles = 0, tiles_cap = 0, stage_max = 0;
+  unsigned short *list16 = nullptr;     // ranked columns, as the list walkers read them
+  unsigned short *list_nat = nullptr;   // the build's columns, in the tile's row order
   size_t list16_elems = 0;
   size_t lds_limit = 65536;
   int partial_blocks = 0;
@@ -2038,24 +2184,27 @@ int ensure_image_capacity(mdx_system *s, int gcap) {
 int ensure_list_capacity(mdx_system *s, int kcap) {
   // padded by one group of entries, which the kick reads past a list's end
   kcap = (kcap + kILP - 1) / kILP * kILP;   // whole groups per column
-  const size_t need = (size_t)std::max(s->ntiles, 1) * kcap * kTile + (size_t)kILP * kTile;
+  const size_t need = (size_t)std::max(s->ntiles, 1) * kcap * kBlk + (size_t)kILP * kBlk;
   if (need <= s->list16_elems && kcap <= s->kcap) return MDX_OK;
   int rc = dev_alloc(s, &s->list16, need);
-  if (rc) return rc;
+  if (rc || (rc = dev_alloc(s, &s->list_nat, need))) return rc;
   s->list16_elems = need;
   s->kcap = kcap;
   return MDX_OK;
 }
 
 // dynamic LDS of the tile kernels: the staged particles plus the sentinel
-// (the adjoint's HVP adds a slot index per staged particle)
-size_t adj_stage_bytes(const mdx_system *s) {
-  return (size_t)(s->plane_max + 1) * (3 * sizeof(double) + sizeof(int)) + 8;
+size_t stage_bytes(const mdx_system *s) { return (size_t)(s->stage_max + 1) * 3 * sizeof(double); }
+
+// the largest staging a tile may have: its sentinel's list entry fits 16
+// bits, and the staging fits the LDS beside the kernels' own (4 KiB)
+int stage_cap(const mdx_system *s) {
+  const long long lds = ((long long)s->lds_limit - 4096) / (3 * (long long)sizeof(double)) - 1;
+  return (int)std::min<long long>(kMaxPlane, lds);
 }
-size_t plane_bytes(const mdx_system *s) { return (size_t)(s->plane_max + 1) * 3 * sizeof(double); }
 
 Cells cells_of(const mdx_system *s) {
-  return Cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out, s->tseg, s->tperm};
+  return Cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out};
 }
 
 // per-block partial buffers for every grid the library launches
@@ -2088,6 +2237,8 @@ int set_geometry(mdx_system *s, const double L[3]) {
   for (int d = 0; d < 3; d++)
     if (!(L[d] >= 2.0 * SL))
       return fail(s, MDX_ERR_BOX, "box edge %g < 2 x search length %g", L[d], SL);
+  if (!(L[0] / SL < (double)kMaxCellsX))
+    return fail(s, MDX_ERR_BOX, "box edge %g in x holds more than %d cells", L[0], kMaxCellsX);
   Geom g{};
   for (int d = 0; d < 3; d++) {
     g.L[d] = L[d];
@@ -2098,7 +2249,7 @@ int set_geometry(mdx_system *s, const double L[3]) {
   }
   g.SL2 = SL * SL;
   g.CL2 = s->prm.cutoff * s->prm.cutoff;
-  const int ncell = g.m[0] * g.m[1] * g.m[2], rows = g.ms[1] * g.ms[2];
+  const int ncell = g.m[0] * g.m[1] * g.m[2], blocks = blk_count(g, 1, 1) * blk_count(g, 2, 1);
   int rc;
   if (ncell > s->cell_cap) {
     if ((rc = dev_alloc(s, &s->cell_count, ncell)) || (rc = dev_alloc(s, &s->cell_start, ncell)) ||
@@ -2111,9 +2262,9 @@ int set_geometry(mdx_system *s, const double L[3]) {
       if ((rc = dev_alloc(s, &s->scan_aux, tiles))) return rc;
     }
   }
-  if (rows > s->row_cap) {
-    if ((rc = dev_alloc(s, &s->row_tiles, rows)) || (rc = dev_alloc(s, &s->row_off, rows))) return rc;
-    s->row_cap = rows;
+  if (blocks > s->blk_cap) {
+    if ((rc = dev_alloc(s, &s->blk_nt, blocks)) || (rc = dev_alloc(s, &s->blk_toff, blocks))) return rc;
+    s->blk_cap = blocks;
   }
   s->geom = g;
   s->ncell = ncell;
@@ -2186,47 +2337,65 @@ int build_pairlist(mdx_system *s) {
   std::swap(s->mom, s->mom_tmp);
   std::swap(s->tag, s->tag_tmp);
   if ((rc = launch_check(s, "sort"))) return rc;
-  // 5. tiles over the interior cell rows
-  Cells cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out, s->tseg, s->tperm};
-  const int rows = s->geom.ms[1] * s->geom.ms[2];
-  k_row_tiles<<<nblk(rows), kBlock, 0, s->stream>>>(s->geom, cells, s->row_tiles);
-  if ((rc = scan(s, s->row_tiles, s->row_off, rows, &s->dstat->ntiles))) return rc;
-  // a row of c reals has ceil(c / kTile) <= c / kTile + 1 tiles: sized for
-  // that bound, the tile arrays never need the count before they are written
-  const int tiles_bound = rows + (n + kTile - 1) / kTile;
-  if (tiles_bound > s->tiles_cap) {
-    const size_t cap = (size_t)tiles_bound;
-    if ((rc = dev_alloc(s, &s->tiles, cap)) || (rc = dev_alloc(s, &s->tseg, cap * kTSeg)) ||
-        (rc = dev_alloc(s, &s->tperm, cap * kTile)))
-      return rc;
-    s->tiles_cap = (int)cap;
+  // 5. tiles over blocks of cell rows, their staging sizes (a lower target
+  // of reals per tile when a staging would not fit), then the staging index
+  const Cells cells = cells_of(s);
+  const int cap = stage_cap(s);
+  for (;;) {
+    const BlkShape shape = kBlkLadder[s->blk_level];
+    const int br = shape.rows;
+    const int nblocks = blk_count(s->geom, 1, br) * blk_count(s->geom, 2, br);
+    k_blk_count<<<nblocks, kBlock, 0, s->stream>>>(s->pos, s->geom, cells, br, shape.target,
+                                                  s->blk_nt);
+    if ((rc = scan(s, s->blk_nt, s->blk_toff, nblocks, &s->dstat->ntiles))) return rc;
+    if ((rc = sync_status(s))) return rc;
+    const int ntiles = s->hstat->ntiles;
+    if (ntiles > s->tiles_cap) {
+      const size_t tc = (size_t)ntiles + ntiles / 8 + 64;
+      if ((rc = dev_alloc(s, &s->tiles, tc)) || (rc = dev_alloc(s, &s->tinfo, tc)) ||
+          (rc = dev_alloc(s, &s->treal, tc * kBlk)) || (rc = dev_alloc(s, &s->tile_S, tc)) ||
+          (rc = dev_alloc(s, &s->tile_s0, tc)))
+        return rc;
+      s->tiles_cap = (int)tc;
+    }
+    if ((rc = ensure_partials(s, std::max(tile_grid(ntiles), nblocks)))) return rc;
+    HIPCHK(s, hipMemsetAsync(&s->dstat->flags, 0, sizeof(int), s->stream));
+    k_blk_tiles<<<nblocks, kBlock, 0, s->stream>>>(s->pos, s->geom, cells, n, br, s->blk_nt,
+                                                     s->blk_toff, cap, s->tiles, s->tinfo,
+                                                     s->tile_S, s->pmax, &s->dstat->flags);
+    k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblocks, &s->dstat->stage_max);
+    if ((rc = scan(s, s->tile_S, s->tile_s0, ntiles, &s->dstat->stage_total))) return rc;
+    if ((rc = launch_check(s, "tiles"))) return rc;
+    if ((rc = sync_status(s))) return rc;
+    if (s->hstat->flags & ST_TILE_OVERFLOW)
+      return fail(s, MDX_ERR_CAPACITY, "more than %d reals share one x in a block of cell rows", kBlk);
+    if (!(s->hstat->flags & ST_STAGE_OVERFLOW)) {
+      s->ntiles = ntiles;
+      s->stage_max = (int)s->hstat->stage_max;
+      break;
+    }
+    if (s->blk_level + 1 >= kBlkLevels)
+      return fail(s, MDX_ERR_CAPACITY,
+                  "a tile of %d reals of one cell row stages %d particles, more than %d (16-bit "
+                  "list offsets, LDS)",
+                  shape.target, (int)s->hstat->stage_max, cap);
+    s->blk_level++;
   }
-  cells.tseg = s->tseg;
-  cells.tperm = s->tperm;
-  const int wt_blocks = (int)(((size_t)rows * 64 + kBlock - 1) / kBlock);
-  if ((rc = ensure_partials(s, std::max(tile_grid(tiles_bound), 2 * wt_blocks)))) return rc;
-  k_write_tiles<<<wt_blocks, kBlock, 0, s->stream>>>(s->geom, cells, n, s->pos, s->row_tiles,
-                                                       s->row_off, s->tiles, s->tseg, s->pmax);
-  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, wt_blocks, &s->dstat->stage_max);
-  k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax + wt_blocks, wt_blocks, &s->dstat->plane_max);
-  if ((rc = sync_status(s))) return rc;
-  s->ntiles = s->hstat->ntiles;
-  s->stage_max = (int)s->hstat->stage_max;
-  s->plane_max = (int)s->hstat->plane_max;
-  if (adj_stage_bytes(s) + 4096 > s->lds_limit)
-    return fail(s, MDX_ERR_CAPACITY, "tile plane staging of %d particles exceeds LDS (%zu B)",
-                s->plane_max, s->lds_limit);
-  if (s->plane_max > kMaxPlane)
-    return fail(s, MDX_ERR_CAPACITY,
-                "a tile plane of %d particles exceeds the 16-bit list offsets (%d particles)",
-                s->plane_max, kMaxPlane);
+  const size_t staged = (size_t)std::max(s->hstat->stage_total, 1);
+  if (staged > s->sidx_cap) {
+    const size_t sc = staged + staged / 8 + 4096;
+    if ((rc = dev_alloc(s, &s->sidx, sc))) return rc;
+    s->sidx_cap = sc;
+  }
+  k_blk_sidx<<<s->ntiles, kBlock, 0, s->stream>>>(s->geom, cells, n, s->ntiles, s->tile_s0,
+                                                   s->tiles, s->tinfo, s->sidx);
   if ((rc = ensure_list_capacity(s, s->kcap))) return rc;
   // 6. neighbour lists, one pass at the current width, again if it overflows
   for (int attempt = 0; attempt < 3; attempt++) {
     HIPCHK(s, hipMemsetAsync(&s->dstat->flags, 0, sizeof(int), s->stream));
-    k_build_tiles<<<tile_grid(s->ntiles), kTile, plane_bytes(s), s->stream>>>(
-        s->pos, s->tiles, s->ntiles, s->geom, cells, n, s->kcap, s->list16, s->nn, s->nb, s->tperm,
-        s->pmax, &s->dstat->flags);
+    k_build_tiles<<<tile_grid(s->ntiles), kBlk, stage_bytes(s), s->stream>>>(
+        s->pos, s->tiles, s->tinfo, s->sidx, s->ntiles, s->geom, s->kcap, s->list_nat, s->list16,
+        s->nn, s->nb, s->treal, s->pmax, &s->dstat->flags);
     k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, tile_grid(s->ntiles), &s->dstat->kmax_d);
     if ((rc = launch_check(s, "list"))) return rc;
     if ((rc = sync_status(s))) return rc;
@@ -2250,9 +2419,9 @@ int build_pairlist(mdx_system *s) {
 int kick(mdx_system *s) {
   const int grid = tile_grid(s->ntiles);
   ev_record(s, 2);
-  k_kick_tiles<0><<<grid, kTile, plane_bytes(s), s->stream>>>(
-      s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), s->n, s->kcap, s->list16, s->nn,
-      s->nb, make_pairk(s->C2, s->prm.dt), s->prm.dt, nullptr, s->pmax);
+  k_kick_tiles<0><<<grid, kBlk, stage_bytes(s), s->stream>>>(
+      s->pos, s->mom, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->nn,
+      make_pairk(s->C2, s->prm.dt), s->prm.dt, s->geom.CL2, nullptr, s->pmax);
   ev_record(s, 3);
   s->vmax_valid = false;   // recomputed from p when the expiry check needs it
   s->n_kick++;
@@ -2339,9 +2508,9 @@ int run_batch(mdx_system *s, int want, int first_checked, int *ran) {
     }
     if (s->timing) (void)hipEventRecord(s->evk[2 * b], s->stream);
     hk.k = s->langevin_k + (unsigned long long)b;   // Langevin kicks since mdx_set_state
-    k_kick_tiles<3><<<grid, kTile, plane_bytes(s), s->stream>>>(
-        cur, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn,
-        s->nb, make_pairk(s->C2, dt), dt, s->pos, s->pmax, LJ{}, &s->dctl->halt, nxt, hk);
+    k_kick_tiles<3><<<grid, kBlk, stage_bytes(s), s->stream>>>(
+        cur, s->mom, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->nn,
+        make_pairk(s->C2, dt), dt, s->geom.CL2, s->pos, s->pmax, LJ{}, &s->dctl->halt, nxt, hk);
     if (s->timing) (void)hipEventRecord(s->evk[2 * b + 1], s->stream);
     const int refresh_blocks = (g > 0 && b + 1 < B) ? (g + kGuard - 1) / kGuard : 0;
     k_guard_refresh<<<1 + refresh_blocks, kGuard, 0, s->stream>>>(
@@ -2467,10 +2636,9 @@ int expired(mdx_system *s, int *out) {
 int reduce_observe(mdx_system *s, double *ke, double *pe, double *vir) {
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "observe before a pairlist exists");
   const int nb = tile_grid(s->ntiles);
-  k_observe_tiles<<<nb, kTile, plane_bytes(s), s->stream>>>(s->pos, s->mom, s->tiles, s->ntiles,
-                                                            s->geom, cells_of(s), s->n, s->kcap,
-                                                            s->list16, s->nn, s->nb, s->C0,
-                                                            s->C2, s->partial);
+  k_observe_tiles<<<nb, kBlk, stage_bytes(s), s->stream>>>(
+      s->pos, s->mom, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->nn,
+      s->geom.CL2, s->C0, s->C2, s->partial);
   k_sum_partials<<<1, kBlock, 0, s->stream>>>(s->partial, nb, s->dstat->sums);
   int rc = launch_check(s, "observe");
   if (rc) return rc;
@@ -2523,9 +2691,9 @@ int lj_step(mdx_system *s, const LJ &lj, double4 *mid) {
   k_drift_refresh<<<nblk(n + g), kBlock, 0, s->stream>>>(s->pos, s->mom, s->gsrc, s->gcode, n, g,
                                                           s->prm.dt * 0.5, s->geom.L[0],
                                                           s->geom.L[1], s->geom.L[2], mid, nullptr);
-  k_kick_tiles<2, true><<<grid, kTile, plane_bytes(s), s->stream>>>(
-      mid, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb,
-      make_pairk(s->C2, s->prm.dt), s->prm.dt, s->pos, s->pmax, lj);
+  k_kick_tiles<2, true><<<grid, kBlk, stage_bytes(s), s->stream>>>(
+      mid, s->mom, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->nn,
+      make_pairk(s->C2, s->prm.dt), s->prm.dt, s->geom.CL2, s->pos, s->pmax, lj);
   return launch_check(s, "lj step");
 }
 
@@ -2580,9 +2748,9 @@ int lj_trajectory(mdx_system *s, double eps, double sigma, int nsteps, int nobs,
       if (k >= nsteps - nobs) {
         if ((rc = refresh(s))) return rc;
         const int nb = tile_grid(s->ntiles);
-        k_lj_observe_tiles<false><<<nb, kTile, plane_bytes(s), s->stream>>>(
-            s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16,
-            s->nn, s->nb, lj, 0.0, 0.0, 0.0, nullptr, nullptr, s->partial);
+        k_lj_observe_tiles<false><<<nb, kBlk, stage_bytes(s), s->stream>>>(
+            s->pos, s->mom, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->nn, lj,
+            0.0, 0.0, 0.0, nullptr, nullptr, s->partial);
         k_sum_partials_k<3><<<1, kBlock, 0, s->stream>>>(s->partial, nb, obs + 3 * (size_t)k, 1.0);
       }
     }
@@ -2635,8 +2803,6 @@ int lj_trajectory(mdx_system *s, double eps, double sigma, int nsteps, int nobs,
       // the checkpoint is already in cell order: the stable sort keeps it and
       // the list, tiles and images come out as in the forward pass
       if ((rc = build_pairlist(s))) return rc;
-      if (adj_stage_bytes(s) + 4096 > s->lds_limit)
-        return fail(s, MDX_ERR_CAPACITY, "adjoint tile staging exceeds LDS");
       const int T = s->n + s->g;
       k_gather4_by_tag<<<nblk(n), kBlock, 0, s->stream>>>(aid_q, s->tag, n, aq);
       k_gather4_by_tag<<<nblk(n), kBlock, 0, s->stream>>>(aid_p, s->tag, n, ap);
@@ -2654,16 +2820,16 @@ int lj_trajectory(mdx_system *s, double eps, double sigma, int nsteps, int nobs,
       for (int k = k1 - 1; k >= k0; k--) {
         if (k >= nsteps - nobs) {
           // sensitivity of the observation after step k
-          k_lj_observe_tiles<true><<<nb, kTile, plane_bytes(s), s->stream>>>(
+          k_lj_observe_tiles<true><<<nb, kBlk, stage_bytes(s), s->stream>>>(
               st_end + (size_t)(k - k0) * totcap, st_mom + (size_t)(k - k0) * n, s->tiles,
-              s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, lj, cw, ce,
-              ck, aq, ap, s->partial);
+              s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->nn, lj, cw, ce, ck, aq, ap,
+              s->partial);
           k_sum_partials_k<4><<<1, kBlock, 0, s->stream>>>(s->partial, nb, theta + 4 * (size_t)slot++, 1.0);
         }
         k_adj_pre<<<nblk(T), kBlock, 0, s->stream>>>(ap, aq, s->gsrc, n, s->g, s->prm.dt * 0.5, aext);
-        k_adj_hvp_tiles<<<nb, kTile, adj_stage_bytes(s), s->stream>>>(
-            st_mid + (size_t)(k - k0) * totcap, s->tiles, s->ntiles, s->geom, cells_of(s), n,
-            s->kcap, s->list16, s->nn, s->nb, lj, s->prm.dt, aext, aq, ap, s->partial);
+        k_adj_hvp_tiles<<<nb, kBlk, stage_bytes(s), s->stream>>>(
+            st_mid + (size_t)(k - k0) * totcap, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap,
+            s->list16, s->nn, lj, s->prm.dt, aext, aq, ap, s->partial);
         k_sum_partials_k<2><<<1, kBlock, 0, s->stream>>>(s->partial, nb, theta + 4 * (size_t)slot++, 1.0);
         if ((rc = launch_check(s, "adjoint step"))) return rc;
       }
@@ -2807,7 +2973,8 @@ int mdx_create(const mdx_params *prm, mdx_system **out) {
 int mdx_destroy(mdx_system *s) {
   if (!s) return MDX_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  void *bufs[] = {s->pos, s->pos_alt, s->pos_mid, s->tseg, s->tperm, s->pmax, s->tiles, s->row_tiles, s->row_off, s->list16,
+  void *bufs[] = {s->pos, s->pos_alt, s->pos_mid, s->tinfo, s->treal, s->tile_S, s->tile_s0, s->sidx,
+                  s->blk_nt, s->blk_toff, s->pmax, s->tiles, s->list16, s->list_nat,
                   s->pos_tmp, s->ref_pos, s->mom, s->mom_tmp, s->gpos_tmp, s->vec_out,
                   s->tag, s->tag_tmp, s->gsrc, s->gcode, s->gsrc_tmp, s->gcode_tmp, s->cellid,
                   s->slot, s->order, s->inv, s->nimg, s->goff, s->cell_count, s->cell_start,
@@ -3087,9 +3254,9 @@ int mdx_observe(mdx_system *s, double *ke, double *pe, double *vir) {
 int mdx_forces(mdx_system *s, double *f3) {
   if (!s || !f3) return fail(s, MDX_ERR_ARG, "null argument");
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
-  k_kick_tiles<1><<<tile_grid(s->ntiles), kTile, plane_bytes(s), s->stream>>>(
-      s->pos, s->mom, s->tiles, s->ntiles, s->geom, cells_of(s), s->n, s->kcap, s->list16, s->nn,
-      s->nb, make_pairk(s->C2, 1.0), 1.0, s->vec_out, nullptr);
+  k_kick_tiles<1><<<tile_grid(s->ntiles), kBlk, stage_bytes(s), s->stream>>>(
+      s->pos, s->mom, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->nn,
+      make_pairk(s->C2, 1.0), 1.0, s->geom.CL2, s->vec_out, nullptr);
   k_scatter_by_tag<<<nblk(s->n), kBlock, 0, s->stream>>>(s->vec_out, s->tag, s->n, s->io_buf);
   HIPCHK(s, hipMemcpyAsync(f3, s->io_buf, sizeof(double) * 3 * s->n, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(s, hipStreamSynchronize(s->stream));
@@ -3100,9 +3267,9 @@ int mdx_pairs_canonical(mdx_system *s, int32_t *out, int64_t cap, int64_t *count
   if (!s) return MDX_ERR_ARG;
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
   const int n = s->n;
-  k_canonical_tiles<false><<<s->ntiles, kTile, 0, s->stream>>>(
-      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, s->tag,
-      s->gsrc, s->gcode, nullptr, s->i_buf, nullptr);
+  k_canonical_tiles<false><<<s->ntiles, kBlk, 0, s->stream>>>(
+      s->tiles, s->ntiles, s->treal, s->sidx, n, s->kcap, s->list16, s->nn, s->tag, s->gsrc,
+      s->gcode, nullptr, s->i_buf, nullptr);
   int rc = scan(s, s->i_buf, s->goff, n, &s->dstat->ghost_total);
   if (rc) return rc;
   if ((rc = sync_status(s))) return rc;
@@ -3111,9 +3278,9 @@ int mdx_pairs_canonical(mdx_system *s, int32_t *out, int64_t cap, int64_t *count
   if (!out) return MDX_OK;
   int32_t *dout = nullptr;
   HIPCHK(s, hipMalloc((void **)&dout, sizeof(int32_t) * 3 * (size_t)std::max(total, 1LL)));
-  k_canonical_tiles<true><<<s->ntiles, kTile, 0, s->stream>>>(
-      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, s->tag,
-      s->gsrc, s->gcode, s->goff, nullptr, dout);
+  k_canonical_tiles<true><<<s->ntiles, kBlk, 0, s->stream>>>(
+      s->tiles, s->ntiles, s->treal, s->sidx, n, s->kcap, s->list16, s->nn, s->tag, s->gsrc,
+      s->gcode, s->goff, nullptr, dout);
   std::vector<int32_t> h(3 * (size_t)total);
   hipError_t e = hipMemcpyAsync(h.data(), dout, sizeof(int32_t) * 3 * (size_t)total,
                                 hipMemcpyDeviceToHost, s->stream);
@@ -3139,9 +3306,9 @@ int mdx_pairs_csr(mdx_system *s, int32_t *key_pointer, int32_t *number_of_partne
   if (cap < 0) return fail(s, MDX_ERR_ARG, "negative capacity");
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
   const int n = s->n, tn = n + s->g;
-  k_canonical_tiles<false, true><<<s->ntiles, kTile, 0, s->stream>>>(
-      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, s->tag,
-      s->gsrc, s->gcode, nullptr, s->i_buf, nullptr);
+  k_canonical_tiles<false, true><<<s->ntiles, kBlk, 0, s->stream>>>(
+      s->tiles, s->ntiles, s->treal, s->sidx, n, s->kcap, s->list16, s->nn, s->tag, s->gsrc,
+      s->gcode, nullptr, s->i_buf, nullptr);
   int rc = scan(s, s->i_buf, s->goff, n, &s->dstat->ghost_total);
   if (rc) return rc;
   if ((rc = sync_status(s))) return rc;
@@ -3165,9 +3332,9 @@ int mdx_pairs_csr(mdx_system *s, int32_t *key_pointer, int32_t *number_of_partne
   if (!sorted_list || cap == 0 || total == 0) return launch_check(s, "csr export");
   int32_t *dout = nullptr;
   HIPCHK(s, hipMalloc((void **)&dout, sizeof(int32_t) * (size_t)total));
-  k_canonical_tiles<true, true><<<s->ntiles, kTile, 0, s->stream>>>(
-      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, s->tag,
-      s->gsrc, s->gcode, s->goff, nullptr, dout);
+  k_canonical_tiles<true, true><<<s->ntiles, kBlk, 0, s->stream>>>(
+      s->tiles, s->ntiles, s->treal, s->sidx, n, s->kcap, s->list16, s->nn, s->tag, s->gsrc,
+      s->gcode, s->goff, nullptr, dout);
   const long long w = std::min<long long>(cap, total);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess)
@@ -3262,9 +3429,9 @@ int mdx_pairs_canonical_noalloc(mdx_system *s, int32_t *out, int64_t cap, int64_
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
   if (!s->scratch_dev) return fail(s, MDX_ERR_STATE, "no export staging on this device");
   const int n = s->n;
-  k_canonical_tiles<false><<<s->ntiles, kTile, 0, s->stream>>>(
-      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, s->tag,
-      s->gsrc, s->gcode, nullptr, s->i_buf, nullptr);
+  k_canonical_tiles<false><<<s->ntiles, kBlk, 0, s->stream>>>(
+      s->tiles, s->ntiles, s->treal, s->sidx, n, s->kcap, s->list16, s->nn, s->tag, s->gsrc,
+      s->gcode, nullptr, s->i_buf, nullptr);
   int rc = scan(s, s->i_buf, s->goff, n, &s->dstat->ghost_total);
   if (rc) return rc;
   if ((rc = sync_status(s))) return rc;
@@ -3276,9 +3443,9 @@ int mdx_pairs_canonical_noalloc(mdx_system *s, int32_t *out, int64_t cap, int64_
                 "export stages (mdx_pairs_canonical has no limit)",
                 total, kNoallocPairs);
   if (!out || cap == 0 || total == 0) return MDX_OK;
-  k_canonical_tiles<true><<<s->ntiles, kTile, 0, s->stream>>>(
-      s->tiles, s->ntiles, s->geom, cells_of(s), n, s->kcap, s->list16, s->nn, s->nb, s->tag,
-      s->gsrc, s->gcode, s->goff, nullptr, s->scratch_dev);
+  k_canonical_tiles<true><<<s->ntiles, kBlk, 0, s->stream>>>(
+      s->tiles, s->ntiles, s->treal, s->sidx, n, s->kcap, s->list16, s->nn, s->tag, s->gsrc,
+      s->gcode, s->goff, nullptr, s->scratch_dev);
   k_sort_runs<<<nblk(n), kBlock, 0, s->stream>>>(s->goff, s->i_buf, n, s->scratch_dev);
   if ((rc = launch_check(s, "canonical export"))) return rc;
   const long long w = std::min<long long>(cap, total);

@@ -328,16 +328,21 @@ __device__ __forceinline__ double list_r2(double xj, double yj, double zj, doubl
   return __fma_rn(dx, dx, __fma_rn(dy, dy, __dmul_rn(dz, dz)));
 }
 
-// one partner of the kick: a += df (q_j - q_i)   (fcalculator.cc:203-217)
+// one partner of the kick: a += df (q_j - q_i)   (fcalculator.cc:203-217).
+// r^2 and the accumulations are written as the fused operations they
+// compile to, so that no kernel is left a choice of which product to fuse:
+// dx dx + dy dy + dz dz contracts to fma(dz,dz, fma(dx,dx, dy dy)) in one
+// kernel and fma(dz,dz, fma(dy,dy, dx dx)) in another, which round
+// differently.  r^2 is the list build's form (list_r2).
 __device__ __forceinline__ void kick_term(double xj, double yj, double zj, double xi, double yi,
                                           double zi, const PairK &k, double CL2, double &ax,
                                           double &ay, double &az) {
   const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
-  const double s2 = dx * dx + dy * dy + dz * dz;
+  const double s2 = __fma_rn(dx, dx, __fma_rn(dy, dy, dz * dz));
   const double df = pair_df(s2, k, CL2);
-  ax += df * dx;
-  ay += df * dy;
-  az += df * dz;
+  ax = __fma_rn(df, dx, ax);
+  ay = __fma_rn(df, dy, ay);
+  az = __fma_rn(df, dz, az);
 }
 
 // q + p dt/2 (ForceCalculator::UpdatePositionHalf, fcalculator.cc:26-28)
