@@ -86,6 +86,18 @@ __device__ __forceinline__ void block_max_to(double v, double *__restrict__ part
   __syncthreads();
 }
 
+// Block maximum of an int, returned to every thread; every thread calls it.
+__device__ __forceinline__ int block_max_int(int v) {
+  __shared__ int lds[16];   // up to 1024 threads
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int m = lds[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = max(m, lds[w]);
+  __syncthreads();
+  return m;
+}
+
 __global__ void k_max_partials(const double *__restrict__ partial, int nb, double *__restrict__ out) {
   __shared__ double lds[kBlock];
   double m = 0.0;
@@ -600,6 +612,18 @@ constexpr BlkShape kBlkLadder[] = {{3, 503}, {3, 402}, {3, 321}, {3, 256}, {2, 2
                                    {1, 38},  {1, 32}};
 constexpr int kBlkLevels = (int)(sizeof kBlkLadder / sizeof kBlkLadder[0]);
 
+// The list columns of a tile hold its reals ranked by list length, 64 ranks
+// to a wavefront.  The wavefronts of a workgroup go round-robin to the CU's
+// four SIMDs, so rank groups 0..3 take wavefronts 0..3 and groups 4..7
+// wavefronts 7..4: each SIMD runs one long and one short wavefront of the
+// tile.  The map is its own inverse.
+__device__ __forceinline__ int col_of_rank(int r) {
+  const int g = r >> 6;
+  return (g < 4 ? g : 11 - g) * 64 + (r & 63);
+}
+// whether list column col of a tile of nreal reals holds a real
+__device__ __forceinline__ bool col_active(int col, int nreal) { return col_of_rank(col) < nreal; }
+
 // per tile, as every list walker reads it
 struct BlkTile {
   int nreal;   // reals, list columns 0 .. nreal-1
@@ -1004,6 +1028,52 @@ __device__ __forceinline__ bool row_window(const Geom &g, const double4 &qi, int
   return true;
 }
 
+// Rank of this thread's key among the keys of the workgroup's active
+// threads: the number of greater keys plus the number of equal keys held
+// by lower threads (keys below kRankBins).  Per wavefront, a ballot per
+// distinct key counts the lanes holding it and gives each lane its place
+// among them; the counts, summed over the wavefronts before and over the
+// greater keys, make the rank.  Every thread calls it.
+constexpr int kRankBins = 64;
+__device__ __forceinline__ int block_rank(bool act, int key, int (*cw)[kRankBins], int *base) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int e = threadIdx.x; e < nw * kRankBins; e += blockDim.x) cw[e / kRankBins][e % kRankBins] = 0;
+  __syncthreads();
+  int tie = 0;
+  unsigned long long todo = __ballot(act);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lkey = __shfl(key, leader);
+    const bool mine = act && key == lkey;
+    const unsigned long long same = __ballot(mine);
+    if (mine) tie = __popcll(same & ((1ull << lane) - 1ull));
+    if (lane == leader) cw[w][lkey] = __popcll(same);
+    todo &= ~same;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < kRankBins) {   // exclusive prefix over wavefronts; bin totals
+    const int b = threadIdx.x;
+    int s = 0;
+    for (int q = 0; q < nw; q++) {
+      const int c = cw[q][b];
+      cw[q][b] = s;
+      s += c;
+    }
+    base[b] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {   // keys above each bin
+    int s = 0;
+    for (int b = kRankBins - 1; b >= 0; b--) {
+      const int c = base[b];
+      base[b] = s;
+      s += c;
+    }
+  }
+  __syncthreads();
+  return act ? base[key] + cw[w][key] + tie : 0;
+}
+
 // Full neighbour list of every real of a tile: every partner j != i (real
 // or image) of its 27 cells with r^2 <= SL^2, r^2 formed as the reference's
 // AVX2 search, fma(dx,dx, fma(dy,dy, dz*dz)) with dq = q_j - q_i
@@ -1013,8 +1083,9 @@ __device__ __forceinline__ bool row_window(const Geom &g, const double4 &qi, int
 // planes 1 and 2 start (low and high 16 bits), nn[i] the padded length.
 // Each real's list goes first to column l (its place in the tile's real
 // rows) of the build buffer; then the columns are ranked by list length
-// (longest first, ties by column) and copied to that rank in the tile's
-// region of the list, and treal gets the real of each ranked column.
+// (longest first, ties by column) and copied to the rank's column
+// (col_of_rank) in the tile's region of the list, and treal gets the real
+// of each column.
 __global__ void __launch_bounds__(kBlk) k_build_tiles(
     const double4 *__restrict__ pos, const BlkTile *__restrict__ tiles,
     const BlkInfo *__restrict__ info, const int *__restrict__ sidx, int ntiles, Geom g, int kcap,
@@ -1023,6 +1094,7 @@ __global__ void __launch_bounds__(kBlk) k_build_tiles(
     double *__restrict__ partial, int *__restrict__ flags) {
   extern __shared__ double stage_mem[];
   __shared__ int keys[kBlk];
+  __shared__ int rank_cw[kBlk / 64][kRankBins], rank_base[kRankBins];
   __shared__ int rs[kSRows + 1];
   const int tile = tile_of_block(blockIdx.x, ntiles);
   if (tile >= ntiles) {
@@ -1034,9 +1106,30 @@ __global__ void __launch_bounds__(kBlk) k_build_tiles(
   const int lane = threadIdx.x;
   if (lane <= kSRows) rs[lane] = I->rs[lane];
   const bool act = lane < t.nreal;
+  // Lanes take the tile's reals grouped by their place in their cell's
+  // y-z cross-section (4 x 4 classes), which sets the x windows of the
+  // off-centre rows: a wavefront then walks windows of similar length.
+  // Which lane builds a real changes nothing in its list.
   int r = 0;
   while (r < kRealRows - 1 && lane >= I->sp[r + 1]) r++;
-  const int i = act ? I->sfirst[r] + (lane - I->sp[r]) : 0;
+  int key = 0;
+  if (act) {
+    const double4 q0 = pos[I->sfirst[r] + (lane - I->sp[r])];
+    const int y0r = I->y0 + r % kBlkRows - 1, z0r = I->z0 + r / kBlkRows - 1;
+    const int cy = (int)((q0.y * g.inv_w[1] - (double)y0r) * 4.0);
+    const int cz = (int)((q0.z * g.inv_w[2] - (double)z0r) * 4.0);
+    key = min(max(cy, 0), 3) * 4 + min(max(cz, 0), 3);
+  }
+  const int place = block_rank(act, key, rank_cw, rank_base);
+  if (act) keys[place] = lane;
+  __syncthreads();
+  int src = 0;
+  if (act) {
+    src = keys[lane];   // this lane's real, by its place in the tile's rows
+    r = 0;
+    while (r < kRealRows - 1 && src >= I->sp[r + 1]) r++;
+  }
+  const int i = act ? I->sfirst[r] + (src - I->sp[r]) : 0;
   const double4 qi = act ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
   const int self = act ? i + I->selfoff[r] : -1;
   const int ry = r % kBlkRows, rz = r / kBlkRows, y = I->y0 + ry, z = I->z0 + rz;
@@ -1067,19 +1160,41 @@ __global__ void __launch_bounds__(kBlk) k_build_tiles(
     nb[i] = min(b1, kcap) | (min(b2, kcap) << 16);
     if (cnt > kcap) atomicOr(flags, ST_LIST_OVERFLOW);
   }
-  keys[lane] = act ? m : -1;
+  // Rank by list length, longest first, ties by place in the tile's rows
+  // (slot order, so a wavefront of the kick loads and stores runs of
+  // consecutive reals): thread k ranks the real at place k, and the lane
+  // that built it moves its column to the rank's column.
+  if (act) keys[src] = m;
   __syncthreads();
-  if (act) {
-    int rank = 0;
-    for (int l = 0; l < kBlk; l++) {
+  const bool actk = lane < t.nreal;
+  const int mk = actk ? keys[lane] : 0;
+  int rank = 0;
+  if (kcap / kILP < kRankBins) {
+    rank = block_rank(actk, mk / kILP, rank_cw, rank_base);
+  } else if (actk) {
+    for (int l = 0; l < t.nreal; l++) {
       const int kl = keys[l];
-      rank += (kl > m || (kl == m && l < lane)) ? 1 : 0;
+      rank += (kl > mk || (kl == mk && l < lane)) ? 1 : 0;
     }
-    treal[(size_t)tile * kBlk + rank] = i;
-    // this thread's own column, written above, to its rank
-    const uint4 *src = reinterpret_cast<const uint4 *>(list_nat + region + (size_t)lane * kILP);
-    uint4 *dst = reinterpret_cast<uint4 *>(list + region + (size_t)rank * kILP);
-    for (int k = 0; k < m; k += kILP) dst[(size_t)(k / kILP) * kBlk] = src[(size_t)(k / kILP) * kBlk];
+  }
+  __syncthreads();   // every rank read its keys
+  if (actk) keys[lane] = col_of_rank(rank);
+  __syncthreads();
+  // Each column then takes its list from the column its real was built
+  // in: reads spread over the tile's rows (the block just wrote them, they
+  // sit in L2), stores in whole runs, no barrier between the groups.
+  __shared__ int csrc[kBlk], glen[kBlk];
+  // every column once: a real's rank's column, or an unused rank's
+  const int col = act ? keys[src] : col_of_rank(lane);
+  csrc[col] = lane;
+  glen[col] = act ? m / kILP : 0;
+  if (act) treal[(size_t)tile * kBlk + col] = i;
+  __syncthreads();
+  {
+    const uint4 *from = reinterpret_cast<const uint4 *>(list_nat + region) + csrc[lane];
+    uint4 *to = reinterpret_cast<uint4 *>(list + region) + lane;
+    const int ng = glen[lane];
+    for (int gr = 0; gr < ng; gr++) to[(size_t)gr * kBlk] = from[(size_t)gr * kBlk];
   }
   block_max_to((double)cnt, partial);
 }
@@ -1306,7 +1421,7 @@ __global__ void __launch_bounds__(kBlk) k_kick_tiles(
   }
   const BlkTile t = tiles[tile];
   const int lane = threadIdx.x;
-  const bool act = lane < t.nreal;
+  const bool act = col_active(lane, t.nreal);
   const int i = act ? treal[(size_t)tile * kBlk + lane] : 0;   // list column -> real
   // the lane's own loads go out before the staging, which hides them
   double4 qi = make_double4(0.0, 0.0, 0.0, 0.0), pm = qi;
@@ -1606,7 +1721,7 @@ __global__ void __launch_bounds__(kBlk) k_lj_observe_tiles(
   const BlkTile t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
   const int lane = threadIdx.x;
-  const bool act = lane < t.nreal;
+  const bool act = col_active(lane, t.nreal);
   const int i = act ? treal[(size_t)tile * kBlk + lane] : 0;
   const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
   const double4 qi = act ? pos[i] : z4;
@@ -1703,7 +1818,7 @@ __global__ void __launch_bounds__(kBlk) k_adj_hvp_tiles(
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
   const int *tsidx = sidx + t.s0;   // global slot of each staged particle
   const int lane = threadIdx.x;
-  const bool act = lane < t.nreal;
+  const bool act = col_active(lane, t.nreal);
   const int i = act ? treal[(size_t)tile * kBlk + lane] : 0;
   const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
   const double4 qi = act ? pos[i] : z4;
@@ -1777,7 +1892,7 @@ __global__ void __launch_bounds__(kBlk) k_observe_tiles(
   const BlkTile t = tiles[tile];
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
   const int lane = threadIdx.x;
-  const bool act = lane < t.nreal;
+  const bool act = col_active(lane, t.nreal);
   const int i = act ? treal[(size_t)tile * kBlk + lane] : 0;
   const double4 qi = act ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
   const unsigned short *lst = list + (size_t)tile * kcap * kBlk + (size_t)lane * kILP;
@@ -1891,7 +2006,7 @@ __global__ void __launch_bounds__(kBlk) k_canonical_tiles(
   if (tile >= ntiles) return;
   const BlkTile t = tiles[tile];
   const int lane = threadIdx.x;
-  if (lane >= t.nreal) return;
+  if (!col_active(lane, t.nreal)) return;
   const int i = treal[(size_t)tile * kBlk + lane];
   const int ti = tag[i];
   const int *tsidx = sidx + t.s0;
