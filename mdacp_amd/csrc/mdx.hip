@@ -1394,6 +1394,10 @@ __device__ __forceinline__ void block_sums_to(const double (&v)[K], double *__re
   __syncthreads();
 }
 
+// How far ahead of its arithmetic the kick reads a partner from LDS, and
+// how many list groups it loads ahead.
+constexpr int kKickAhead = 1, kKickPrefetch = 2;
+
 // The force kick over a tile: one lane per list column, the ranked reals of
 // the tile; the tile's partners are staged once, and each lane walks its
 // whole list (plane_groups over [0, nn)).
@@ -1441,7 +1445,8 @@ __global__ void __launch_bounds__(kBlk) k_kick_tiles(
   for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
   const double xi = qi.x, yi = qi.y, zi = qi.z;
   if (act) {
-    plane_groups<kBlk>(lst, 0, m, stage_mem, [&](int u, double xj, double yj, double zj) {
+    plane_groups<kBlk, kKickAhead, kKickPrefetch>(lst, 0, m, stage_mem,
+                                                  [&](int u, double xj, double yj, double zj) {
       if (SCALED) {
         const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
         const double df = pair_df_lj(dx * dx + dy * dy + dz * dz, lj, dt);

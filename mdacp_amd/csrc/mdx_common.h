@@ -211,19 +211,22 @@ struct GroupWriter {
 // Entries [kb, ke) of one lane's list (column base lst), both multiples of
 // kILP, S columns, partners staged at s3: entry k goes to term(k % kILP, xj,
 // yj, zj) in list order.  The list streams from memory, so the loads run
-// kPrefetch groups ahead in a ring of registers: group j computes from
-// ring slot j % kPrefetch, which then loads group j + kPrefetch (clamped
-// to the last group: a reload that is never used).  The ring turns once
-// per loop pass, so no register copies; the groups past the last full
-// turn compute from the ring as it stands.  Partner positions are read
-// from LDS one entry ahead: the reads of entry k + 1 (the next group's
-// first entry after a group's last, the clamped reload past the end) are
-// issued before the arithmetic of entry k, and scheduling barriers keep
-// them there, so their latency runs under that arithmetic.
+// PF groups ahead in a ring of registers: group j computes from ring slot
+// j % PF, which then loads group j + PF (clamped to the last group: a
+// reload that is never used).  The ring turns once per loop pass, so no
+// register copies; the groups past the last full turn compute from the
+// ring as it stands.  Partner positions are read from LDS AHEAD entries
+// ahead: the reads of entry k + AHEAD (in the next group after a group's
+// last entries, the clamped reload past the end) are issued before the
+// arithmetic of entry k, and scheduling barriers keep them there, so their
+// latency runs under that arithmetic.  The loads and reads change nothing
+// in the sums.
 constexpr int kPrefetch = 3;
-template <int S, class Term>
+template <int S, int AHEAD = 1, int PF = kPrefetch, class Term>
 __device__ __forceinline__ void plane_groups(const unsigned short *__restrict__ lst, int kb, int ke,
                                              const double *s3, Term term) {
+  static_assert(AHEAD == 1 || AHEAD == 2, "partners are read one or two entries ahead");
+  static_assert(PF >= 2, "the group after the one in hand must be in the ring");
   if (kb >= ke) return;
   // every k loaded is a group start (a multiple of kILP, as kb and ke are),
   // where list_index<S>(k) = k S: one unsigned 32-bit byte offset
@@ -235,41 +238,57 @@ __device__ __forceinline__ void plane_groups(const unsigned short *__restrict__ 
     const unsigned h = u < 2 ? w.x : (u < 4 ? w.y : (u < 6 ? w.z : w.w));
     return staged_at(s3, (h >> (16 * (u % 2))) & 0xffffu);
   };
-  double x, y, z;   // the partner of the entry in hand
+  double x, y, z;      // the partner of the entry in hand
+  double x1, y1, z1;   // and of the next one (AHEAD 2)
   auto run = [&](const uint4 &w, const uint4 &next) {
 #pragma unroll
     for (int u = 0; u < kILP; u++) {
-      const double *q = u + 1 < kILP ? entry(w, u + 1) : entry(next, 0);
+      const double *q = u + AHEAD < kILP ? entry(w, u + AHEAD) : entry(next, u + AHEAD - kILP);
       const double nx = q[0], ny = q[1], nz = q[2];
       __builtin_amdgcn_sched_barrier(0);
       term(u, x, y, z);
       __builtin_amdgcn_sched_barrier(0);
-      x = nx;
-      y = ny;
-      z = nz;
+      if (AHEAD == 1) {
+        x = nx;
+        y = ny;
+        z = nz;
+      } else {
+        x = x1;
+        y = y1;
+        z = z1;
+        x1 = nx;
+        y1 = ny;
+        z1 = nz;
+      }
     }
   };
   const int last = ke - kILP, ng = (ke - kb) / kILP;
-  uint4 ring[kPrefetch];
+  uint4 ring[PF];
 #pragma unroll
-  for (int d = 0; d < kPrefetch; d++) ring[d] = load(min(kb + d * kILP, last));
+  for (int d = 0; d < PF; d++) ring[d] = load(min(kb + d * kILP, last));
   {
     const double *q = entry(ring[0], 0);
     x = q[0];
     y = q[1];
     z = q[2];
-  }
-  int k = kb;
-  for (int j = 0; j < ng / kPrefetch; j++, k += kPrefetch * kILP) {
-#pragma unroll
-    for (int d = 0; d < kPrefetch; d++) {
-      run(ring[d], ring[(d + 1) % kPrefetch]);
-      ring[d] = load(min(k + (kPrefetch + d) * kILP, last));
+    if (AHEAD == 2) {
+      const double *q1 = entry(ring[0], 1);
+      x1 = q1[0];
+      y1 = q1[1];
+      z1 = q1[2];
     }
   }
-  const int rem = ng % kPrefetch;
+  int k = kb;
+  for (int j = 0; j < ng / PF; j++, k += PF * kILP) {
 #pragma unroll
-  for (int d = 0; d < kPrefetch - 1; d++)
+    for (int d = 0; d < PF; d++) {
+      run(ring[d], ring[(d + 1) % PF]);
+      ring[d] = load(min(k + (PF + d) * kILP, last));
+    }
+  }
+  const int rem = ng % PF;
+#pragma unroll
+  for (int d = 0; d < PF - 1; d++)
     if (d < rem) run(ring[d], ring[d + 1]);
 }
 
