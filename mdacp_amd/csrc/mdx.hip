@@ -36,6 +36,7 @@
 
 #include "../../include/mdx.h"
 #include "mdx_common.h"
+#include "mdx_tiles.h"
 
 namespace {
 
@@ -575,123 +576,8 @@ __global__ void k_gather_images(const int *__restrict__ order, int n, int g,
 }
 
 // ---------------------------------------------------------------- tiles
-// A tile is up to kBlk reals from a block of at most 3 x 3 interior cell
-// rows (y0 .. y0+ny-1, z0 .. z0+nz-1): a slice of each of the block's rows,
-// the block's reals split by rank into tiles of about equal size.  Every
-// partner of a tile's reals lies in the (ny+2) x (nz+2) cell rows around
-// the block, within the x window [xa - h, xb + h] of the tile's reals
-// (h a little over the search length).  Each cell row is sorted by x as a
-// whole (image cell 0, real cells 1..ms, image cell ms+1, or image cells
-// only), so its window is one run of that order, found by two binary
-// searches.  The runs are staged into LDS once per tile, row after row
-// (staged row gr = gy + 5 gz), as interleaved (x, y, z); sidx holds the
-// global slot of every staged particle.  A neighbour list entry is 24 x the
-// partner's staging index (its byte offset), so a tile stages at most
-// kMaxPlane particles and the sentinel after them.
-//
-// Against tiles of one cell row staged a z-plane at a time, a block tile
-// stages 4.5 instead of 11 particles per real at the same LDS per
-// wavefront, and each list is walked in one pass: the wavefront's lanes
-// (list columns, sorted by list length) run for the longest list of the
-// wavefront instead of the longest per plane.
-constexpr int kBlk = 512;                                 // reals (list columns) per tile
-constexpr int kBlkRows = 3;                               // most cell rows per block and axis
-constexpr int kSRow = kBlkRows + 2;                       // staged rows per axis, at most
-constexpr int kSRows = kSRow * kSRow;                     // staged rows per tile, at most
-constexpr int kRealRows = kBlkRows * kBlkRows;            // real rows per block, at most
-
-// Block shapes and reals aimed at per tile, tried in this order while some
-// tile's staging overflows: 3 x 3 rows of cells suit the usual densities
-// (about 2,300 staged particles for 500 reals at rho 0.5); dense boxes of
-// wide cells need smaller blocks and tiles.
-struct BlkShape {
-  int rows, target;
-};
-constexpr BlkShape kBlkLadder[] = {{3, 503}, {3, 402}, {3, 321}, {3, 256}, {2, 224}, {2, 179},
-                                   {2, 143}, {2, 114}, {1, 96},  {1, 76},  {1, 61},  {1, 48},
-                                   {1, 38},  {1, 32}};
-constexpr int kBlkLevels = (int)(sizeof kBlkLadder / sizeof kBlkLadder[0]);
-
-// The list columns of a tile hold its reals ranked by list length, 64 ranks
-// to a wavefront.  The wavefronts of a workgroup go round-robin to the CU's
-// four SIMDs, so rank groups 0..3 take wavefronts 0..3 and groups 4..7
-// wavefronts 7..4: each SIMD runs one long and one short wavefront of the
-// tile.  The map is its own inverse.
-__device__ __forceinline__ int col_of_rank(int r) {
-  const int g = r >> 6;
-  return (g < 4 ? g : 11 - g) * 64 + (r & 63);
-}
-// whether list column col of a tile of nreal reals holds a real
-__device__ __forceinline__ bool col_active(int col, int nreal) { return col_of_rank(col) < nreal; }
-
-// per tile, as every list walker reads it
-struct BlkTile {
-  int nreal;   // reals, list columns 0 .. nreal-1
-  int S;       // staged particles; the sentinel sits at S
-  int s0;      // first entry of the tile in sidx
-  int pad;
-};
-
-// per tile, for the list build and the staging index
-struct BlkInfo {
-  int y0, z0, ny, nz;
-  double xa, xb;              // x range of the tile's reals
-  int sp[kRealRows + 1];      // natural column of the first real of real row r = ry + 3 rz
-  int sfirst[kRealRows];      // slot of that real
-  int selfoff[kRealRows];     // staging index of real i of row r is i + selfoff[r]
-  int rs[kSRows + 1];         // staging index of the first particle of staged row gr
-  int rlo[kSRows];            // that particle's index in its row's x-sorted run
-};
-
-struct Cells {
-  const int *scan_in, *cnt_in, *scan_out, *cnt_out;
-};
-
-// The cell row (y, z) of the padded grid in x order: slot ranges a (image
-// cell x = 0, or every cell of an image row), b (real cells 1..ms), c
-// (image cell ms + 1).
-struct RowRun {
-  int a0, na, b0, nb, c0, nc;
-};
-__device__ __forceinline__ RowRun row_run(const Geom &g, const Cells &c, int n, int y, int z) {
-  const int cl0 = cell_linear(g, 0, y, z), cl1 = cell_linear(g, g.m[0] - 1, y, z);
-  RowRun r;
-  if (y >= 1 && y <= g.ms[1] && z >= 1 && z <= g.ms[2]) {
-    r.a0 = n + c.scan_out[cl0];
-    r.na = c.cnt_out[cl0];
-    r.b0 = c.scan_in[cl0 + 1];
-    r.nb = c.scan_in[cl1 - 1] + c.cnt_in[cl1 - 1] - r.b0;
-    r.c0 = n + c.scan_out[cl1];
-    r.nc = c.cnt_out[cl1];
-  } else {
-    r.a0 = n + c.scan_out[cl0];
-    r.na = c.scan_out[cl1] + c.cnt_out[cl1] - c.scan_out[cl0];
-    r.b0 = r.c0 = r.a0;
-    r.nb = r.nc = 0;
-  }
-  return r;
-}
-__device__ __forceinline__ int run_len(const RowRun &r) { return r.na + r.nb + r.nc; }
-__device__ __forceinline__ int run_slot(const RowRun &r, int t) {
-  return t < r.na ? r.a0 + t : (t < r.na + r.nb ? r.b0 + (t - r.na) : r.c0 + (t - r.na - r.nb));
-}
-// first index of the run whose x is >= v (ABOVE: > v)
-template <bool ABOVE>
-__device__ __forceinline__ int run_bound(const RowRun &r, const double4 *__restrict__ pos, double v) {
-  int lo = 0, hi = run_len(r);
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    const double x = pos[run_slot(r, mid)].x;
-    if (ABOVE ? x <= v : x < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-// blocks of br x br cell rows: by fastest, bz-major
-__host__ __device__ inline int blk_count(const Geom &g, int axis, int br) {
-  return (g.ms[axis] + br - 1) / br;
-}
+// Block tiles (mdx_tiles.h) over the whole box: the system's cells from
+// cell 0, images at slots n + scan_out.
 
 // XCD-aware tile order: the 8 XCDs each take one contiguous range of tiles
 // (workgroup b runs on XCD b % 8), so tiles of neighbouring blocks, which
@@ -702,118 +588,22 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles) {
 }
 __host__ __device__ inline int tile_grid(int ntiles) { return ((ntiles + 7) >> 3) << 3; }
 
-// reals of real row r of block (by, bz) of br x br rows (ry = r % 3, rz =
-// r / 3; none outside the block or the grid)
-__device__ __forceinline__ void blk_row(const Geom &g, const Cells &c, int by, int bz, int br,
-                                        int r, int *first, int *cnt) {
-  const int ry = r % kBlkRows, rz = r / kBlkRows;
-  const int y = br * by + ry + 1, z = br * bz + rz + 1;
-  *first = 0;
-  *cnt = 0;
-  if (ry >= br || rz >= br || y > g.ms[1] || z > g.ms[2]) return;
-  const int a = cell_linear(g, 1, y, z), b = cell_linear(g, g.ms[0], y, z);
-  *first = c.scan_in[a];
-  *cnt = c.scan_in[b] + c.cnt_in[b] - *first;
-}
-
-// Cumulative count of a block's reals over its x cells by one workgroup into
-// LDS: hist[c] = reals of the block in x cells 1..c, hist[0] = 0.
-constexpr int kMaxCellsX = 4096;
-__device__ void blk_hist(const Geom &g, const Cells &c, int by, int bz, int br, int *hist) {
-  const int ms = g.ms[0];
-  for (int cx = (int)threadIdx.x + 1; cx <= ms; cx += (int)blockDim.x) {
-    int s = 0;
-    for (int ry = 0; ry < br; ry++)
-      for (int rz = 0; rz < br; rz++) {
-        const int y = br * by + ry + 1, z = br * bz + rz + 1;
-        if (y <= g.ms[1] && z <= g.ms[2]) s += c.cnt_in[cell_linear(g, cx, y, z)];
-      }
-    hist[cx] = s;
-  }
-  if (threadIdx.x == 0) hist[0] = 0;
-  __syncthreads();
-  if (threadIdx.x == 0)
-    for (int cx = 1; cx <= ms; cx++) hist[cx] += hist[cx - 1];
-  __syncthreads();
-}
-
-// Where tile k of nt starts in real row r of the block: the first real of
-// the row (slots [first, first + cnt), sorted by x) at or past cut k, the x
-// at which the block's cumulative count, taken as linear within each x
-// cell, reaches k M / nt.  Cut 0 is the row's start, cut nt its end, so the
-// tiles of a block share their x cuts across its rows and cover every real
-// once.
-__device__ int blk_cut_slot(const Geom &g, const int *hist, const double4 *__restrict__ pos,
-                            int first, int cnt, int k, int nt) {
-  if (k <= 0) return first;
-  if (k >= nt) return first + cnt;
-  const int ms = g.ms[0];
-  const double cum = (double)k * (double)hist[ms] / (double)nt;
-  int lo = 1, hi = ms;   // first x cell whose cumulative count exceeds cum
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if ((double)hist[mid] > cum) hi = mid;
-    else lo = mid + 1;
-  }
-  const double f = (cum - (double)hist[lo - 1]) / (double)max(hist[lo] - hist[lo - 1], 1);
-  const double X = ((double)(lo - 1) + f) / g.inv_w[0];
-  int a = 0, b = cnt;
-  while (a < b) {
-    const int mid = (a + b) >> 1;
-    if (pos[first + mid].x < X) a = mid + 1;
-    else b = mid;
-  }
-  return first + a;
-}
-
-// Tiles of each block, one workgroup per block: nt = ceil(reals / target),
-// raised until no tile holds more than kBlk reals (the shared x cuts follow
-// the block's x histogram, so a tile's count is near M / nt); none for an
-// empty block.
+// tiles of each block of br x br rows, one workgroup per block
 __global__ void k_blk_count(const double4 *__restrict__ pos, Geom g, Cells c, int br, int target,
                             int *__restrict__ blk_nt) {
   __shared__ int hist[kMaxCellsX + 1];
-  __shared__ int rfirst[kRealRows], rcnt[kRealRows];
-  constexpr int kChunk = kBlock / kRealRows;   // tiles counted per round
-  __shared__ int len[kChunk][kRealRows];
-  __shared__ int over;
-  const int b = blockIdx.x, BY = blk_count(g, 1, br), by = b % BY, bz = b / BY;
-  const int tid = threadIdx.x;
-  if (tid < kRealRows) blk_row(g, c, by, bz, br, tid, &rfirst[tid], &rcnt[tid]);
+  const int b = blockIdx.x, BY = blk_count(g, 1, br);
+  const int by = b % BY, bz = b / BY;
   blk_hist(g, c, by, bz, br, hist);
-  const int M = hist[g.ms[0]];
-  int nt = (M + target - 1) / target;
-  for (int pass = 0; pass < 64 && nt > 0; pass++) {
-    if (tid == 0) over = 0;
-    __syncthreads();
-    for (int k0 = 0; k0 < nt; k0 += kChunk) {
-      const int kk = tid / kRealRows, r = tid % kRealRows, k = k0 + kk;
-      if (kk < kChunk && k < nt)
-        len[kk][r] = blk_cut_slot(g, hist, pos, rfirst[r], rcnt[r], k + 1, nt) -
-                     blk_cut_slot(g, hist, pos, rfirst[r], rcnt[r], k, nt);
-      __syncthreads();
-      if (tid < kChunk && k0 + tid < nt) {
-        int s = 0;
-        for (int q = 0; q < kRealRows; q++) s += len[tid][q];
-        if (s > kBlk) atomicOr(&over, 1);
-      }
-      __syncthreads();
-    }
-    if (!over) break;
-    nt += max(1, nt / 8);
-    __syncthreads();
-  }
-  if (tid == 0) blk_nt[b] = nt;
+  const int nt = blk_ntiles(g, c, pos, hist, by, bz, br, target);
+  if (threadIdx.x == 0) blk_nt[b] = nt;
 }
 
-// Tiles of one block per workgroup, one wavefront per tile: real row r
-// (lanes 0..8) gives the tile the slots between its cuts.  Lanes 0..24 then
-// cut the staged rows to the window [xa - h, xb + h] of the tile's reals, h
-// = SL (1 + 1e-10) + eps, wider than any |dx| of a pair with r^2 <= SL^2 as
-// rounded, eps covering the rounding of positions near the box (the
-// images' shift).  tile_S gets each tile's staging size; a staging above
-// cap raises ST_STAGE_OVERFLOW.
-__global__ void k_blk_tiles(const double4 *__restrict__ pos, Geom g, Cells c, int n, int br,
+// The tiles of one block per workgroup, one wavefront per tile; tile_S gets
+// each tile's staging size.  A staging above cap raises ST_STAGE_OVERFLOW,
+// a tile above kBlk reals ST_TILE_OVERFLOW; block maxima of the stagings
+// go to partial.
+__global__ void k_blk_tiles(const double4 *__restrict__ pos, Geom g, Cells c, int br,
                             const int *__restrict__ blk_nt, const int *__restrict__ blk_toff,
                             int cap, BlkTile *__restrict__ tiles, BlkInfo *__restrict__ info,
                             int *__restrict__ tile_S, double *__restrict__ partial,
@@ -822,73 +612,16 @@ __global__ void k_blk_tiles(const double4 *__restrict__ pos, Geom g, Cells c, in
   const int b = blockIdx.x, BY = blk_count(g, 1, br);
   const int by = b % BY, bz = b / BY;
   const int nt = blk_nt[b], lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int y0 = br * by + 1, z0 = br * bz + 1;
-  const int ny = min(br, g.ms[1] - y0 + 1), nz = min(br, g.ms[2] - z0 + 1);
-  const double h = sqrt(g.SL2) * (1.0 + 1e-10) + 1e-9 * (g.L[0] + g.L[1] + g.L[2]);
   blk_hist(g, c, by, bz, br, hist);
   double smax = 0.0;
   for (int k = w; k < nt; k += (int)(blockDim.x >> 6)) {
     const int t = blk_toff[b] + k;
-    // slices of the real rows (lanes 0..8)
-    int lo = 0, len = 0;
-    if (lane < kRealRows) {
-      int rf, rm;
-      blk_row(g, c, by, bz, br, lane, &rf, &rm);
-      lo = blk_cut_slot(g, hist, pos, rf, rm, k, nt);
-      len = blk_cut_slot(g, hist, pos, rf, rm, k + 1, nt) - lo;
-    }
-    double xa = kFar, xb = -kFar;
-    if (len > 0) {
-      xa = pos[lo].x;
-      xb = pos[lo + len - 1].x;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      xa = fmin(xa, __shfl_xor(xa, off));
-      xb = fmax(xb, __shfl_xor(xb, off));
-    }
-    // exclusive prefix of the slice lengths over lanes 0..8
-    int sp = 0;
-    for (int r = 0; r < kRealRows; r++) {
-      const int lr = __shfl(len, r);
-      if (r < lane) sp += lr;
-    }
-    const int nreal = __shfl(sp, kRealRows);   // lane 9 holds the total
-    // the window of staged row gr = lane (0..24)
-    int wlo = 0, wlen = 0;
-    RowRun run{};
-    const int gy = lane % kSRow, gz = lane / kSRow;
-    if (lane < kSRows && gy < ny + 2 && gz < nz + 2 && nreal > 0) {
-      run = row_run(g, c, n, y0 - 1 + gy, z0 - 1 + gz);
-      wlo = run_bound<false>(run, pos, xa - h);
-      wlen = run_bound<true>(run, pos, xb + h) - wlo;
-    }
-    int rs = 0;
-    for (int r = 0; r < kSRows; r++) {
-      const int lr = __shfl(wlen, r);
-      if (r < lane) rs += lr;
-    }
-    const int S = __shfl(rs, kSRows);   // lane 25 holds the total
-    BlkInfo *I = info + t;
+    int nreal;
+    const int S = blk_make_tile(g, c, pos, hist, by, bz, br, k, nt, tiles + t, info + t, &nreal);
     if (lane == 0) {
-      tiles[t] = BlkTile{nreal, S, 0, 0};
       tile_S[t] = S;
-      I->y0 = y0;
-      I->z0 = z0;
-      I->ny = ny;
-      I->nz = nz;
-      I->xa = xa;
-      I->xb = xb;
       if (S > cap) atomicOr(flags, ST_STAGE_OVERFLOW);
       if (nreal > kBlk) atomicOr(flags, ST_TILE_OVERFLOW);
-    }
-    if (lane <= kRealRows) I->sp[lane] = sp;
-    if (lane < kRealRows) I->sfirst[lane] = lo;
-    if (lane <= kSRows) I->rs[lane] = rs;
-    if (lane < kSRows) I->rlo[lane] = wlo;
-    // the real rows' self offsets: real row r sits in staged row (ry + 1) + 5 (rz + 1)
-    if (lane < kSRows && gy >= 1 && gy <= ny && gz >= 1 && gz <= nz) {
-      const int r = (gy - 1) + kBlkRows * (gz - 1);
-      I->selfoff[r] = rs + run.na - run.b0 - wlo;
     }
     smax = fmax(smax, (double)S);
   }
@@ -896,196 +629,18 @@ __global__ void k_blk_tiles(const double4 *__restrict__ pos, Geom g, Cells c, in
 }
 
 // sidx of every tile (one workgroup per tile) and the tile's first entry in it
-__global__ void k_blk_sidx(Geom g, Cells c, int n, int ntiles, const int *__restrict__ s0,
+__global__ void k_blk_sidx(Geom g, Cells c, int ntiles, const int *__restrict__ s0,
                            BlkTile *__restrict__ tiles, const BlkInfo *__restrict__ info,
                            int *__restrict__ sidx) {
-  __shared__ RowRun runs[kSRows];
-  __shared__ int rs[kSRows + 1], rlo[kSRows];
   const int t = blockIdx.x;
   if (t >= ntiles) return;
-  const BlkInfo *I = info + t;
-  const int tid = threadIdx.x;
-  if (tid < kSRows) {
-    const int gy = tid % kSRow, gz = tid / kSRow;
-    RowRun r{};
-    if (gy < I->ny + 2 && gz < I->nz + 2) r = row_run(g, c, n, I->y0 - 1 + gy, I->z0 - 1 + gz);
-    runs[tid] = r;
-    rlo[tid] = I->rlo[tid];
-  }
-  if (tid <= kSRows) rs[tid] = I->rs[tid];
-  __syncthreads();
-  const int S = rs[kSRows], base = s0[t];
-  if (tid == 0) tiles[t].s0 = base;
-  for (int u = tid; u < S; u += blockDim.x) {
-    int gr = 0;
-    while (u >= rs[gr + 1]) gr++;
-    sidx[base + u] = run_slot(runs[gr], rlo[gr] + (u - rs[gr]));
-  }
+  if (threadIdx.x == 0) tiles[t].s0 = s0[t];
+  blk_write_sidx(g, c, info + t, sidx + s0[t]);
 }
 
-// Stage a tile: its S particles, load(sidx[u]) for staged index u, into s3
-// as interleaved (x, y, z), and the sentinel after them.  kB loads per
-// thread go out together; kB NT covers the largest staging.  Every thread
-// calls it; it ends with a barrier.
-template <int NT, class Load>
-__device__ __forceinline__ void stage_tile(int S, const int *__restrict__ sidx, double *s3,
-                                           Load load) {
-  constexpr int kB = (kMaxPlane + NT) / NT;
-  for (int u0 = 0; u0 < S; u0 += kB * NT) {
-    int sl[kB];
-#pragma unroll
-    for (int j = 0; j < kB; j++) {
-      const int u = u0 + j * NT + (int)threadIdx.x;
-      sl[j] = sidx[u < S ? u : 0];
-    }
-    double4 q[kB];
-#pragma unroll
-    for (int j = 0; j < kB; j++) q[j] = load(sl[j]);
-#pragma unroll
-    for (int j = 0; j < kB; j++) {
-      const int u = u0 + j * NT + (int)threadIdx.x;
-      if (u < S) {
-        s3[3 * u] = q[j].x;
-        s3[3 * u + 1] = q[j].y;
-        s3[3 * u + 2] = q[j].z;
-      }
-    }
-  }
-  if (threadIdx.x == 0) s3[3 * S] = s3[3 * S + 1] = s3[3 * S + 2] = kFar;
-  __syncthreads();
-}
-
-// Partners of one list column (entries [0, m), column base lst, kBlk
-// columns) in list order: f(o) with o = 3 x the partner's staging index;
-// sentinel entries (>= lim = 24 S) are skipped.  One 16-byte load per group.
-template <class F>
-__device__ __forceinline__ void for_each_partner(const unsigned short *__restrict__ lst, int m,
-                                                 unsigned lim, F f) {
-  for (int k = 0; k < m; k += kILP) {
-    const uint4 w = *reinterpret_cast<const uint4 *>(lst + list_index<kBlk>(k));
-    const unsigned h[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-    for (int u = 0; u < kILP; u++) {
-      const unsigned e = (h[u / 2] >> (16 * (u % 2))) & 0xffffu;
-      if (e < lim) f((int)(e >> 3));
-    }
-  }
-}
-
-// Partners of a real at qi among staged candidates [lo, hi) (candidate o
-// at s3 + 3 o): r^2 <= SL2 (list_r2) and not the real itself, put in
-// candidate order.  Four candidates are tested at a time so that their LDS
-// reads are in flight together.
-template <class Writer>
-__device__ __forceinline__ void range_partners(const double *s3, const double4 &qi, double SL2,
-                                               int lo, int hi, int self, Writer &gw) {
-  constexpr int kU = 4;
-  for (int o = lo; o < hi; o += kU) {
-    bool ok[kU];
-#pragma unroll
-    for (int u = 0; u < kU; u++) {
-      const double *q = s3 + 3 * min(o + u, hi - 1);
-      ok[u] = o + u < hi && list_r2(q[0], q[1], q[2], qi.x, qi.y, qi.z) <= SL2 && o + u != self;
-    }
-#pragma unroll
-    for (int u = 0; u < kU; u++)
-      if (ok[u]) gw.put(plane_entry(o + u));
-  }
-}
-
-// First staged candidate in [lo, hi) (candidate o at s3 + 3 o) whose x is
-// >= v (ABOVE: > v); the range is sorted by x.
-template <bool ABOVE>
-__device__ __forceinline__ int x_bound(const double *s3, int lo, int hi, double v) {
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    const double x = s3[3 * mid];
-    if (ABOVE ? x <= v : x < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-// The x window of the cell row (dy, dz) around a real at qi of cell row
-// (y, z): every candidate of that row with list_r2 <= SL2 has |x - qi.x| <=
-// h.  gy, gz are lower bounds of its distance to the row in y and z (the
-// cell's faces (c - 1) w, c w, less eps for the rounding of the faces and of
-// the binning), and the slack 1e-10 SL2 covers the roundings of r^2, so h
-// is only ever too wide.  Returns false when no candidate of the row can be
-// in range.
-__device__ __forceinline__ bool row_window(const Geom &g, const double4 &qi, int y, int z,
-                                           int dy, int dz, double eps, double *h) {
-  double gy = 0.0, gz = 0.0;
-  if (dy < 0) gy = qi.y - (double)(y - 1) / g.inv_w[1];
-  if (dy > 0) gy = (double)y / g.inv_w[1] - qi.y;
-  if (dz < 0) gz = qi.z - (double)(z - 1) / g.inv_w[2];
-  if (dz > 0) gz = (double)z / g.inv_w[2] - qi.z;
-  gy = fmax(gy - eps, 0.0);
-  gz = fmax(gz - eps, 0.0);
-  const double h2 = g.SL2 * (1.0 + 1e-10) - gy * gy - gz * gz;
-  if (h2 < 0.0) return false;
-  *h = sqrt(h2) + eps;
-  return true;
-}
-
-// Rank of this thread's key among the keys of the workgroup's active
-// threads: the number of greater keys plus the number of equal keys held
-// by lower threads (keys below kRankBins).  Per wavefront, a ballot per
-// distinct key counts the lanes holding it and gives each lane its place
-// among them; the counts, summed over the wavefronts before and over the
-// greater keys, make the rank.  Every thread calls it.
-constexpr int kRankBins = 64;
-__device__ __forceinline__ int block_rank(bool act, int key, int (*cw)[kRankBins], int *base) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int e = threadIdx.x; e < nw * kRankBins; e += blockDim.x) cw[e / kRankBins][e % kRankBins] = 0;
-  __syncthreads();
-  int tie = 0;
-  unsigned long long todo = __ballot(act);
-  while (todo) {
-    const int leader = __ffsll((long long)todo) - 1;
-    const int lkey = __shfl(key, leader);
-    const bool mine = act && key == lkey;
-    const unsigned long long same = __ballot(mine);
-    if (mine) tie = __popcll(same & ((1ull << lane) - 1ull));
-    if (lane == leader) cw[w][lkey] = __popcll(same);
-    todo &= ~same;
-  }
-  __syncthreads();
-  if ((int)threadIdx.x < kRankBins) {   // exclusive prefix over wavefronts; bin totals
-    const int b = threadIdx.x;
-    int s = 0;
-    for (int q = 0; q < nw; q++) {
-      const int c = cw[q][b];
-      cw[q][b] = s;
-      s += c;
-    }
-    base[b] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {   // keys above each bin
-    int s = 0;
-    for (int b = kRankBins - 1; b >= 0; b--) {
-      const int c = base[b];
-      base[b] = s;
-      s += c;
-    }
-  }
-  __syncthreads();
-  return act ? base[key] + cw[w][key] + tie : 0;
-}
-
-// Full neighbour list of every real of a tile: every partner j != i (real
-// or image) of its 27 cells with r^2 <= SL^2, r^2 formed as the reference's
-// AVX2 search, fma(dx,dx, fma(dy,dy, dz*dz)) with dq = q_j - q_i
-// (meshlist.cc:354-362).  The partners go plane by plane (dz = -1, 0, +1),
-// row by row (dy = -1, 0, +1), in x order within a row; each plane's entries
-// are padded with the sentinel to a multiple of kILP.  nb[i] holds where
-// planes 1 and 2 start (low and high 16 bits), nn[i] the padded length.
-// Each real's list goes first to column l (its place in the tile's real
-// rows) of the build buffer; then the columns are ranked by list length
-// (longest first, ties by column) and copied to the rank's column
-// (col_of_rank) in the tile's region of the list, and treal gets the real
-// of each column.
+// The neighbour lists of every tile (build_tile_lists), one workgroup per
+// tile; block maxima of the list lengths go to partial, a list longer than
+// kcap raises ST_LIST_OVERFLOW.
 __global__ void __launch_bounds__(kBlk) k_build_tiles(
     const double4 *__restrict__ pos, const BlkTile *__restrict__ tiles,
     const BlkInfo *__restrict__ info, const int *__restrict__ sidx, int ntiles, Geom g, int kcap,
@@ -1093,109 +648,16 @@ __global__ void __launch_bounds__(kBlk) k_build_tiles(
     int *__restrict__ nn, int *__restrict__ nb, int *__restrict__ treal,
     double *__restrict__ partial, int *__restrict__ flags) {
   extern __shared__ double stage_mem[];
-  __shared__ int keys[kBlk];
-  __shared__ int rank_cw[kBlk / 64][kRankBins], rank_base[kRankBins];
-  __shared__ int rs[kSRows + 1];
   const int tile = tile_of_block(blockIdx.x, ntiles);
   if (tile >= ntiles) {
     block_max_to(0.0, partial);
     return;
   }
   const BlkTile t = tiles[tile];
-  const BlkInfo *I = info + tile;
-  const int lane = threadIdx.x;
-  if (lane <= kSRows) rs[lane] = I->rs[lane];
-  const bool act = lane < t.nreal;
-  // Lanes take the tile's reals grouped by their place in their cell's
-  // y-z cross-section (4 x 4 classes), which sets the x windows of the
-  // off-centre rows: a wavefront then walks windows of similar length.
-  // Which lane builds a real changes nothing in its list.
-  int r = 0;
-  while (r < kRealRows - 1 && lane >= I->sp[r + 1]) r++;
-  int key = 0;
-  if (act) {
-    const double4 q0 = pos[I->sfirst[r] + (lane - I->sp[r])];
-    const int y0r = I->y0 + r % kBlkRows - 1, z0r = I->z0 + r / kBlkRows - 1;
-    const int cy = (int)((q0.y * g.inv_w[1] - (double)y0r) * 4.0);
-    const int cz = (int)((q0.z * g.inv_w[2] - (double)z0r) * 4.0);
-    key = min(max(cy, 0), 3) * 4 + min(max(cz, 0), 3);
-  }
-  const int place = block_rank(act, key, rank_cw, rank_base);
-  if (act) keys[place] = lane;
-  __syncthreads();
-  int src = 0;
-  if (act) {
-    src = keys[lane];   // this lane's real, by its place in the tile's rows
-    r = 0;
-    while (r < kRealRows - 1 && src >= I->sp[r + 1]) r++;
-  }
-  const int i = act ? I->sfirst[r] + (src - I->sp[r]) : 0;
-  const double4 qi = act ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
-  const int self = act ? i + I->selfoff[r] : -1;
-  const int ry = r % kBlkRows, rz = r / kBlkRows, y = I->y0 + ry, z = I->z0 + rz;
-  stage_tile<kBlk>(t.S, sidx + t.s0, stage_mem, [&](int slot) { return pos[slot]; });
-  // rounding allowance of the cell faces and the binning, far above it
-  const double eps = 1e-9 * (g.L[0] + g.L[1] + g.L[2]);
   const size_t region = (size_t)tile * kcap * kBlk;
-  GroupWriter<kBlk> gw(list_nat + region + (size_t)lane * kILP, kcap);
-  int b1 = 0, b2 = 0;
-  if (act) {
-    for (int p = 0; p < kPlanes; p++) {
-      for (int dy = -1; dy <= 1; dy++) {
-        const int gr = (ry + 1 + dy) + kSRow * (rz + p);
-        double h;
-        if (!row_window(g, qi, y, z, dy, p - 1, eps, &h)) continue;
-        const int lo = x_bound<false>(stage_mem, rs[gr], rs[gr + 1], qi.x - h);
-        range_partners(stage_mem, qi, g.SL2, lo, x_bound<true>(stage_mem, lo, rs[gr + 1], qi.x + h),
-                       self, gw);
-      }
-      gw.pad_plane(t.S);
-      if (p == 0) b1 = gw.cnt;
-      if (p == 1) b2 = gw.cnt;
-    }
-  }
-  const int cnt = gw.cnt, m = min(cnt, kcap);
-  if (act) {
-    nn[i] = m;
-    nb[i] = min(b1, kcap) | (min(b2, kcap) << 16);
-    if (cnt > kcap) atomicOr(flags, ST_LIST_OVERFLOW);
-  }
-  // Rank by list length, longest first, ties by place in the tile's rows
-  // (slot order, so a wavefront of the kick loads and stores runs of
-  // consecutive reals): thread k ranks the real at place k, and the lane
-  // that built it moves its column to the rank's column.
-  if (act) keys[src] = m;
-  __syncthreads();
-  const bool actk = lane < t.nreal;
-  const int mk = actk ? keys[lane] : 0;
-  int rank = 0;
-  if (kcap / kILP < kRankBins) {
-    rank = block_rank(actk, mk / kILP, rank_cw, rank_base);
-  } else if (actk) {
-    for (int l = 0; l < t.nreal; l++) {
-      const int kl = keys[l];
-      rank += (kl > mk || (kl == mk && l < lane)) ? 1 : 0;
-    }
-  }
-  __syncthreads();   // every rank read its keys
-  if (actk) keys[lane] = col_of_rank(rank);
-  __syncthreads();
-  // Each column then takes its list from the column its real was built
-  // in: reads spread over the tile's rows (the block just wrote them, they
-  // sit in L2), stores in whole runs, no barrier between the groups.
-  __shared__ int csrc[kBlk], glen[kBlk];
-  // every column once: a real's rank's column, or an unused rank's
-  const int col = act ? keys[src] : col_of_rank(lane);
-  csrc[col] = lane;
-  glen[col] = act ? m / kILP : 0;
-  if (act) treal[(size_t)tile * kBlk + col] = i;
-  __syncthreads();
-  {
-    const uint4 *from = reinterpret_cast<const uint4 *>(list_nat + region) + csrc[lane];
-    uint4 *to = reinterpret_cast<uint4 *>(list + region) + lane;
-    const int ng = glen[lane];
-    for (int gr = 0; gr < ng; gr++) to[(size_t)gr * kBlk] = from[(size_t)gr * kBlk];
-  }
+  const int cnt = build_tile_lists(pos, t, info + tile, sidx + t.s0, g, kcap, list_nat + region,
+                                   list + region, nn, nb, treal + (size_t)tile * kBlk, stage_mem);
+  if (cnt > kcap) atomicOr(flags, ST_LIST_OVERFLOW);
   block_max_to((double)cnt, partial);
 }
 
@@ -2324,7 +1786,7 @@ int stage_cap(const mdx_system *s) {
 }
 
 Cells cells_of(const mdx_system *s) {
-  return Cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out};
+  return Cells{s->scan_in, s->cnt_in, s->scan_out, s->cnt_out, 0, s->n};
 }
 
 // per-block partial buffers for every grid the library launches
@@ -2480,7 +1942,7 @@ int build_pairlist(mdx_system *s) {
     }
     if ((rc = ensure_partials(s, std::max(tile_grid(ntiles), nblocks)))) return rc;
     HIPCHK(s, hipMemsetAsync(&s->dstat->flags, 0, sizeof(int), s->stream));
-    k_blk_tiles<<<nblocks, kBlock, 0, s->stream>>>(s->pos, s->geom, cells, n, br, s->blk_nt,
+    k_blk_tiles<<<nblocks, kBlock, 0, s->stream>>>(s->pos, s->geom, cells, br, s->blk_nt,
                                                      s->blk_toff, cap, s->tiles, s->tinfo,
                                                      s->tile_S, s->pmax, &s->dstat->flags);
     k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, nblocks, &s->dstat->stage_max);
@@ -2507,7 +1969,7 @@ int build_pairlist(mdx_system *s) {
     if ((rc = dev_alloc(s, &s->sidx, sc))) return rc;
     s->sidx_cap = sc;
   }
-  k_blk_sidx<<<s->ntiles, kBlock, 0, s->stream>>>(s->geom, cells, n, s->ntiles, s->tile_s0,
+  k_blk_sidx<<<s->ntiles, kBlock, 0, s->stream>>>(s->geom, cells, s->ntiles, s->tile_s0,
                                                    s->tiles, s->tinfo, s->sidx);
   if ((rc = ensure_list_capacity(s, s->kcap))) return rc;
   // 6. neighbour lists, one pass at the current width, again if it overflows
