@@ -276,7 +276,7 @@ int mdx_lj_trajectory(mdx_system *sys, double eps, double sigma, int nsteps, int
 typedef struct mdx_ensemble mdx_ensemble;
 
 typedef struct mdx_ensemble_config {
-  int tile;            /* reals per tile: 64 or 128 */
+  int tile;            /* reals aimed at per tile at first: 512 or 256 (a tile holds at most 512) */
   int grid_mult;       /* persistent step grid = grid_mult x resident workgroups */
   int step_grid;       /* workgroups of the step kernel */
   int stage_cap;       /* particles one tile may stage in LDS */
@@ -305,11 +305,12 @@ int mdx_ensemble_status(mdx_ensemble *e, int *status, int *steps);
  * mdx_observe of each replica alone */
 int mdx_ensemble_observe(mdx_ensemble *e, double *ke, double *pe, double *vir);
 int mdx_ensemble_get_config(const mdx_ensemble *e, mdx_ensemble_config *cfg);
-/* tile size changes only before the first step after mdx_ensemble_set_state */
+/* tile (512 or 256) changes only before the first step after mdx_ensemble_set_state */
 int mdx_ensemble_set_config(mdx_ensemble *e, int tile, int grid_mult);
-/* times trial_steps steps of every tile size and grid multiple from the
- * loaded state, keeps the fastest; only before the first step; the loaded
- * state is left in place, and results do not depend on the choice */
+/* times trial_steps steps of every tile target and grid multiple from the
+ * loaded state (after one untimed trial per target), keeps the fastest;
+ * only before the first step; the loaded state is left in place, and
+ * results do not depend on the choice */
 int mdx_ensemble_autotune(mdx_ensemble *e, int trial_steps, mdx_ensemble_config *best);
 
 /* cumulative device time (ms) of the pairlist build, force kick and

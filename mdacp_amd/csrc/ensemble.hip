@@ -41,6 +41,7 @@
 
 #include "../../include/mdx.h"
 #include "mdx_common.h"
+#include "mdx_tiles.h"
 
 namespace {
 
@@ -52,16 +53,21 @@ constexpr int kMaxCand = 1024;         // CheckByDisplacement candidates held in
 constexpr int kMaxImages = 7;          // images per real
 constexpr int kScanPer = 8;            // items per thread in the per-replica scans
 
-enum EnsFlags : int { EF_INVALID_POS = 1, EF_STAGE = 2, EF_LIST = 4, EF_TOO_FAST = 8, EF_IMAGES = 16 };
+enum EnsFlags : int {
+  EF_INVALID_POS = 1, EF_STAGE = 2, EF_LIST = 4, EF_TOO_FAST = 8, EF_IMAGES = 16, EF_TILES = 32
+};
 
 struct Rep {
   Geom g;
   size_t lbase;      // first list entry of its tiles
+  size_t sbase;      // first staging index entry of its tiles
   int n, rbase;      // reals: slots [rbase, rbase + n)
   int ibase, gcap;   // images: slots NR + [ibase, ibase + gcap)
   int cbase, ncell;  // padded cell grid
   int tbase, tcap;   // tile slots
-  int kcap, pad;     // list entries per real
+  int kcap;          // list entries per real
+  int level;         // block shape and reals per tile (kBlkLadder)
+  int bbase, bcap;   // per-block tile counts: [bbase, bbase + bcap)
 };
 
 // Device-resident state machine of one replica (PairList, pairlist.h:10-31,
@@ -99,8 +105,7 @@ struct EnsDev {
   int *checklist; // [2R] replicas that run a displacement check after this step: those not
                   // rebuilt in it first, the rebuilt ones from R on
   int R, NR, NI, NT;
-  int T;          // tile size (reals per tile)
-  int stage_cap;  // particles one plane of a tile may stage (dynamic LDS / 24 B)
+  int stage_cap;  // particles one tile may stage (dynamic LDS / 24 B)
   int check_list_length;
   double dt, buffer, SL;
   PairK pk;       // pair-term constants with dt folded in
@@ -113,8 +118,14 @@ struct EnsDev {
   int *tag_tmp, *gsrc_tmp, *gcode_tmp;
   int *cellid, *slot, *nimg, *goff, *order, *inv;
   int *cell_count, *cnt_in, *cnt_out, *scan_in, *scan_out, *cell_start;
-  TileDesc *tiles;
-  unsigned short *list;
+  BlkTile *btiles;      // [NT] tiles, each replica's at [tbase, tbase + tcap); pad = replica
+  BlkInfo *binfo;       // [NT]
+  int *treal;           // [NT][kBlk] real of each list column
+  int *tile_S;          // [NT] staged particles of each tile
+  int *sidx;            // staging index, each replica's from sbase on (tile s0 relative to it)
+  int *blk_nt, *blk_toff;   // per block of cell rows: tiles, first tile
+  unsigned short *list;     // lists in ranked columns
+  unsigned short *list_nat; // the build's columns
   int *nn;
   int *nb;              // [NR] where planes 1 and 2 start in each list (16 + 16 bits)
   double *partial;      // [NT] max |p|^2 per tile
@@ -134,7 +145,7 @@ struct EnsDev {
 // displacement check runs in this step (reals and images).
 enum OffKind {
   OFF_REALS = 0, OFF_IMAGES = 1, OFF_CELLS = 2, OFF_TILES = 3, OFF_CHECK = 4, OFF_CHECK2 = 5,
-  kOffKinds = 6
+  OFF_BLOCKS = 6, kOffKinds = 7
 };
 
 __device__ __forceinline__ const int *offsets(const EnsDev &d, int kind) {
@@ -246,61 +257,26 @@ __device__ int block_max_int(int v) {
 }
 
 // ------------------------------------------------------------ tiles
-// [start, len) of segment s (0..17) of a tile of replica rp, as absolute
-// slots: even s reals (interior cells), odd s images (outer cells).
-__device__ __forceinline__ void e_segment(const EnsDev &d, const Rep &rp, int y, int z, int xf, int xl,
-                                          int s, int *start, int *len) {
-  const int r = s >> 1, dy = r % 3 - 1, dz = r / 3 - 1;
-  const int a = rp.cbase + cell_linear(rp.g, xf - 1, y + dy, z + dz);
-  const int b = rp.cbase + cell_linear(rp.g, xl + 1, y + dy, z + dz);
-  if ((s & 1) == 0) {
-    *start = d.scan_in[a];
-    *len = d.scan_in[b] + d.cnt_in[b] - *start;
-  } else {
-    *start = d.scan_out[a];
-    *len = d.scan_out[b] + d.cnt_out[b] - *start;
-  }
+// Block tiles (mdx_tiles.h) of one replica: its cells from rp.cbase, reals
+// and images at the absolute slots the scans hold.
+__device__ __forceinline__ Cells rep_cells(const EnsDev &d, const Rep &rp) {
+  return Cells{d.scan_in, d.cnt_in, d.scan_out, d.cnt_out, rp.cbase, 0};
+}
+__device__ __forceinline__ int rep_blocks(const Rep &rp) {
+  const int br = kBlkLadder[rp.level].rows;
+  return blk_count(rp.g, 1, br) * blk_count(rp.g, 2, br);
 }
 
-// The tile's 18 segments into sm (starts and prefix sums); every thread
-// of the block calls it.
-__device__ void e_segments(const EnsDev &d, const Rep &rp, const TileDesc &t, TileSmem &sm) {
-  if (threadIdx.x < kSeg) {
-    int st, ln;
-    e_segment(d, rp, t.y, t.z, t.xf, t.xl, threadIdx.x, &st, &ln);
-    sm.seg_start[threadIdx.x] = st;
-    sm.seg_base[threadIdx.x + 1] = ln;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    sm.seg_base[0] = 0;
-    for (int k = 0; k < kSeg; k++) sm.seg_base[k + 1] += sm.seg_base[k];
-  }
-  __syncthreads();
-}
-
-// Stage plane p of a tile (segments 6p .. 6p+5) into LDS as interleaved
-// (x, y, z) (list entries are 3 x the index in the whole tile's staging).
-// Reals come from `reals`; images from the positions stored at the build
-// (mid == nullptr) or as their source's mid-step position plus the shift,
-// the image refresh of MDUnit::MakeBufferForBorderParticles
-// (mdunit.cc:181-191) done while staging.  Every thread calls it.
-__device__ void e_stage_plane(const EnsDev &d, const Rep &rp, int p, const double4 *reals,
-                              const double4 *mid, const TileSmem &sm, double *s3) {
-  if (mid) {
-    // branch-free two-level gather: every item reads an image entry (a
-    // clamped index for reals) and selects one load at a valid address
-    stage_plane(sm, p, s3, [&](int k, int slot) {
-      const int idx = max(slot - d.NR, rp.ibase);
-      const int gs = d.gsrc[idx], gc = d.gcode[idx];
-      const bool img = (k & 1) != 0;
-      const double4 *src = img ? mid : reals;
-      return image_of(src[img ? gs : slot], img ? gc : kCodeNone, rp.g.L);
-    });
-  } else {
-    stage_plane(sm, p, s3,
-                [&](int k, int slot) { return (k & 1) ? d.qe[slot] : reals[slot]; });
-  }
+// The position a tile stages for slot s: a real's from `reals`; an image's
+// from `mid`, its source's mid-step position plus the shift (the image
+// refresh of MDUnit::MakeBufferForBorderParticles, mdunit.cc:181-191, done
+// while staging), or, with mid == nullptr, as stored at the build.
+__device__ __forceinline__ double4 staged_pos(const EnsDev &d, const Rep &rp, const double4 *reals,
+                                              const double4 *mid, int s) {
+  if (s < d.NR) return reals[s];
+  if (!mid) return d.qe[s];
+  const int gi = s - d.NR;
+  return image_of(mid[d.gsrc[gi]], d.gcode[gi], rp.g.L);
 }
 
 // Persistent tile loop: workgroup b runs on XCD b % 8; the workgroups of one
@@ -356,7 +332,11 @@ __global__ void __launch_bounds__(kReplicaThreads) k_e_offsets(EnsDev d, int mas
         const Rep &rp = d.rep[r];
         if (chk) v = rp.n + c->g;
         else if (!c->eflags)
-          v = kind == OFF_REALS ? rp.n : kind == OFF_IMAGES ? c->g : kind == OFF_CELLS ? rp.ncell : c->ntiles;
+          v = kind == OFF_REALS    ? rp.n
+              : kind == OFF_IMAGES ? c->g
+              : kind == OFF_CELLS  ? rp.ncell
+              : kind == OFF_BLOCKS ? rep_blocks(rp)
+                                   : c->ntiles;
       }
       int tot;
       const int ex = block_excl_scan(v, &tot);
@@ -518,10 +498,8 @@ __global__ void k_e_images(EnsDev d) {
 }
 
 // Per flagged replica: split the cell counts into reals (interior) and
-// images (outer layer), scan each into absolute slots (the counting sort of
-// MeshList::MakeMesh, meshlist.cc:176-228), then cut every interior cell
-// row into tiles of at most T consecutive reals and record each tile's
-// staging size.
+// images (outer layer) and scan each into absolute slots (the counting sort
+// of MeshList::MakeMesh, meshlist.cc:176-228).
 __global__ void __launch_bounds__(kReplicaThreads) k_e_cells(EnsDev d) {
   const int r = blockIdx.x;
   RepCtl *c = d.ctl + r;
@@ -568,70 +546,6 @@ __global__ void __launch_bounds__(kReplicaThreads) k_e_cells(EnsDev d) {
     }
     carry_in += tin;
     carry_out += tout;
-  }
-  __syncthreads();
-  const int T = d.T;
-  const int rows = g.ms[1] * g.ms[2];
-  int tcarry = 0, smax = 0;
-  for (int base = 0; base < rows; base += blockDim.x) {
-    const int w = base + threadIdx.x;
-    int y = 0, z = 0, first = 0, cnt = 0, nt = 0;
-    if (w < rows) {
-      y = w % g.ms[1] + 1;
-      z = w / g.ms[1] + 1;
-      const int a = rp.cbase + cell_linear(g, 1, y, z), b = rp.cbase + cell_linear(g, g.ms[0], y, z);
-      first = d.scan_in[a];
-      cnt = d.scan_in[b] + d.cnt_in[b] - first;
-      nt = (cnt + T - 1) / T;
-    }
-    int tot;
-    const int ex = block_excl_scan(nt, &tot);
-    for (int k = 0; k < nt; k++) {
-      TileDesc t;
-      t.i0 = first + k * T;
-      t.ni = min(T, cnt - k * T);
-      t.y = y;
-      t.z = z;
-      // cells holding the tile's first and last real (row-local slots)
-      const int lo = k * T, hi = k * T + t.ni - 1;
-      int acc = 0, xf = 1, xl = 1;
-      for (int x = 1; x <= g.ms[0]; x++) {
-        const int cc = d.cnt_in[rp.cbase + cell_linear(g, x, y, z)];
-        if (cc > 0 && lo >= acc && lo < acc + cc) xf = x;
-        if (cc > 0 && hi >= acc && hi < acc + cc) xl = x;
-        acc += cc;
-      }
-      t.xf = xf;
-      t.xl = xl;
-      // largest staging of one z-plane of partners (six segments)
-      int stage = 0;
-      for (int pl = 0; pl < kPlanes; pl++) {
-        int ps = 0;
-        for (int sgi = 6 * pl; sgi < 6 * pl + 6; sgi++) {
-          int st, ln;
-          e_segment(d, rp, y, z, xf, xl, sgi, &st, &ln);
-          ps += ln;
-        }
-        stage = max(stage, ps);
-      }
-      t.stage = stage;
-      t.pad = r;
-      d.tiles[rp.tbase + tcarry + ex + k] = t;
-      smax = max(smax, stage);
-    }
-    tcarry += tot;
-  }
-  // unused slots of the region: empty tiles
-  for (int s = tcarry + threadIdx.x; s < rp.tcap; s += blockDim.x) {
-    TileDesc t{};
-    t.pad = r;
-    d.tiles[rp.tbase + s] = t;
-  }
-  smax = block_max_int(smax);
-  if (threadIdx.x == 0) {
-    c->ntiles = tcarry;
-    c->stage_max = smax;
-    if (smax > d.stage_cap || tcarry > rp.tcap) c->eflags |= EF_STAGE;
   }
 }
 
@@ -699,74 +613,154 @@ __global__ void k_e_gather_images(EnsDev d) {
   });
 }
 
-// Full neighbour list per tile (as the single-system build): lane = real,
-// every partner in its 27 cells with r^2 <= SL^2 (meshlist.cc:354-362),
-// in the plane-list form at list[lbase + ((tile * kcap) + k) * T + lane].
-template <int T>
-__global__ void __launch_bounds__(T) k_e_build(EnsDev d) {
-  extern __shared__ double stage_mem[];
-  __shared__ TileSmem sm;
+// ---- block tiles of the flagged replicas, after their sort and gathers
+// (the tiles cut the x-sorted cell rows).  A replica whose rebuild already
+// failed is passed over: its flags are read once per workgroup, from
+// kernels before.
+
+// Flags of replica r as one workgroup reads them (one load, then shared).
+__device__ __forceinline__ int rep_flags(const EnsDev &d, int r) {
+  __shared__ int f;
+  __syncthreads();
+  if (threadIdx.x == 0) f = d.ctl[r].eflags;
+  __syncthreads();
+  return f;
+}
+
+// tiles of every block of the flagged replicas, one workgroup per block
+__global__ void k_e_blk_count(EnsDev d) {
+  __shared__ int hist[kMaxCellsX + 1];
   __shared__ int soff[kOffLds];
-  const OffView v = load_offsets(d, OFF_TILES, soff);
-  // work items: the tiles of every flagged replica, spread over all blocks
+  const OffView v = load_offsets(d, OFF_BLOCKS, soff);
   for (int item = blockIdx.x; item < v.total; item += gridDim.x) {
     const int f = replica_at(v.off, v.nf, item);
-    const int s = item - v.off[f];
     const int r = d.flagged[f];
+    if (rep_flags(d, r)) continue;
     const Rep &rp = d.rep[r];
-    const Geom &g = rp.g;
-    {
-      const TileDesc t = d.tiles[rp.tbase + s];
-      double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-      const int lane = threadIdx.x;
-      const bool act = lane < t.ni;
-      const int i = t.i0 + lane;
-      const double4 qi = act ? d.qe[i] : make_double4(0.0, 0.0, 0.0, 0.0);
-      e_segments(d, rp, t, sm);
-      const int self = act ? sm.seg_base[8] + (i - sm.seg_start[8]) : -1;
-      const int cx = interior_coord(qi.x, g.inv_w[0], g.ms[0]);
-      GroupWriter<T> gw(d.list + rp.lbase + (size_t)s * rp.kcap * T + (size_t)lane * kILP, rp.kcap);
-      int b1 = 0, b2 = 0;
-      // the 27 cells plane by plane (dz = -1, 0, +1), each plane staged alone
-      for (int pl = 0; pl < kPlanes; pl++) {
-        if (pl > 0) __syncthreads();
-        e_stage_plane(d, rp, pl, d.qe, nullptr, sm, sx);
-        const int pb = sm.seg_base[6 * pl];
-        if (act) {
-          for (int rr = 3 * pl; rr < 3 * pl + 3; rr++) {
-            const int y = t.y + rr % 3 - 1, z = t.z + rr / 3 - 1;
-            for (int dx = -1; dx <= 1; dx++) {
-              const int x = cx + dx;
-              const int cl = rp.cbase + cell_linear(g, x, y, z);
-              int lo, len;
-              if (is_outer(g, x, y, z)) {
-                lo = sm.seg_base[2 * rr + 1] + (d.scan_out[cl] - sm.seg_start[2 * rr + 1]);
-                len = d.cnt_out[cl];
-              } else {
-                lo = sm.seg_base[2 * rr] + (d.scan_in[cl] - sm.seg_start[2 * rr]);
-                len = d.cnt_in[cl];
-              }
-              for (int o = lo; o < lo + len; o++) {
-                const int e = 3 * (o - pb);
-                if (list_r2(sx[e], sy[e], sz[e], qi.x, qi.y, qi.z) <= g.SL2 && o != self)
-                  gw.put(plane_entry(o - pb));
-              }
-            }
-          }
-          gw.pad_plane(sm.seg_base[6 * pl + 6] - pb);
+    const int b = item - v.off[f];
+    const int br = kBlkLadder[rp.level].rows, BY = blk_count(rp.g, 1, br);
+    const Cells c = rep_cells(d, rp);
+    blk_hist(rp.g, c, b % BY, b / BY, br, hist);
+    const int nt = blk_ntiles(rp.g, c, d.qe, hist, b % BY, b / BY, br, kBlkLadder[rp.level].target,
+                              d.stage_cap);
+    if (threadIdx.x == 0) d.blk_nt[rp.bbase + b] = nt;
+  }
+}
+
+// per flagged replica: each block's first tile, the tile count against the
+// replica's tile slots (EF_TILES), and its unused slots emptied
+__global__ void __launch_bounds__(kReplicaThreads) k_e_blk_scan(EnsDev d) {
+  const int r = blockIdx.x;
+  RepCtl *c = d.ctl + r;
+  if (!c->phase || c->eflags) return;   // as k_e_img_scan
+  const Rep &rp = d.rep[r];
+  const int nb = rep_blocks(rp);
+  int carry = 0;
+  for (int base = 0; base < nb; base += blockDim.x) {
+    const int b = base + threadIdx.x;
+    const int v = b < nb ? d.blk_nt[rp.bbase + b] : 0;
+    int tot;
+    const int ex = block_excl_scan(v, &tot);
+    if (b < nb) d.blk_toff[rp.bbase + b] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    c->ntiles = carry;
+    c->stage_max = 0;
+    if (carry > rp.tcap) c->eflags |= EF_TILES;
+  }
+  for (int s = carry + threadIdx.x; s < rp.tcap; s += blockDim.x) d.btiles[rp.tbase + s] = BlkTile{0, 0, 0, r};
+}
+
+// the tiles of every block of the flagged replicas, one workgroup per
+// block, one wavefront per tile; a staging above the LDS or a tile above
+// kBlk reals raises EF_STAGE with the largest staging
+__global__ void k_e_blk_tiles(EnsDev d) {
+  __shared__ int hist[kMaxCellsX + 1];
+  __shared__ int soff[kOffLds];
+  const OffView v = load_offsets(d, OFF_BLOCKS, soff);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int item = blockIdx.x; item < v.total; item += gridDim.x) {
+    const int f = replica_at(v.off, v.nf, item);
+    const int r = d.flagged[f];
+    if (rep_flags(d, r) & ~EF_STAGE) continue;
+    RepCtl *c = d.ctl + r;
+    const Rep &rp = d.rep[r];
+    const int b = item - v.off[f];
+    const int br = kBlkLadder[rp.level].rows, BY = blk_count(rp.g, 1, br);
+    const Cells cl = rep_cells(d, rp);
+    blk_hist(rp.g, cl, b % BY, b / BY, br, hist);
+    const int nt = d.blk_nt[rp.bbase + b], t0 = d.blk_toff[rp.bbase + b];
+    for (int k = w; k < nt; k += nw) {
+      const int s = rp.tbase + t0 + k;
+      int nreal;
+      // the replica's id goes with the tile in one store: the step of the
+      // other replicas, running meanwhile, reads it to pass the tile over
+      const int S = blk_make_tile(rp.g, cl, d.qe, hist, b % BY, b / BY, br, k, nt, r, d.btiles + s,
+                                  d.binfo + s, &nreal);
+      if (lane == 0) {
+        d.tile_S[s] = S;
+        if (S > d.stage_cap || nreal > kBlk) {
+          atomicOr(&c->eflags, EF_STAGE);
+          atomicMax(&c->stage_max, S);
         }
-        if (pl == 0) b1 = gw.cnt;
-        if (pl == 1) b2 = gw.cnt;
       }
-      const int cnt = gw.cnt;
-      if (act) {
-        d.nn[i] = cnt < rp.kcap ? cnt : rp.kcap;
-        d.nb[i] = min(b1, rp.kcap) | (min(b2, rp.kcap) << 16);
-      }
-      const int km = block_max_int(cnt);
-      if (threadIdx.x == 0) d.tile_kmax[rp.tbase + s] = km;
-      __syncthreads();
     }
+  }
+}
+
+// per flagged replica: each tile's first staging index entry (relative to
+// the replica's)
+__global__ void __launch_bounds__(kReplicaThreads) k_e_tile_scan(EnsDev d) {
+  const int r = blockIdx.x;
+  RepCtl *c = d.ctl + r;
+  if (!c->phase || c->eflags) return;
+  const Rep &rp = d.rep[r];
+  int carry = 0;
+  for (int base = 0; base < c->ntiles; base += blockDim.x) {
+    const int k = base + threadIdx.x;
+    const int v = k < c->ntiles ? d.tile_S[rp.tbase + k] : 0;
+    int tot;
+    const int ex = block_excl_scan(v, &tot);
+    if (k < c->ntiles) d.btiles[rp.tbase + k].s0 = carry + ex;
+    carry += tot;
+  }
+}
+
+// the staging index of every tile of the flagged replicas, one workgroup per tile
+__global__ void k_e_blk_sidx(EnsDev d) {
+  __shared__ int soff[kOffLds];
+  const OffView v = load_offsets(d, OFF_TILES, soff);
+  for (int item = blockIdx.x; item < v.total; item += gridDim.x) {
+    const int f = replica_at(v.off, v.nf, item);
+    const int r = d.flagged[f];
+    if (rep_flags(d, r)) continue;
+    const Rep &rp = d.rep[r];
+    const int s = rp.tbase + item - v.off[f];
+    blk_write_sidx(rp.g, rep_cells(d, rp), d.binfo + s, d.sidx + rp.sbase + d.btiles[s].s0);
+  }
+}
+
+// The neighbour lists of every tile of the flagged replicas
+// (build_tile_lists, as the single system builds them), one workgroup per
+// tile; the longest list of each tile to tile_kmax.
+__global__ void __launch_bounds__(kBlk) k_e_build(EnsDev d) {
+  extern __shared__ double stage_mem[];
+  __shared__ int soff[kOffLds];
+  const OffView v = load_offsets(d, OFF_TILES, soff);
+  for (int item = blockIdx.x; item < v.total; item += gridDim.x) {
+    const int f = replica_at(v.off, v.nf, item);
+    const int r = d.flagged[f];
+    if (rep_flags(d, r)) continue;
+    const Rep &rp = d.rep[r];
+    const int k = item - v.off[f], s = rp.tbase + k;
+    const BlkTile t = d.btiles[s];
+    const size_t region = rp.lbase + (size_t)k * rp.kcap * kBlk;
+    const int cnt = build_tile_lists(d.qe, t, d.binfo + s, d.sidx + rp.sbase + t.s0, rp.g, rp.kcap,
+                                     d.list_nat + region, d.list + region, d.nn, d.nb,
+                                     d.treal + (size_t)s * kBlk, stage_mem);
+    const int km = block_max_int(cnt);
+    if (threadIdx.x == 0) d.tile_kmax[s] = km;
   }
 }
 
@@ -807,15 +801,14 @@ __global__ void k_e_end(EnsDev d) {
 // q + p dt/2 (images formed on the fly), p += F dt over the tile lists,
 // q_end = q_mid + p dt/2 (fcalculator.cc:18-30), and the next step's
 // q_end + p dt/2 into the other mid-step buffer.  The arithmetic is the
-// single-system kick's (kick_term, kILP chains, drift_half).
-template <int T>
-__global__ void __launch_bounds__(T) k_e_step(EnsDev d, int mode) {
+// single-system kick's (kick_term, kILP chains, drift_half) over the same
+// lists, so every replica's sums are the single system's bit for bit.
+__global__ void __launch_bounds__(kBlk) k_e_step(EnsDev d, int mode) {
   extern __shared__ double stage_mem[];
-  __shared__ TileSmem sm;
   const double dt2 = d.dt * 0.5;
   auto tile_step = [&](int slot) {
-    const TileDesc t = d.tiles[slot];
-    if (t.ni == 0) return;
+    const BlkTile t = d.btiles[slot];
+    if (t.nreal == 0) return;
     const int r = t.pad;
     const RepCtl *c = d.ctl + r;
     // mode 1: only replicas not rebuilding in this step (they run beside
@@ -826,32 +819,27 @@ __global__ void __launch_bounds__(T) k_e_step(EnsDev d, int mode) {
     const Rep &rp = d.rep[r];
     const double4 *qh = c->par ? d.qh1 : d.qh0;
     double4 *qn = c->par ? d.qh0 : d.qh1;
-    double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
     const int lane = threadIdx.x;
-    const bool act = lane < t.ni;
-    const int i = t.i0 + lane;
+    const bool act = col_active(lane, t.nreal);
+    const int i = act ? d.treal[(size_t)slot * kBlk + lane] : 0;
     double4 qi = make_double4(0.0, 0.0, 0.0, 0.0), pm = qi;
-    int m = 0, bnd = 0;
+    int m = 0;
     if (act) {
       qi = qh[i];
       m = d.nn[i];
-      bnd = d.nb[i];
       pm = d.mom[i];
     }
-    e_segments(d, rp, t, sm);
-    const unsigned short *lst = d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * T + (size_t)lane * kILP;
+    stage_tile<kBlk>(t.S, d.sidx + rp.sbase + t.s0, stage_mem,
+                     [&](int s) { return staged_pos(d, rp, qh, qh, s); });
+    const unsigned short *lst =
+        d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * kBlk + (size_t)lane * kILP;
     const double CL2 = rp.g.CL2, xi = qi.x, yi = qi.y, zi = qi.z;
-    // the single-system kick's sums, bit for bit: entry k feeds chain
-    // k % kILP; partners are staged one z-plane at a time
+    // the single-system kick's sums: entry k feeds chain k % kILP
     double ax[kILP], ay[kILP], az[kILP];
 #pragma unroll
     for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
-    for (int pl = 0; pl < kPlanes; pl++) {
-      if (pl > 0) __syncthreads();
-      e_stage_plane(d, rp, pl, qh, qh, sm, sx);
-      if (!act) continue;
-      plane_groups<T>(lst, plane_begin(pl, bnd), plane_end(pl, bnd, m), stage_mem,
-                      [&](int u, double xj, double yj, double zj) {
+    if (act) {
+      plane_groups<kBlk>(lst, 0, m, stage_mem, [&](int u, double xj, double yj, double zj) {
         kick_term(xj, yj, zj, xi, yi, zi, d.pk, CL2, ax[u], ay[u], az[u]);
       });
     }
@@ -874,7 +862,7 @@ __global__ void __launch_bounds__(T) k_e_step(EnsDev d, int mode) {
       d.qe[i] = q;
       qn[i] = drift_half(q, p, dt2);
     }
-    v2 = T == 64 ? wave_max(v2) : block_max(v2);
+    v2 = block_max(v2);
     if (threadIdx.x == 0) d.partial[slot] = v2;
     __syncthreads();
   };
@@ -1083,62 +1071,55 @@ __global__ void __launch_bounds__(kReplicaThreads) k_e_decide(EnsDev d, int grp)
 // KE, PE and virial per tile (observer.cc:5-79 over the full list, halves
 // for pairs counted from both sides); images at the positions the reference's
 // ghosts hold: the last step's mid-step refresh, or the build if no step ran.
-template <int T>
-__global__ void __launch_bounds__(T) k_e_observe(EnsDev d, double C0, double C2) {
+__global__ void __launch_bounds__(kBlk) k_e_observe(EnsDev d, double C0, double C2) {
   extern __shared__ double stage_mem[];
-  __shared__ TileSmem sm;
-  __shared__ double red[3][T];
+  __shared__ double red[3][kBlk / 64];
   const int slot = blockIdx.x;
-  const TileDesc t = d.tiles[slot];
+  const BlkTile t = d.btiles[slot];
   const int r = t.pad;
   const RepCtl *c = d.ctl + r;
   const int lane = threadIdx.x;
-  if (t.ni == 0 || c->error || c->need_rebuild) {
+  if (t.nreal == 0 || c->error || c->need_rebuild) {
     if (lane < 3) d.obs_partial[3 * (size_t)slot + lane] = 0.0;
     return;
   }
   const Rep &rp = d.rep[r];
+  // reals at the end of the step; images at their mid-step refresh when a
+  // step ran since the build, else as built
   const double4 *mid = c->stepped ? (c->par ? d.qh0 : d.qh1) : nullptr;
   double *sx = stage_mem, *sy = stage_mem + 1, *sz = stage_mem + 2;
-  const bool act = lane < t.ni;
-  const int i = t.i0 + lane;
-  e_segments(d, rp, t, sm);
+  const bool act = col_active(lane, t.nreal);
+  const int i = act ? d.treal[(size_t)slot * kBlk + lane] : 0;
+  stage_tile<kBlk>(t.S, d.sidx + rp.sbase + t.s0, stage_mem,
+                   [&](int s) { return staged_pos(d, rp, d.qe, mid, s); });
   double ke = 0.0, pe = 0.0, vir = 0.0;
-  const int m = act ? d.nn[i] : 0, bnd = act ? d.nb[i] : 0;
+  const int m = act ? d.nn[i] : 0;
   const double4 qi = act ? d.qe[i] : make_double4(0.0, 0.0, 0.0, 0.0);
-  const unsigned short *lst = d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * T + (size_t)lane * kILP;
-  for (int pl = 0; pl < kPlanes; pl++) {
-    if (pl > 0) __syncthreads();
-    e_stage_plane(d, rp, pl, d.qe, mid, sm, sx);
-    // a sentinel entry lies past the cutoff
-    const int kend = plane_end(pl, bnd, m);
-    for (int k = plane_begin(pl, bnd); k < kend; k++) {
-      const double *q = staged_at(stage_mem, lst[list_index<T>(k)]);
-      const double dx = qi.x - q[0], dy = qi.y - q[1], dz = qi.z - q[2];
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 > rp.g.CL2) continue;
-      observe_term(r2, C0, C2, pe, vir);
-    }
-  }
+  const unsigned short *lst =
+      d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * kBlk + (size_t)lane * kILP;
+  for_each_partner(lst, m, (unsigned)(kEntryBytes * t.S), [&](int o) {
+    const double dx = qi.x - sx[o], dy = qi.y - sy[o], dz = qi.z - sz[o];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 <= rp.g.CL2) observe_term(r2, C0, C2, pe, vir);
+  });
   if (act) {
     const double4 p = d.mom[i];
     ke = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z;
     pe *= 0.5;
     vir *= 0.5;
   }
-  red[0][lane] = ke;
-  red[1][lane] = pe;
-  red[2][lane] = vir;
+  // fixed order: a butterfly in each wavefront, then the wavefronts in turn
+  double v[3] = {ke, pe, vir};
+  for (int k = 0; k < 3; k++)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+  if ((lane & 63) == 0)
+    for (int k = 0; k < 3; k++) red[k][lane >> 6] = v[k];
   __syncthreads();
-  for (int off = T / 2; off > 0; off >>= 1) {
-    if (lane < off) {
-      red[0][lane] += red[0][lane + off];
-      red[1][lane] += red[1][lane + off];
-      red[2][lane] += red[2][lane + off];
-    }
-    __syncthreads();
+  if (lane < 3) {
+    double s = red[lane][0];
+    for (int q = 1; q < kBlk / 64; q++) s += red[lane][q];
+    d.obs_partial[3 * (size_t)slot + lane] = s;
   }
-  if (lane < 3) d.obs_partial[3 * (size_t)slot + lane] = red[lane][0];
 }
 
 // per replica, the tile sums in tile order (fixed tree)
@@ -1207,10 +1188,10 @@ struct mdx_ensemble {
   hipEvent_t fork = nullptr, join = nullptr;
   double SL = 0, C0 = 0, C2 = 0;
   int R = 0, NR = 0, NI = 0, NC = 0, NT = 0;
-  int tile = 128;          // reals per tile
+  int tile = 512;          // reals aimed at per tile at first: 512 or 256 (kBlkLadder level 0 or 3)
   int grid_mult = 2;       // persistent step grid = grid_mult * resident workgroups
   int step_grid = 0;
-  int stage_cap = 0;       // particles per tile staging (dynamic LDS / 24 B)
+  int stage_cap = 0;       // particles a tile may stage (dynamic LDS / 24 B)
   size_t lds_limit = 65536;
   int num_cu = 256;
   std::vector<Rep> hrep;
@@ -1229,9 +1210,12 @@ struct mdx_ensemble {
       *order = nullptr, *inv = nullptr;
   int *cell_count = nullptr, *cnt_in = nullptr, *cnt_out = nullptr, *scan_in = nullptr,
       *scan_out = nullptr, *cell_start = nullptr;
-  TileDesc *tiles = nullptr;
-  unsigned short *list = nullptr;
-  size_t list_elems = 0;
+  BlkTile *btiles = nullptr;
+  BlkInfo *binfo = nullptr;
+  int *treal = nullptr, *tile_S = nullptr, *sidx = nullptr, *blk_nt = nullptr, *blk_toff = nullptr;
+  unsigned short *list = nullptr, *list_nat = nullptr;
+  size_t list_elems = 0, sidx_elems = 0;
+  int NB = 0;              // per-block slots over all replicas
   int *nn = nullptr, *nb = nullptr, *tile_kmax = nullptr;
   double *partial = nullptr, *obs_partial = nullptr, *obs_out = nullptr, *io = nullptr;
   bool have_state = false;
@@ -1289,7 +1273,6 @@ EnsDev dev_view(const mdx_ensemble *e) {
   d.NR = e->NR;
   d.NI = e->NI;
   d.NT = e->NT;
-  d.T = e->tile;
   d.stage_cap = e->stage_cap;
   d.check_list_length = e->prm.check_list_length;
   d.dt = e->prm.dt;
@@ -1322,8 +1305,15 @@ EnsDev dev_view(const mdx_ensemble *e) {
   d.scan_in = e->scan_in;
   d.scan_out = e->scan_out;
   d.cell_start = e->cell_start;
-  d.tiles = e->tiles;
+  d.btiles = e->btiles;
+  d.binfo = e->binfo;
+  d.treal = e->treal;
+  d.tile_S = e->tile_S;
+  d.sidx = e->sidx;
+  d.blk_nt = e->blk_nt;
+  d.blk_toff = e->blk_toff;
   d.list = e->list;
+  d.list_nat = e->list_nat;
   d.nn = e->nn;
   d.nb = e->nb;
   d.partial = e->partial;
@@ -1332,8 +1322,11 @@ EnsDev dev_view(const mdx_ensemble *e) {
   return d;
 }
 
-// dynamic LDS of the tile kernels: one plane of a tile plus the sentinel
+// dynamic LDS of the tile kernels: a tile's staging plus the sentinel
 size_t stage_bytes(const mdx_ensemble *e) { return (size_t)(e->stage_cap + 1) * 3 * sizeof(double); }
+
+// LDS of the list build beside its staging (ranks, row starts, columns)
+constexpr size_t kBuildLds = 12 * 1024;
 
 int launch_check(mdx_ensemble *e, const char *what) {
   hipError_t er = hipGetLastError();
@@ -1341,79 +1334,110 @@ int launch_check(mdx_ensemble *e, const char *what) {
   return MDX_OK;
 }
 
-// Tile slots and list regions for tile size T: tcap = rows + ceil(n / T)
-// bounds the tiles of any binning; lists hold kcap entries per real.
-int layout_tiles(mdx_ensemble *e, int T) {
-  size_t tb = 0, lb = 0;
+// Tile slots of a replica at its block shape: one per block plus twice its
+// reals over the reals aimed at per tile, room for the tiles that the
+// 512-real limit (a crystal's rows are uneven) and the staging limit (a
+// dense replica's blocks cut into narrower tiles) add.
+int tile_slots(const Rep &rp) {
+  const BlkShape sh = kBlkLadder[rp.level];
+  const int nb = blk_count(rp.g, 1, sh.rows) * blk_count(rp.g, 2, sh.rows);
+  return nb + (int)((2LL * rp.n + sh.target - 1) / sh.target) + 8;
+}
+
+// Regions of every replica (tiles, staging index, lists, block counts) from
+// hrep's sizes, in new buffers.  With `old` given, a replica whose tile
+// slots and list width are unchanged keeps its tiles and lists (copied);
+// the others are rebuilt before they step again.
+int layout(mdx_ensemble *e, const std::vector<Rep> *old) {
+  size_t tb = 0, lb = 0, sb = 0, bb = 0;
   for (int r = 0; r < e->R; r++) {
     Rep &rp = e->hrep[r];
     rp.tbase = (int)tb;
-    rp.tcap = rp.g.ms[1] * rp.g.ms[2] + (rp.n + T - 1) / T;
     rp.kcap = e->kcap[r];
     rp.lbase = lb;
+    rp.sbase = sb;
+    rp.bbase = (int)bb;
+    rp.bcap = rp.g.ms[1] * rp.g.ms[2];
     tb += rp.tcap;
-    lb += (size_t)rp.tcap * rp.kcap * T;
+    lb += (size_t)rp.tcap * rp.kcap * kBlk;
+    sb += (size_t)rp.tcap * (kMaxPlane + 1);
+    bb += rp.bcap;
   }
-  if (tb > (size_t)0x7fffffff) return efail(e, MDX_ERR_CAPACITY, "too many tile slots");
-  e->NT = (int)tb;
-  e->tile = T;
-  int rc;
-  if ((rc = ealloc(e, &e->tiles, tb)) || (rc = ealloc(e, &e->partial, tb)) ||
-      (rc = ealloc(e, &e->obs_partial, 3 * tb)) || (rc = ealloc(e, &e->tile_kmax, tb)) ||
-      (rc = ealloc(e, &e->list, lb)))
-    return rc;
-  e->list_elems = lb;
-  EHIP(e, hipMemsetAsync(e->tiles, 0, tb * sizeof(TileDesc), e->stream));
-  EHIP(e, hipMemcpyAsync(e->rep, e->hrep.data(), sizeof(Rep) * e->R, hipMemcpyHostToDevice, e->stream));
-  EHIP(e, hipStreamSynchronize(e->stream));
-  e->graph_dirty = true;
-  return MDX_OK;
-}
-
-// Grow the list capacity of the replicas in `grow` (their rebuild failed
-// for it) and move every other replica's lists to the new layout.
-int regrow_lists(mdx_ensemble *e, const std::vector<int> &grow) {
-  std::vector<Rep> old = e->hrep;
-  for (int r : grow) e->kcap[r] = ((std::max(e->kcap[r], e->hctl[r].kmax) * 5 / 4 + 16) + 7) & ~7;
-  size_t lb = 0;
-  for (int r = 0; r < e->R; r++) {
-    e->hrep[r].kcap = e->kcap[r];
-    e->hrep[r].lbase = lb;
-    lb += (size_t)e->hrep[r].tcap * e->kcap[r] * e->tile;
-  }
+  if (tb > (size_t)0x7fffffff || bb > (size_t)0x7fffffff)
+    return efail(e, MDX_ERR_CAPACITY, "too many tile slots");
+  BlkTile *nt = nullptr;
+  BlkInfo *ni = nullptr;
+  int *ntr = nullptr, *nts = nullptr, *nsx = nullptr, *nkm = nullptr;
   unsigned short *nl = nullptr;
+  EHIP(e, hipMalloc((void **)&nt, std::max<size_t>(tb, 1) * sizeof(BlkTile)));
+  EHIP(e, hipMalloc((void **)&ni, std::max<size_t>(tb, 1) * sizeof(BlkInfo)));
+  EHIP(e, hipMalloc((void **)&ntr, std::max<size_t>(tb * kBlk, 1) * sizeof(int)));
+  EHIP(e, hipMalloc((void **)&nts, std::max<size_t>(tb, 1) * sizeof(int)));
+  EHIP(e, hipMalloc((void **)&nsx, std::max<size_t>(sb, 1) * sizeof(int)));
+  EHIP(e, hipMalloc((void **)&nkm, std::max<size_t>(tb, 1) * sizeof(int)));
   EHIP(e, hipMalloc((void **)&nl, std::max<size_t>(lb, 1) * sizeof(unsigned short)));
-  for (int r = 0; r < e->R; r++) {
-    if (old[r].kcap != e->hrep[r].kcap) continue;
-    const size_t len = (size_t)old[r].tcap * old[r].kcap * e->tile;
-    EHIP(e, hipMemcpyAsync(nl + e->hrep[r].lbase, e->list + old[r].lbase, len * sizeof(unsigned short),
-                           hipMemcpyDeviceToDevice, e->stream));
+  EHIP(e, hipMemsetAsync(nt, 0, std::max<size_t>(tb, 1) * sizeof(BlkTile), e->stream));
+  if (old) {
+    for (int r = 0; r < e->R; r++) {
+      const Rep &o = (*old)[r], &n = e->hrep[r];
+      if (o.tcap != n.tcap || o.kcap != n.kcap) continue;
+      EHIP(e, hipMemcpyAsync(nt + n.tbase, e->btiles + o.tbase, sizeof(BlkTile) * o.tcap, hipMemcpyDeviceToDevice, e->stream));
+      EHIP(e, hipMemcpyAsync(ni + n.tbase, e->binfo + o.tbase, sizeof(BlkInfo) * o.tcap, hipMemcpyDeviceToDevice, e->stream));
+      EHIP(e, hipMemcpyAsync(ntr + (size_t)n.tbase * kBlk, e->treal + (size_t)o.tbase * kBlk, sizeof(int) * o.tcap * kBlk, hipMemcpyDeviceToDevice, e->stream));
+      EHIP(e, hipMemcpyAsync(nts + n.tbase, e->tile_S + o.tbase, sizeof(int) * o.tcap, hipMemcpyDeviceToDevice, e->stream));
+      EHIP(e, hipMemcpyAsync(nkm + n.tbase, e->tile_kmax + o.tbase, sizeof(int) * o.tcap, hipMemcpyDeviceToDevice, e->stream));
+      EHIP(e, hipMemcpyAsync(nsx + n.sbase, e->sidx + o.sbase, sizeof(int) * o.tcap * (kMaxPlane + 1), hipMemcpyDeviceToDevice, e->stream));
+      EHIP(e, hipMemcpyAsync(nl + n.lbase, e->list + o.lbase, sizeof(unsigned short) * o.tcap * o.kcap * kBlk, hipMemcpyDeviceToDevice, e->stream));
+    }
   }
   EHIP(e, hipStreamSynchronize(e->stream));
-  (void)hipFree(e->list);
+  void *oldbufs[] = {e->btiles, e->binfo, e->treal, e->tile_S, e->sidx, e->tile_kmax, e->list};
+  for (void *b : oldbufs)
+    if (b) (void)hipFree(b);
+  e->btiles = nt;
+  e->binfo = ni;
+  e->treal = ntr;
+  e->tile_S = nts;
+  e->sidx = nsx;
+  e->tile_kmax = nkm;
   e->list = nl;
   e->list_elems = lb;
+  e->sidx_elems = sb;
+  e->NT = (int)tb;
+  e->NB = (int)bb;
+  int rc;
+  if ((rc = ealloc(e, &e->list_nat, lb)) || (rc = ealloc(e, &e->partial, tb)) ||
+      (rc = ealloc(e, &e->obs_partial, 3 * tb)) || (rc = ealloc(e, &e->blk_nt, bb)) ||
+      (rc = ealloc(e, &e->blk_toff, bb)))
+    return rc;
   EHIP(e, hipMemcpyAsync(e->rep, e->hrep.data(), sizeof(Rep) * e->R, hipMemcpyHostToDevice, e->stream));
+  EHIP(e, hipStreamSynchronize(e->stream));
   e->graph_dirty = true;
   return MDX_OK;
 }
 
-int set_stage_cap(mdx_ensemble *e, int cap) {
-  const size_t need = (size_t)(cap + 1) * 3 * sizeof(double) + 1024;
-  if (need > e->lds_limit)
-    return efail(e, MDX_ERR_CAPACITY, "tile staging of %d particles exceeds LDS (%zu B)", cap,
-                 e->lds_limit);
-  if (cap > kMaxPlane)
-    return efail(e, MDX_ERR_CAPACITY,
-                 "a tile plane of %d particles exceeds the 16-bit list offsets (%d particles)", cap,
-                 kMaxPlane);
-  e->stage_cap = cap;
-  e->graph_dirty = true;
-  return MDX_OK;
+// first block shape of a tile target: 512 reals (kBlkLadder level 0) or 256 (level 3)
+int tile_level(int tile) { return tile == 512 ? 0 : 3; }
+
+// Every replica at block shape `level` (or the one it already needed, if
+// larger), new regions, nothing kept.
+int layout_level(mdx_ensemble *e, int level) {
+  for (auto &rp : e->hrep) {
+    rp.level = std::max(rp.level, level);
+    rp.tcap = tile_slots(rp);
+  }
+  return layout(e, nullptr);
 }
 
-// the masked rebuild after k_e_begin
-template <int T>
+// The largest tile staging the LDS allows: the list build's own LDS beside
+// it, the sentinel's list entry within 16 bits.
+int max_stage_cap(const mdx_ensemble *e) {
+  const long long lds = ((long long)e->lds_limit - (long long)kBuildLds) / (3 * (long long)sizeof(double)) - 1;
+  return (int)std::min<long long>(kMaxPlane, lds);
+}
+
+// the masked rebuild after k_e_begin: bin, images, sort and gather, then
+// the block tiles of the sorted rows and their lists
 void enqueue_rebuild_body(mdx_ensemble *e, const EnsDev &d) {
   k_e_offsets<<<1, kReplicaThreads, 0, e->stream>>>(d, 1 << OFF_REALS);
   k_e_wrap<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
@@ -1421,29 +1445,29 @@ void enqueue_rebuild_body(mdx_ensemble *e, const EnsDev &d) {
   k_e_offsets<<<1, kReplicaThreads, 0, e->stream>>>(d, 1 << OFF_IMAGES);
   k_e_images<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
   k_e_cells<<<e->R, kReplicaThreads, 0, e->stream>>>(d);
-  k_e_offsets<<<1, kReplicaThreads, 0, e->stream>>>(d, (1 << OFF_IMAGES) | (1 << OFF_CELLS) | (1 << OFF_TILES));
+  k_e_offsets<<<1, kReplicaThreads, 0, e->stream>>>(d, (1 << OFF_IMAGES) | (1 << OFF_CELLS) | (1 << OFF_BLOCKS));
   k_e_scatter<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
   k_e_cell_sort<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
   k_e_gather_reals<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
   k_e_gather_images<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
-  k_e_build<T><<<e->step_grid, T, stage_bytes(e), e->stream>>>(d);
+  k_e_blk_count<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
+  k_e_blk_scan<<<e->R, kReplicaThreads, 0, e->stream>>>(d);
+  k_e_blk_tiles<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
+  k_e_tile_scan<<<e->R, kReplicaThreads, 0, e->stream>>>(d);
+  k_e_offsets<<<1, kReplicaThreads, 0, e->stream>>>(d, 1 << OFF_TILES);
+  k_e_blk_sidx<<<kRebuildBlocks, kBlock, 0, e->stream>>>(d);
+  k_e_build<<<e->step_grid, kBlk, stage_bytes(e), e->stream>>>(d);
   k_e_end<<<e->R, kBlock, 0, e->stream>>>(d);
 }
 
-template <int T>
-int enqueue_rebuild_t(mdx_ensemble *e) {
+int enqueue_rebuild(mdx_ensemble *e) {
   const EnsDev d = dev_view(e);
   k_e_begin<<<e->R, kBlock, 0, e->stream>>>(d);
-  enqueue_rebuild_body<T>(e, d);
+  enqueue_rebuild_body(e, d);
   return launch_check(e, "rebuild");
 }
 
-// One step: the masked rebuild on `stream`, the step of the replicas not
-// rebuilding on `side` at the same time (forked after k_e_begin has flagged
-// the replicas), the step of the rebuilt replicas after the rebuild, then
-// (joined) the expiry check of all.  Each replica's kernels run in the same
-// order as one after another, so its arithmetic is unchanged.
-template <int T>
+// the expiry check of one group of replicas on stream st
 void enqueue_check(mdx_ensemble *e, const EnsDev &d, hipStream_t st, int grp) {
   k_e_budget<<<e->R, kBlock, 0, st>>>(d, grp);
   k_e_offsets<<<1, kReplicaThreads, 0, st>>>(d, 1 << (grp == 2 ? OFF_CHECK2 : OFF_CHECK));
@@ -1459,36 +1483,25 @@ void enqueue_check(mdx_ensemble *e, const EnsDev &d, hipStream_t st, int grp) {
 // Each replica's kernels run in the same order as one after another, so
 // its arithmetic is unchanged; the two branches touch disjoint replicas and
 // keep their own check lists.
-template <int T>
-int enqueue_step_t(mdx_ensemble *e) {
+int enqueue_step(mdx_ensemble *e) {
   const EnsDev d = dev_view(e);
   k_e_begin<<<e->R, kBlock, 0, e->stream>>>(d);
   EHIP(e, hipEventRecord(e->fork, e->stream));
   EHIP(e, hipStreamWaitEvent(e->side, e->fork, 0));
-  k_e_step<T><<<e->step_grid, T, stage_bytes(e), e->side>>>(d, 1);
-  enqueue_check<T>(e, d, e->side, 1);
+  k_e_step<<<e->step_grid, kBlk, stage_bytes(e), e->side>>>(d, 1);
+  enqueue_check(e, d, e->side, 1);
   EHIP(e, hipEventRecord(e->join, e->side));
-  enqueue_rebuild_body<T>(e, d);
-  k_e_step<T><<<e->step_grid, T, stage_bytes(e), e->stream>>>(d, 2);
-  enqueue_check<T>(e, d, e->stream, 2);
+  enqueue_rebuild_body(e, d);
+  k_e_step<<<e->step_grid, kBlk, stage_bytes(e), e->stream>>>(d, 2);
+  enqueue_check(e, d, e->stream, 2);
   EHIP(e, hipStreamWaitEvent(e->stream, e->join, 0));
   return launch_check(e, "step");
-}
-
-int enqueue_rebuild(mdx_ensemble *e) {
-  return e->tile == 64 ? enqueue_rebuild_t<64>(e) : enqueue_rebuild_t<128>(e);
-}
-
-int enqueue_step(mdx_ensemble *e) {
-  return e->tile == 64 ? enqueue_step_t<64>(e) : enqueue_step_t<128>(e);
 }
 
 // resident workgroups of the step kernel times grid_mult, a multiple of 8
 int size_step_grid(mdx_ensemble *e) {
   int per_cu = 0;
-  hipError_t er = e->tile == 64
-                      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_e_step<64>, 64, stage_bytes(e))
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_e_step<128>, 128, stage_bytes(e));
+  hipError_t er = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_e_step, kBlk, stage_bytes(e));
   if (er != hipSuccess || per_cu < 1) per_cu = 1;
   long long g = (long long)per_cu * e->num_cu * e->grid_mult;
   g = std::min<long long>(g, ((long long)e->NT + 7) / 8 * 8);
@@ -1508,37 +1521,45 @@ int write_ctl(mdx_ensemble *e) {
   return MDX_OK;
 }
 
-// Replicas whose rebuild failed on a capacity that can grow: grow it and
-// clear the failure so the rebuild is retried.  Returns whether any was fixed.
+// Replicas whose rebuild failed on a capacity that can grow: grow it (a
+// longer list width, more tile slots, or a smaller block shape when a
+// tile's staging overflows) and clear the failure, so the rebuild is
+// retried; the other replicas keep their tiles and lists.  Returns whether
+// any was fixed.
 int fix_capacity(mdx_ensemble *e, bool *fixed) {
   *fixed = false;
-  std::vector<int> grow;
-  int need_stage = 0;
+  const std::vector<Rep> old = e->hrep;
+  bool any = false;
   for (int r = 0; r < e->R; r++) {
     RepCtl &c = e->hctl[r];
     if (c.error != MDX_ERR_CAPACITY) continue;
-    if (c.eflags & EF_LIST) grow.push_back(r);
-    if (c.eflags & EF_STAGE) need_stage = std::max(need_stage, c.stage_max);
     if (c.eflags & EF_IMAGES) return efail(e, MDX_ERR_CAPACITY, "replica %d: image capacity", r);
-  }
-  int rc;
-  if (need_stage > 0) {
-    if ((rc = set_stage_cap(e, std::max(need_stage, std::min(need_stage + need_stage / 8 + 64, kMaxPlane))))) return rc;
-    if ((rc = size_step_grid(e))) return rc;
-  }
-  if (!grow.empty() && (rc = regrow_lists(e, grow))) return rc;
-  if (need_stage > 0 || !grow.empty()) {
-    for (int r = 0; r < e->R; r++) {
-      RepCtl &c = e->hctl[r];
-      if (c.error == MDX_ERR_CAPACITY && !(c.eflags & EF_IMAGES)) {
-        c.error = 0;
-        c.eflags = 0;
-        c.need_rebuild = 1;
-      }
+    Rep &rp = e->hrep[r];
+    if (c.eflags & EF_LIST) e->kcap[r] = ((std::max(e->kcap[r], c.kmax) * 5 / 4 + 16) + 7) & ~7;
+    if (c.eflags & EF_STAGE) {
+      if (rp.level + 1 >= kBlkLevels)
+        return efail(e, MDX_ERR_CAPACITY,
+                     "replica %d: a tile of one cell row stages %d particles, more than %d", r,
+                     c.stage_max, e->stage_cap);
+      rp.level++;
+      rp.tcap = std::max(rp.tcap, tile_slots(rp));
     }
-    if ((rc = write_ctl(e))) return rc;
-    *fixed = true;
+    if (c.eflags & EF_TILES) rp.tcap = std::max(rp.tcap + rp.tcap / 2, c.ntiles + 8);
+    any = true;
   }
+  if (!any) return MDX_OK;
+  int rc;
+  if ((rc = layout(e, &old)) || (rc = size_step_grid(e))) return rc;
+  for (int r = 0; r < e->R; r++) {
+    RepCtl &c = e->hctl[r];
+    if (c.error == MDX_ERR_CAPACITY) {
+      c.error = 0;
+      c.eflags = 0;
+      c.need_rebuild = 1;
+    }
+  }
+  if ((rc = write_ctl(e))) return rc;
+  *fixed = true;
   return MDX_OK;
 }
 
@@ -1585,28 +1606,25 @@ int reset_state(mdx_ensemble *e) {
   if (rc) return rc;
   EHIP(e, hipMemsetAsync(e->meta, 0, sizeof(int) * 4, e->stream));
   // the first step's pairlist construction, retried while a capacity grows
-  for (int attempt = 0; attempt < 4; attempt++) {
+  for (int attempt = 0; attempt < 2 * kBlkLevels; attempt++) {
     if ((rc = enqueue_rebuild(e))) return rc;
     if ((rc = read_ctl(e))) return rc;
     bool fixed = false;
     if ((rc = fix_capacity(e, &fixed))) return rc;
     if (!fixed) break;
   }
-  int smax = 0;
   for (int r = 0; r < e->R; r++) {
     RepCtl &c = e->hctl[r];
     if (c.error) return efail(e, c.error, "replica %d: first pairlist build failed (flags %d)", r, c.eflags);
     c.target = 0;
-    smax = std::max(smax, c.stage_max);
   }
-  if ((rc = write_ctl(e))) return rc;
-  // LDS per tile sized to the tiles as built, with room to grow
-  const int cap = std::max(smax, std::min(smax + smax / 8 + 64, kMaxPlane));
-  if (cap < e->stage_cap) {
-    if ((rc = set_stage_cap(e, cap)) || (rc = size_step_grid(e))) return rc;
-  }
-  return MDX_OK;
+  return write_ctl(e);
 }
+
+// graph launches between two reads of the replicas' state, and the most
+// capacity growth rounds one call makes
+constexpr int kLaunchChunk = 64;
+constexpr int kMaxFixRounds = 4 * kBlkLevels;
 
 }  // namespace
 
@@ -1628,6 +1646,9 @@ int mdx_ensemble_create(const mdx_params *prm, int nrep, const int *n, const dou
       if (!(L3[3 * r + a] >= 2.0 * SL))
         return efail(nullptr, MDX_ERR_BOX, "replica %d: box edge %g < 2 x search length %g", r,
                      L3[3 * r + a], SL);
+    if (!(L3[3 * r] / SL < (double)kMaxCellsX))
+      return efail(nullptr, MDX_ERR_BOX, "replica %d: box edge %g in x holds more than %d cells", r,
+                   L3[3 * r], kMaxCellsX);
     NR += n[r];
     NI += (long long)kMaxImages * n[r];
   }
@@ -1679,6 +1700,7 @@ int mdx_ensemble_create(const mdx_params *prm, int nrep, const int *n, const dou
     // partners within SL at the mean density, with room for fluctuations
     const double mean = 4.18879 * SL * SL * SL * (n[r] / V);
     e->kcap[r] = ((int)(mean * 1.4) + 24 + kPlanes * (kILP - 1) + 7) & ~7;
+    rp.level = 0;
   }
   e->NC = cb;
   int rc = MDX_OK;
@@ -1725,18 +1747,13 @@ int mdx_ensemble_create(const mdx_params *prm, int nrep, const int *n, const dou
     ;
   if (!rc && hipMemsetAsync(e->meta, 0, 4 * sizeof(int), e->stream) != hipSuccess)
     rc = efail(e, MDX_ERR_HIP, "hipMemsetAsync failed");
-  if (!rc) rc = layout_tiles(e, 128);
-  // staging of the densest replica: its 9 cell rows by 4 cells, doubled
+  // every tile may stage as much as the LDS allows (two workgroups of kBlk
+  // threads fit a CU at any staging up to it)
   if (!rc) {
-    double dmax = 0.0;
-    for (const Rep &rp : e->hrep) {
-      const double cellv = (rp.g.L[0] / rp.g.ms[0]) * (rp.g.L[1] / rp.g.ms[1]) * (rp.g.L[2] / rp.g.ms[2]);
-      dmax = std::max(dmax, rp.n / (rp.g.L[0] * rp.g.L[1] * rp.g.L[2]) * cellv);
-    }
-    const int want = (int)(dmax * 12.0 * 1.25) + 256;   // one plane: 3 rows of 4 cells
-    const int lim = (int)((e->lds_limit - 1024) / 24);
-    rc = set_stage_cap(e, std::min(std::min(want, lim), kMaxPlane));
+    e->stage_cap = max_stage_cap(e);
+    if (e->stage_cap < 1024) rc = efail(e, MDX_ERR_CAPACITY, "LDS of %zu B per workgroup", e->lds_limit);
   }
+  if (!rc) rc = layout_level(e, 0);
   if (!rc) rc = size_step_grid(e);
   if (rc) {
     g_ens_create_error = e->err.empty() ? "allocation failed" : e->err;
@@ -1755,8 +1772,9 @@ int mdx_ensemble_destroy(mdx_ensemble *e) {
                   e->pos_tmp, e->mom_tmp, e->gpos_tmp, e->init_q, e->init_p, e->tag, e->tag_tmp,
                   e->gsrc, e->gcode, e->gsrc_tmp, e->gcode_tmp, e->cellid, e->slot, e->nimg, e->goff,
                   e->order, e->inv, e->cell_count, e->cnt_in, e->cnt_out, e->scan_in, e->scan_out,
-                  e->cell_start, e->tiles, e->list, e->nn, e->nb, e->tile_kmax, e->partial, e->obs_partial,
-                  e->obs_out, e->io};
+                  e->cell_start, e->btiles, e->binfo, e->treal, e->tile_S, e->sidx, e->blk_nt,
+                  e->blk_toff, e->list, e->list_nat, e->nn, e->nb, e->tile_kmax, e->partial,
+                  e->obs_partial, e->obs_out, e->io};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &ev : e->ev)
@@ -1818,12 +1836,19 @@ int mdx_ensemble_calculate(mdx_ensemble *e, int nsteps, int *rebuilds) {
   if ((rc = write_ctl(e))) return rc;
   if ((rc = capture(e))) return rc;
   EHIP(e, hipEventRecord(e->ev[0], e->stream));
-  int launches = nsteps;
-  for (int round = 0; round < 8 && launches > 0; round++) {
-    for (int k = 0; k < launches; k++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
+  // The step graph goes out in chunks; between chunks the replicas stopped
+  // on a capacity get it grown and rebuild in the next chunk, so one falls
+  // at most a chunk behind the others, and the launches after the last
+  // chunk, which step it alone, are at most that many.  The growth rounds
+  // are bounded: a capacity that never fits ends the call.
+  int launches = nsteps, fixes = 0;
+  while (launches > 0) {
+    const int chunk = std::min(launches, kLaunchChunk);
+    for (int k = 0; k < chunk; k++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
     if ((rc = read_ctl(e))) return rc;
     bool fixed = false;
     if ((rc = fix_capacity(e, &fixed))) return rc;
+    if (fixed && ++fixes > kMaxFixRounds) break;
     if ((rc = capture(e))) return rc;
     launches = 0;
     for (const auto &c : e->hctl)
@@ -1857,9 +1882,9 @@ int mdx_ensemble_calculate(mdx_ensemble *e, int nsteps, int *rebuilds) {
     const RepCtl &c = e->hctl[r];
     if (c.steps < c.target)
       return efail(e, MDX_ERR_CAPACITY,
-                   "replica %d completed %d of %d steps: its capacity was grown in every launch "
-                   "round without the rebuild fitting (mdx_ensemble_status gives the counts)",
-                   r, c.steps - (c.target - nsteps), nsteps);
+                   "replica %d completed %d of %d steps: its capacity was grown %d times without "
+                   "the rebuild fitting (mdx_ensemble_status gives the counts)",
+                   r, c.steps - (c.target - nsteps), nsteps, kMaxFixRounds);
   }
   return worst;
 }
@@ -1879,10 +1904,7 @@ int mdx_ensemble_observe(mdx_ensemble *e, double *ke, double *pe, double *vir) {
   if (!e) return efail(nullptr, MDX_ERR_ARG, "null ensemble");
   if (!e->have_state) return efail(e, MDX_ERR_STATE, "no state");
   const EnsDev d = dev_view(e);
-  if (e->tile == 64)
-    k_e_observe<64><<<e->NT, 64, stage_bytes(e), e->stream>>>(d, e->C0, e->C2);
-  else
-    k_e_observe<128><<<e->NT, 128, stage_bytes(e), e->stream>>>(d, e->C0, e->C2);
+  k_e_observe<<<e->NT, kBlk, stage_bytes(e), e->stream>>>(d, e->C0, e->C2);
   k_e_observe_fold<<<e->R, kBlock, 0, e->stream>>>(d, e->obs_out);
   int rc = launch_check(e, "observe");
   if (rc) return rc;
@@ -1907,12 +1929,12 @@ int mdx_ensemble_get_config(const mdx_ensemble *e, mdx_ensemble_config *cfg) {
   return MDX_OK;
 }
 
-// Tile size and persistent grid.  Changing the tile size re-tiles every
-// replica, so it is accepted only before the first step after
-// mdx_ensemble_set_state, and rebuilds from the loaded state.
+// Reals aimed at per tile and persistent grid.  Changing the tile target
+// re-tiles every replica, so it is accepted only before the first step
+// after mdx_ensemble_set_state, and rebuilds from the loaded state.
 int mdx_ensemble_set_config(mdx_ensemble *e, int tile, int grid_mult) {
-  if (!e || (tile != 64 && tile != 128) || grid_mult < 1 || grid_mult > 16)
-    return efail(e, MDX_ERR_ARG, "tile must be 64 or 128, grid_mult in [1, 16]");
+  if (!e || (tile != 256 && tile != 512) || grid_mult < 1 || grid_mult > 16)
+    return efail(e, MDX_ERR_ARG, "tile must be 256 or 512, grid_mult in [1, 16]");
   int rc;
   const bool retile = tile != e->tile;
   if (retile && e->have_state) {
@@ -1920,7 +1942,11 @@ int mdx_ensemble_set_config(mdx_ensemble *e, int tile, int grid_mult) {
     for (const auto &c : e->hctl)
       if (c.steps > 0) return efail(e, MDX_ERR_STATE, "tile size changes only before the first step");
   }
-  if (retile && (rc = layout_tiles(e, tile))) return rc;
+  if (retile) {
+    e->tile = tile;
+    for (auto &rp : e->hrep) rp.level = 0;
+    if ((rc = layout_level(e, tile_level(tile)))) return rc;
+  }
   e->grid_mult = grid_mult;
   if ((rc = size_step_grid(e))) return rc;
   e->graph_dirty = true;
@@ -1928,11 +1954,12 @@ int mdx_ensemble_set_config(mdx_ensemble *e, int tile, int grid_mult) {
   return MDX_OK;
 }
 
-// Times trial_steps steps for every tile size and grid multiple, each from
-// the loaded state, keeps the fastest, and leaves the loaded state (with its
-// first pairlist) in place.  Results do not depend on the choice: the tile
-// size and the grid change which workgroup computes a real, not its
-// arithmetic or the order of its partners.
+// Times trial_steps steps for every tile target and grid multiple, each from
+// the loaded state (after one untimed trial per target), keeps the fastest,
+// and leaves the loaded state (with its first pairlist) in place.  Results
+// do not depend on the choice: the tiles and the grid change which
+// workgroup computes a real, not its arithmetic or the order of its
+// partners.
 int mdx_ensemble_autotune(mdx_ensemble *e, int trial_steps, mdx_ensemble_config *best) {
   if (!e || trial_steps < 1) return efail(e, MDX_ERR_ARG, "bad arguments to mdx_ensemble_autotune");
   if (!e->have_state) return efail(e, MDX_ERR_STATE, "autotune needs a loaded state");
@@ -1940,13 +1967,22 @@ int mdx_ensemble_autotune(mdx_ensemble *e, int trial_steps, mdx_ensemble_config 
   if ((rc = read_ctl(e))) return rc;
   for (const auto &c : e->hctl)
     if (c.steps > 0) return efail(e, MDX_ERR_STATE, "autotune runs only before the first step");
-  const int tiles[2] = {64, 128};
+  const int tiles[2] = {512, 256};
   const int mults[3] = {1, 2, 4};
   double best_ms = 1e300;
   int bt = e->tile, bm = e->grid_mult;
-  for (int T : tiles)
+  for (int T : tiles) {
+    if (T != e->tile) {
+      e->tile = T;
+      for (auto &rp : e->hrep) rp.level = 0;
+      if ((rc = layout_level(e, tile_level(T)))) return rc;
+    }
+    // an untimed trial first: the capacities this target needs grow in it
+    e->grid_mult = mults[0];
+    if ((rc = size_step_grid(e)) || (rc = reset_state(e))) return rc;
+    e->graph_dirty = true;
+    if ((rc = mdx_ensemble_calculate(e, trial_steps, nullptr))) return rc;
     for (int gm : mults) {
-      if (T != e->tile && (rc = layout_tiles(e, T))) return rc;
       e->grid_mult = gm;
       if ((rc = size_step_grid(e)) || (rc = reset_state(e))) return rc;
       e->graph_dirty = true;
@@ -1958,7 +1994,12 @@ int mdx_ensemble_autotune(mdx_ensemble *e, int trial_steps, mdx_ensemble_config 
         bm = gm;
       }
     }
-  if (bt != e->tile && (rc = layout_tiles(e, bt))) return rc;
+  }
+  if (bt != e->tile) {
+    e->tile = bt;
+    for (auto &rp : e->hrep) rp.level = 0;
+    if ((rc = layout_level(e, tile_level(bt)))) return rc;
+  }
   e->grid_mult = bm;
   if ((rc = size_step_grid(e)) || (rc = reset_state(e))) return rc;
   e->graph_dirty = true;

@@ -588,14 +588,15 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles) {
 }
 __host__ __device__ inline int tile_grid(int ntiles) { return ((ntiles + 7) >> 3) << 3; }
 
-// tiles of each block of br x br rows, one workgroup per block
+// tiles of each block of br x br rows, one workgroup per block: each within
+// kBlk reals and cap staged particles
 __global__ void k_blk_count(const double4 *__restrict__ pos, Geom g, Cells c, int br, int target,
-                            int *__restrict__ blk_nt) {
+                            int cap, int *__restrict__ blk_nt) {
   __shared__ int hist[kMaxCellsX + 1];
   const int b = blockIdx.x, BY = blk_count(g, 1, br);
   const int by = b % BY, bz = b / BY;
   blk_hist(g, c, by, bz, br, hist);
-  const int nt = blk_ntiles(g, c, pos, hist, by, bz, br, target);
+  const int nt = blk_ntiles(g, c, pos, hist, by, bz, br, target, cap);
   if (threadIdx.x == 0) blk_nt[b] = nt;
 }
 
@@ -617,7 +618,7 @@ __global__ void k_blk_tiles(const double4 *__restrict__ pos, Geom g, Cells c, in
   for (int k = w; k < nt; k += (int)(blockDim.x >> 6)) {
     const int t = blk_toff[b] + k;
     int nreal;
-    const int S = blk_make_tile(g, c, pos, hist, by, bz, br, k, nt, tiles + t, info + t, &nreal);
+    const int S = blk_make_tile(g, c, pos, hist, by, bz, br, k, nt, 0, tiles + t, info + t, &nreal);
     if (lane == 0) {
       tile_S[t] = S;
       if (S > cap) atomicOr(flags, ST_STAGE_OVERFLOW);
@@ -1622,6 +1623,7 @@ struct mdx_system {
   size_t sidx_cap = 0;
   int *blk_nt = nullptr, *blk_toff = nullptr;  // [blk_cap] tiles of each block, their prefix
   int blk_cap = 0, blk_level = 0;   // block shape in use (kBlkLadder)
+  bool blk_stage_check = false;     // tile counts also bound the staging (after an overflow)
   int ntiles = 0, tiles_cap = 0, stage_max = 0;
   unsigned short *list16 = nullptr;     // ranked columns, as the list walkers read them
   unsigned short *list_nat = nullptr;   // the build's columns, in the tile's row order
@@ -1779,9 +1781,10 @@ int ensure_list_capacity(mdx_system *s, int kcap) {
 size_t stage_bytes(const mdx_system *s) { return (size_t)(s->stage_max + 1) * 3 * sizeof(double); }
 
 // the largest staging a tile may have: its sentinel's list entry fits 16
-// bits, and the staging fits the LDS beside the kernels' own (4 KiB)
+// bits, and the staging fits the LDS beside the kernels' own (the list
+// build's ranks and columns, 12 KiB)
 int stage_cap(const mdx_system *s) {
-  const long long lds = ((long long)s->lds_limit - 4096) / (3 * (long long)sizeof(double)) - 1;
+  const long long lds = ((long long)s->lds_limit - 12288) / (3 * (long long)sizeof(double)) - 1;
   return (int)std::min<long long>(kMaxPlane, lds);
 }
 
@@ -1919,8 +1922,10 @@ int build_pairlist(mdx_system *s) {
   std::swap(s->mom, s->mom_tmp);
   std::swap(s->tag, s->tag_tmp);
   if ((rc = launch_check(s, "sort"))) return rc;
-  // 5. tiles over blocks of cell rows, their staging sizes (a lower target
-  // of reals per tile when a staging would not fit), then the staging index
+  // 5. tiles over blocks of cell rows, their staging sizes, then the
+  // staging index; once a staging has not fitted, the tile counts bound the
+  // stagings too (more tiles per block), and smaller blocks follow if even
+  // that fails
   const Cells cells = cells_of(s);
   const int cap = stage_cap(s);
   for (;;) {
@@ -1928,7 +1933,7 @@ int build_pairlist(mdx_system *s) {
     const int br = shape.rows;
     const int nblocks = blk_count(s->geom, 1, br) * blk_count(s->geom, 2, br);
     k_blk_count<<<nblocks, kBlock, 0, s->stream>>>(s->pos, s->geom, cells, br, shape.target,
-                                                  s->blk_nt);
+                                                  s->blk_stage_check ? cap : INT_MAX, s->blk_nt);
     if ((rc = scan(s, s->blk_nt, s->blk_toff, nblocks, &s->dstat->ntiles))) return rc;
     if ((rc = sync_status(s))) return rc;
     const int ntiles = s->hstat->ntiles;
@@ -1955,6 +1960,10 @@ int build_pairlist(mdx_system *s) {
       s->ntiles = ntiles;
       s->stage_max = (int)s->hstat->stage_max;
       break;
+    }
+    if (!s->blk_stage_check) {   // first more tiles per block, then smaller blocks
+      s->blk_stage_check = true;
+      continue;
     }
     if (s->blk_level + 1 >= kBlkLevels)
       return fail(s, MDX_ERR_CAPACITY,

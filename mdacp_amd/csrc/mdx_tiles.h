@@ -24,6 +24,8 @@
 #ifndef MDX_TILES_H
 #define MDX_TILES_H
 
+#include <climits>
+
 #include "mdx_common.h"
 
 namespace mdxk {
@@ -199,18 +201,91 @@ __device__ inline int blk_cut_slot(const Geom &g, const int *hist, const double4
   return first + a;
 }
 
+// The cut of tile k of nt of block (by, bz), by one wavefront (hist filled
+// by blk_hist): real row r (lanes 0..8) gives the tile the slots between
+// its cuts.  Lanes 0..24 then cut the staged rows to the window [xa - h,
+// xb + h] of the tile's reals, h = SL (1 + 1e-10) + eps, wider than any
+// |dx| of a pair with r^2 <= SL^2 as rounded, eps covering the rounding of
+// positions near the box (the images' shift).
+struct TileCut {
+  int y0, z0, ny, nz;
+  int lo, len;       // lanes 0..8: the tile's slots of real row `lane`
+  int sp;            // lanes 0..9: reals of the rows before
+  int nreal;         // every lane
+  double xa, xb;     // every lane: x range of the tile's reals
+  RowRun run;        // lanes 0..24: staged row `lane`
+  int wlo, wlen;     // its window: first index in the run, length
+  int rs;            // lanes 0..25: staged particles of the rows before
+  int S;             // every lane: the staging size
+};
+__device__ inline void blk_cut_tile(const Geom &g, const Cells &c, const double4 *__restrict__ pos,
+                                    const int *hist, int by, int bz, int br, int k, int nt,
+                                    TileCut &t) {
+  const int lane = threadIdx.x & 63;
+  t.y0 = br * by + 1;
+  t.z0 = br * bz + 1;
+  t.ny = min(br, g.ms[1] - t.y0 + 1);
+  t.nz = min(br, g.ms[2] - t.z0 + 1);
+  const double h = sqrt(g.SL2) * (1.0 + 1e-10) + 1e-9 * (g.L[0] + g.L[1] + g.L[2]);
+  t.lo = 0;
+  t.len = 0;
+  if (lane < kRealRows) {
+    int rf, rm;
+    blk_row(g, c, by, bz, br, lane, &rf, &rm);
+    t.lo = blk_cut_slot(g, hist, pos, rf, rm, k, nt);
+    t.len = blk_cut_slot(g, hist, pos, rf, rm, k + 1, nt) - t.lo;
+  }
+  double xa = kFar, xb = -kFar;
+  if (t.len > 0) {
+    xa = pos[t.lo].x;
+    xb = pos[t.lo + t.len - 1].x;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    xa = fmin(xa, __shfl_xor(xa, off));
+    xb = fmax(xb, __shfl_xor(xb, off));
+  }
+  t.xa = xa;
+  t.xb = xb;
+  int sp = 0;
+  for (int r = 0; r < kRealRows; r++) {
+    const int lr = __shfl(t.len, r);
+    if (r < lane) sp += lr;
+  }
+  t.sp = sp;
+  t.nreal = __shfl(sp, kRealRows);   // lane 9 holds the total
+  t.wlo = 0;
+  t.wlen = 0;
+  t.run = RowRun{};
+  const int gy = lane % kSRow, gz = lane / kSRow;
+  if (lane < kSRows && gy < t.ny + 2 && gz < t.nz + 2 && t.nreal > 0) {
+    t.run = row_run(g, c, t.y0 - 1 + gy, t.z0 - 1 + gz);
+    t.wlo = run_bound<false>(t.run, pos, xa - h);
+    t.wlen = run_bound<true>(t.run, pos, xb + h) - t.wlo;
+  }
+  int rs = 0;
+  for (int r = 0; r < kSRows; r++) {
+    const int lr = __shfl(t.wlen, r);
+    if (r < lane) rs += lr;
+  }
+  t.rs = rs;
+  t.S = __shfl(rs, kSRows);   // lane 25 holds the total
+}
+
 // Tiles of block (by, bz), by one workgroup (hist filled by blk_hist): nt =
-// ceil(reals / target), raised until no tile holds more than kBlk reals
-// (the shared x cuts follow the block's x histogram, so a tile's count is
-// near M / nt); none for an empty block.  Every thread calls it and gets
-// nt; it ends with a barrier.
+// ceil(reals / target), raised until no tile holds more than kBlk reals and
+// none stages more than cap particles (the shared x cuts follow the block's
+// x histogram, so a tile's count is near M / nt, and a narrower x window
+// stages fewer; cap = INT_MAX skips that check); none for an empty block.
+// Every thread calls it and gets nt; it ends with a barrier.  A tile that
+// still overflows after the last raise is left to the tile writer to
+// report.
 __device__ inline int blk_ntiles(const Geom &g, const Cells &c, const double4 *__restrict__ pos,
-                                 const int *hist, int by, int bz, int br, int target) {
+                                 const int *hist, int by, int bz, int br, int target, int cap) {
   __shared__ int rfirst[kRealRows], rcnt[kRealRows];
   constexpr int kChunk = kBlock / kRealRows;   // tiles counted per round of kBlock threads
   __shared__ int len[kChunk][kRealRows];
   __shared__ int over;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, w = tid >> 6, nw = (int)(blockDim.x >> 6);
   if (tid < kRealRows) blk_row(g, c, by, bz, br, tid, &rfirst[tid], &rcnt[tid]);
   __syncthreads();
   const int M = hist[g.ms[0]];
@@ -231,6 +306,15 @@ __device__ inline int blk_ntiles(const Geom &g, const Cells &c, const double4 *_
       }
       __syncthreads();
     }
+    if (!over && cap < INT_MAX) {
+      // the stagings, one wavefront per tile
+      for (int k = w; k < nt; k += nw) {
+        TileCut tc;
+        blk_cut_tile(g, c, pos, hist, by, bz, br, k, nt, tc);
+        if ((tid & 63) == 0 && tc.S > cap) atomicOr(&over, 1);
+      }
+      __syncthreads();
+    }
     if (!over) break;
     nt += max(1, nt / 8);
     __syncthreads();
@@ -239,79 +323,37 @@ __device__ inline int blk_ntiles(const Geom &g, const Cells &c, const double4 *_
   return nt;
 }
 
-// Tile k of nt of block (by, bz), by one wavefront (hist filled): real row r
-// (lanes 0..8) gives the tile the slots between its cuts.  Lanes 0..24 then
-// cut the staged rows to the window [xa - h, xb + h] of the tile's reals, h
-// = SL (1 + 1e-10) + eps, wider than any |dx| of a pair with r^2 <= SL^2 as
-// rounded, eps covering the rounding of positions near the box (the
-// images' shift).  Writes *I, and *T with s0 = 0; returns the staging size
-// S (every lane).  *nreal gets the tile's reals.
+// Tile k of nt of block (by, bz), by one wavefront (hist filled): its cut
+// (blk_cut_tile) into *I, and *T with s0 = 0 and the given pad, in one
+// store; returns the staging size S (every lane).  *nreal gets the tile's
+// reals.
 __device__ inline int blk_make_tile(const Geom &g, const Cells &c, const double4 *__restrict__ pos,
                                     const int *hist, int by, int bz, int br, int k, int nt,
-                                    BlkTile *T, BlkInfo *I, int *nreal_out) {
+                                    int pad, BlkTile *T, BlkInfo *I, int *nreal_out) {
   const int lane = threadIdx.x & 63;
-  const int y0 = br * by + 1, z0 = br * bz + 1;
-  const int ny = min(br, g.ms[1] - y0 + 1), nz = min(br, g.ms[2] - z0 + 1);
-  const double h = sqrt(g.SL2) * (1.0 + 1e-10) + 1e-9 * (g.L[0] + g.L[1] + g.L[2]);
-  // slices of the real rows (lanes 0..8)
-  int lo = 0, len = 0;
-  if (lane < kRealRows) {
-    int rf, rm;
-    blk_row(g, c, by, bz, br, lane, &rf, &rm);
-    lo = blk_cut_slot(g, hist, pos, rf, rm, k, nt);
-    len = blk_cut_slot(g, hist, pos, rf, rm, k + 1, nt) - lo;
-  }
-  double xa = kFar, xb = -kFar;
-  if (len > 0) {
-    xa = pos[lo].x;
-    xb = pos[lo + len - 1].x;
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    xa = fmin(xa, __shfl_xor(xa, off));
-    xb = fmax(xb, __shfl_xor(xb, off));
-  }
-  // exclusive prefix of the slice lengths over lanes 0..8
-  int sp = 0;
-  for (int r = 0; r < kRealRows; r++) {
-    const int lr = __shfl(len, r);
-    if (r < lane) sp += lr;
-  }
-  const int nreal = __shfl(sp, kRealRows);   // lane 9 holds the total
-  // the window of staged row gr = lane (0..24)
-  int wlo = 0, wlen = 0;
-  RowRun run{};
-  const int gy = lane % kSRow, gz = lane / kSRow;
-  if (lane < kSRows && gy < ny + 2 && gz < nz + 2 && nreal > 0) {
-    run = row_run(g, c, y0 - 1 + gy, z0 - 1 + gz);
-    wlo = run_bound<false>(run, pos, xa - h);
-    wlen = run_bound<true>(run, pos, xb + h) - wlo;
-  }
-  int rs = 0;
-  for (int r = 0; r < kSRows; r++) {
-    const int lr = __shfl(wlen, r);
-    if (r < lane) rs += lr;
-  }
-  const int S = __shfl(rs, kSRows);   // lane 25 holds the total
+  TileCut t;
+  blk_cut_tile(g, c, pos, hist, by, bz, br, k, nt, t);
   if (lane == 0) {
-    *T = BlkTile{nreal, S, 0, 0};
-    I->y0 = y0;
-    I->z0 = z0;
-    I->ny = ny;
-    I->nz = nz;
-    I->xa = xa;
-    I->xb = xb;
+    *T = BlkTile{t.nreal, t.S, 0, pad};
+    I->y0 = t.y0;
+    I->z0 = t.z0;
+    I->ny = t.ny;
+    I->nz = t.nz;
+    I->xa = t.xa;
+    I->xb = t.xb;
   }
-  if (lane <= kRealRows) I->sp[lane] = sp;
-  if (lane < kRealRows) I->sfirst[lane] = lo;
-  if (lane <= kSRows) I->rs[lane] = rs;
-  if (lane < kSRows) I->rlo[lane] = wlo;
+  if (lane <= kRealRows) I->sp[lane] = t.sp;
+  if (lane < kRealRows) I->sfirst[lane] = t.lo;
+  if (lane <= kSRows) I->rs[lane] = t.rs;
+  if (lane < kSRows) I->rlo[lane] = t.wlo;
   // the real rows' self offsets: real row r sits in staged row (ry + 1) + 5 (rz + 1)
-  if (lane < kSRows && gy >= 1 && gy <= ny && gz >= 1 && gz <= nz) {
+  const int gy = lane % kSRow, gz = lane / kSRow;
+  if (lane < kSRows && gy >= 1 && gy <= t.ny && gz >= 1 && gz <= t.nz) {
     const int r = (gy - 1) + kBlkRows * (gz - 1);
-    I->selfoff[r] = rs + run.na - run.b0 - wlo;
+    I->selfoff[r] = t.rs + t.run.na - t.run.b0 - t.wlo;
   }
-  *nreal_out = nreal;
-  return S;
+  *nreal_out = t.nreal;
+  return t.S;
 }
 
 // The slot of every staged particle of a tile into sidx[0, S), by one

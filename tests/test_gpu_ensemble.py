@@ -1,6 +1,6 @@
 """Batched replicas (libmdx mdx_ensemble_*): every replica of an ensemble
 must end bit for bit where the single-system path (mdx_calculate) takes it
-alone, with the same number of pairlist constructions, whatever tile size
+alone, with the same number of pairlist constructions, whatever tile target
 and grid the autotuner picks; observables agree with mdx_observe.
 
 Replica sets follow the batched workload: FCC blocks with n ~ U{5..20}
@@ -58,7 +58,7 @@ def test_small_set_matches_single_system_bitwise():
 def test_tile_sizes_and_grids_give_identical_trajectories():
     states = small_set()
     outs = []
-    for tile, gm in ((64, 1), (128, 2), (64, 4)):
+    for tile, gm in ((256, 1), (512, 2), (256, 4)):
         with Ensemble(states) as e:
             e.set_config(tile, gm)
             e.calculate(120)
