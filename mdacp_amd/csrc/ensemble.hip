@@ -127,7 +127,6 @@ struct EnsDev {
   unsigned short *list;     // lists in ranked columns
   unsigned short *list_nat; // the build's columns
   int *nn;
-  int *nb;              // [NR] where planes 1 and 2 start in each list (16 + 16 bits)
   double *partial;      // [NT] max |p|^2 per tile
   double *disp;         // [NR + NI] squared displacement since the build
   int *foff;            // [kOffKinds][R + 1] prefix sums over the flagged / checked replicas
@@ -757,7 +756,7 @@ __global__ void __launch_bounds__(kBlk) k_e_build(EnsDev d) {
     const BlkTile t = d.btiles[s];
     const size_t region = rp.lbase + (size_t)k * rp.kcap * kBlk;
     const int cnt = build_tile_lists(d.qe, t, d.binfo + s, d.sidx + rp.sbase + t.s0, rp.g, rp.kcap,
-                                     d.list_nat + region, d.list + region, d.nn, d.nb,
+                                     d.list_nat + region, d.list + region, d.nn,
                                      d.treal + (size_t)s * kBlk, stage_mem);
     const int km = block_max_int(cnt);
     if (threadIdx.x == 0) d.tile_kmax[s] = km;
@@ -839,7 +838,7 @@ __global__ void __launch_bounds__(kBlk) k_e_step(EnsDev d, int mode) {
 #pragma unroll
     for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
     if (act) {
-      plane_groups<kBlk>(lst, 0, m, stage_mem, [&](int u, double xj, double yj, double zj) {
+      list_groups<kBlk>(lst, 0, m, stage_mem, [&](int u, double xj, double yj, double zj) {
         kick_term(xj, yj, zj, xi, yi, zi, d.pk, CL2, ax[u], ay[u], az[u]);
       });
     }
@@ -1216,7 +1215,7 @@ struct mdx_ensemble {
   unsigned short *list = nullptr, *list_nat = nullptr;
   size_t list_elems = 0, sidx_elems = 0;
   int NB = 0;              // per-block slots over all replicas
-  int *nn = nullptr, *nb = nullptr, *tile_kmax = nullptr;
+  int *nn = nullptr, *tile_kmax = nullptr;
   double *partial = nullptr, *obs_partial = nullptr, *obs_out = nullptr, *io = nullptr;
   bool have_state = false;
   hipGraph_t graph = nullptr;
@@ -1315,7 +1314,6 @@ EnsDev dev_view(const mdx_ensemble *e) {
   d.list = e->list;
   d.list_nat = e->list_nat;
   d.nn = e->nn;
-  d.nb = e->nb;
   d.partial = e->partial;
   d.obs_partial = e->obs_partial;
   d.tile_kmax = e->tile_kmax;
@@ -1360,7 +1358,7 @@ int layout(mdx_ensemble *e, const std::vector<Rep> *old) {
     rp.bcap = rp.g.ms[1] * rp.g.ms[2];
     tb += rp.tcap;
     lb += (size_t)rp.tcap * rp.kcap * kBlk;
-    sb += (size_t)rp.tcap * (kMaxPlane + 1);
+    sb += (size_t)rp.tcap * (kMaxStaged + 1);
     bb += rp.bcap;
   }
   if (tb > (size_t)0x7fffffff || bb > (size_t)0x7fffffff)
@@ -1386,7 +1384,7 @@ int layout(mdx_ensemble *e, const std::vector<Rep> *old) {
       EHIP(e, hipMemcpyAsync(ntr + (size_t)n.tbase * kBlk, e->treal + (size_t)o.tbase * kBlk, sizeof(int) * o.tcap * kBlk, hipMemcpyDeviceToDevice, e->stream));
       EHIP(e, hipMemcpyAsync(nts + n.tbase, e->tile_S + o.tbase, sizeof(int) * o.tcap, hipMemcpyDeviceToDevice, e->stream));
       EHIP(e, hipMemcpyAsync(nkm + n.tbase, e->tile_kmax + o.tbase, sizeof(int) * o.tcap, hipMemcpyDeviceToDevice, e->stream));
-      EHIP(e, hipMemcpyAsync(nsx + n.sbase, e->sidx + o.sbase, sizeof(int) * o.tcap * (kMaxPlane + 1), hipMemcpyDeviceToDevice, e->stream));
+      EHIP(e, hipMemcpyAsync(nsx + n.sbase, e->sidx + o.sbase, sizeof(int) * o.tcap * (kMaxStaged + 1), hipMemcpyDeviceToDevice, e->stream));
       EHIP(e, hipMemcpyAsync(nl + n.lbase, e->list + o.lbase, sizeof(unsigned short) * o.tcap * o.kcap * kBlk, hipMemcpyDeviceToDevice, e->stream));
     }
   }
@@ -1433,7 +1431,7 @@ int layout_level(mdx_ensemble *e, int level) {
 // it, the sentinel's list entry within 16 bits.
 int max_stage_cap(const mdx_ensemble *e) {
   const long long lds = ((long long)e->lds_limit - (long long)kBuildLds) / (3 * (long long)sizeof(double)) - 1;
-  return (int)std::min<long long>(kMaxPlane, lds);
+  return (int)std::min<long long>(kMaxStaged, lds);
 }
 
 // the masked rebuild after k_e_begin: bin, images, sort and gather, then
@@ -1699,7 +1697,7 @@ int mdx_ensemble_create(const mdx_params *prm, int nrep, const int *n, const dou
     cb += rp.ncell;
     // partners within SL at the mean density, with room for fluctuations
     const double mean = 4.18879 * SL * SL * SL * (n[r] / V);
-    e->kcap[r] = ((int)(mean * 1.4) + 24 + kPlanes * (kILP - 1) + 7) & ~7;
+    e->kcap[r] = ((int)(mean * 1.4) + 24 + (kILP - 1) + 7) & ~7;
     rp.level = 0;
   }
   e->NC = cb;
@@ -1738,7 +1736,7 @@ int mdx_ensemble_create(const mdx_params *prm, int nrep, const int *n, const dou
               (rc = ealloc(e, &e->cellid, tot)) || (rc = ealloc(e, &e->slot, tot)) ||
               (rc = ealloc(e, &e->order, tot)) || (rc = ealloc(e, &e->nimg, NR)) ||
               (rc = ealloc(e, &e->goff, NR)) || (rc = ealloc(e, &e->inv, NR)) ||
-              (rc = ealloc(e, &e->nn, NR)) || (rc = ealloc(e, &e->nb, NR)) ||
+              (rc = ealloc(e, &e->nn, NR)) ||
               (rc = ealloc(e, &e->cell_count, cb)) ||
               (rc = ealloc(e, &e->cnt_in, cb)) || (rc = ealloc(e, &e->cnt_out, cb)) ||
               (rc = ealloc(e, &e->scan_in, cb)) || (rc = ealloc(e, &e->scan_out, cb)) ||
@@ -1773,7 +1771,7 @@ int mdx_ensemble_destroy(mdx_ensemble *e) {
                   e->gsrc, e->gcode, e->gsrc_tmp, e->gcode_tmp, e->cellid, e->slot, e->nimg, e->goff,
                   e->order, e->inv, e->cell_count, e->cnt_in, e->cnt_out, e->scan_in, e->scan_out,
                   e->cell_start, e->btiles, e->binfo, e->treal, e->tile_S, e->sidx, e->blk_nt,
-                  e->blk_toff, e->list, e->list_nat, e->nn, e->nb, e->tile_kmax, e->partial,
+                  e->blk_toff, e->list, e->list_nat, e->nn, e->tile_kmax, e->partial,
                   e->obs_partial, e->obs_out, e->io};
   for (void *b : bufs)
     if (b) (void)hipFree(b);

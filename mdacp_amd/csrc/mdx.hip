@@ -646,8 +646,8 @@ __global__ void __launch_bounds__(kBlk) k_build_tiles(
     const double4 *__restrict__ pos, const BlkTile *__restrict__ tiles,
     const BlkInfo *__restrict__ info, const int *__restrict__ sidx, int ntiles, Geom g, int kcap,
     unsigned short *__restrict__ list_nat, unsigned short *__restrict__ list,
-    int *__restrict__ nn, int *__restrict__ nb, int *__restrict__ treal,
-    double *__restrict__ partial, int *__restrict__ flags) {
+    int *__restrict__ nn, int *__restrict__ treal, double *__restrict__ partial,
+    int *__restrict__ flags) {
   extern __shared__ double stage_mem[];
   const int tile = tile_of_block(blockIdx.x, ntiles);
   if (tile >= ntiles) {
@@ -657,7 +657,7 @@ __global__ void __launch_bounds__(kBlk) k_build_tiles(
   const BlkTile t = tiles[tile];
   const size_t region = (size_t)tile * kcap * kBlk;
   const int cnt = build_tile_lists(pos, t, info + tile, sidx + t.s0, g, kcap, list_nat + region,
-                                   list + region, nn, nb, treal + (size_t)tile * kBlk, stage_mem);
+                                   list + region, nn, treal + (size_t)tile * kBlk, stage_mem);
   if (cnt > kcap) atomicOr(flags, ST_LIST_OVERFLOW);
   block_max_to((double)cnt, partial);
 }
@@ -863,7 +863,7 @@ constexpr int kKickAhead = 1, kKickPrefetch = 2;
 
 // The force kick over a tile: one lane per list column, the ranked reals of
 // the tile; the tile's partners are staged once, and each lane walks its
-// whole list (plane_groups over [0, nn)).
+// whole list (list_groups over [0, nn)).
 //   MODE 0: p += F dt;  MODE 1: out = F (dt = 1);  MODE 2: p += F dt and
 //   out = q + p dt/2;  MODE 3 (batched step): as 2, plus the next mid-step
 //   position qn = out + p dt/2 and the heat bath's epilogue.
@@ -908,7 +908,7 @@ __global__ void __launch_bounds__(kBlk) k_kick_tiles(
   for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
   const double xi = qi.x, yi = qi.y, zi = qi.z;
   if (act) {
-    plane_groups<kBlk, kKickAhead, kKickPrefetch>(lst, 0, m, stage_mem,
+    list_groups<kBlk, kKickAhead, kKickPrefetch>(lst, 0, m, stage_mem,
                                                   [&](int u, double xj, double yj, double zj) {
       if (SCALED) {
         const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
@@ -1613,7 +1613,7 @@ struct mdx_system {
       *goff = nullptr;
   int *cell_count = nullptr, *cell_start = nullptr, *cnt_in = nullptr, *cnt_out = nullptr,
       *scan_in = nullptr, *scan_out = nullptr, *scan_aux = nullptr;
-  int *nn = nullptr, *nb = nullptr;   // list length; plane starts in it
+  int *nn = nullptr;   // padded list length
   // tiles over blocks of cell rows and their 16-bit lists
   BlkTile *tiles = nullptr;
   BlkInfo *tinfo = nullptr;
@@ -1724,7 +1724,6 @@ int ensure_capacity(mdx_system *s, int n) {
       (rc = dev_alloc(s, &s->tag, cap)) || (rc = dev_alloc(s, &s->tag_tmp, cap)) ||
       (rc = dev_alloc(s, &s->inv, cap)) || (rc = dev_alloc(s, &s->nimg, cap)) ||
       (rc = dev_alloc(s, &s->goff, cap)) || (rc = dev_alloc(s, &s->nn, cap)) ||
-      (rc = dev_alloc(s, &s->nb, cap)) ||
       (rc = dev_alloc(s, &s->io_buf, 3 * (size_t)cap)) || (rc = dev_alloc(s, &s->i_buf, cap)))
     return rc;
   s->cap = cap;
@@ -1785,7 +1784,7 @@ size_t stage_bytes(const mdx_system *s) { return (size_t)(s->stage_max + 1) * 3 
 // build's ranks and columns, 12 KiB)
 int stage_cap(const mdx_system *s) {
   const long long lds = ((long long)s->lds_limit - 12288) / (3 * (long long)sizeof(double)) - 1;
-  return (int)std::min<long long>(kMaxPlane, lds);
+  return (int)std::min<long long>(kMaxStaged, lds);
 }
 
 Cells cells_of(const mdx_system *s) {
@@ -1986,7 +1985,7 @@ int build_pairlist(mdx_system *s) {
     HIPCHK(s, hipMemsetAsync(&s->dstat->flags, 0, sizeof(int), s->stream));
     k_build_tiles<<<tile_grid(s->ntiles), kBlk, stage_bytes(s), s->stream>>>(
         s->pos, s->tiles, s->tinfo, s->sidx, s->ntiles, s->geom, s->kcap, s->list_nat, s->list16,
-        s->nn, s->nb, s->treal, s->pmax, &s->dstat->flags);
+        s->nn, s->treal, s->pmax, &s->dstat->flags);
     k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, tile_grid(s->ntiles), &s->dstat->kmax_d);
     if ((rc = launch_check(s, "list"))) return rc;
     if ((rc = sync_status(s))) return rc;
@@ -2569,7 +2568,7 @@ int mdx_destroy(mdx_system *s) {
                   s->pos_tmp, s->ref_pos, s->mom, s->mom_tmp, s->gpos_tmp, s->vec_out,
                   s->tag, s->tag_tmp, s->gsrc, s->gcode, s->gsrc_tmp, s->gcode_tmp, s->cellid,
                   s->slot, s->order, s->inv, s->nimg, s->goff, s->cell_count, s->cell_start,
-                  s->cnt_in, s->cnt_out, s->scan_in, s->scan_out, s->scan_aux, s->nn, s->nb,
+                  s->cnt_in, s->cnt_out, s->scan_in, s->scan_out, s->scan_aux, s->nn,
                   s->disp, s->partial, s->cand, s->io_buf, s->i_buf, s->dstat, s->dhb};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -2602,7 +2601,7 @@ int mdx_set_state(mdx_system *s, int n, const double *q3, const double *p3) {
   const int gguess = (int)std::min(8.0 * n, 2.0 * n * (Vx - V) / V + 4096);
   if ((rc = ensure_image_capacity(s, gguess))) return rc;
   const double rho = n / V;
-  const int kguess = (int)(4.18879 * s->SL * s->SL * s->SL * rho * 1.35) + 16 + kPlanes * (kILP - 1);
+  const int kguess = (int)(4.18879 * s->SL * s->SL * s->SL * rho * 1.35) + 16 + (kILP - 1);
   s->kcap = std::max((kguess + kILP - 1) / kILP * kILP, s->kcap);
   // positions staged in io_buf, momenta in vec_out (double4[cap] >= 3n doubles)
   double *pbuf = reinterpret_cast<double *>(s->vec_out);

@@ -63,119 +63,33 @@ __device__ __forceinline__ double shifted(double x, int s, double L) {
   return __fma_rn((double)s, L, x);
 }
 
-constexpr int kTile = 128;
-constexpr int kSeg = 18;
 constexpr int kILP = 8;   // independent partner chains per lane in the kick
 
-struct TileDesc {
-  int i0, ni, y, z, xf, xl, stage, pad;
-};
-
-__device__ __forceinline__ bool is_outer(const Geom &g, int x, int y, int z) {
-  return x == 0 || y == 0 || z == 0 || x == g.m[0] - 1 || y == g.m[1] - 1 || z == g.m[2] - 1;
-}
-
-struct TileSmem {
-  int seg_start[kSeg];
-  int seg_base[kSeg + 1];
-};
-
-// Copy one z-plane of a tile (segments 6p .. 6p+5, seg_base prefix sums
-// filled) into LDS as interleaved (x, y, z), positions relative to the
-// plane's first segment: staged particle a of segment k comes from load(k,
-// seg_start[k] + a - seg_base[k]).  The segments are walked as one index
-// space, four items per thread at a time.  The plane's six segment starts
-// and offsets sit in scalar registers, so an item's segment is found by
-// five comparisons with no LDS round trip; then all four global loads are
-// issued together, so their latencies overlap.  Items past the end load
-// the first real of the tile's own row (segment 8, never empty) and are not
-// stored, so no branch separates the loads.  Every thread calls it; it ends
-// with a barrier.
+// far beyond any cutoff: the staged sentinel after a tile's partners, which
+// padding entries of a list point at
 constexpr double kFar = 1.0e100;
-constexpr int kPlaneSeg = 6;
 
-template <class Load>
-__device__ __forceinline__ void stage_plane(const TileSmem &sm, int p, double *s3, Load load) {
-  constexpr int kBatch = 4;
-  const int k0 = kPlaneSeg * p;
-  int base[kPlaneSeg], delta[kPlaneSeg];
-#pragma unroll
-  for (int k = 0; k < kPlaneSeg; k++) {
-    base[k] = __builtin_amdgcn_readfirstlane(sm.seg_base[k0 + k]);
-    delta[k] = __builtin_amdgcn_readfirstlane(sm.seg_start[k0 + k] - sm.seg_base[k0 + k]);
-  }
-  const int b0 = base[0];
-  const int total = __builtin_amdgcn_readfirstlane(sm.seg_base[k0 + kPlaneSeg]) - b0;
-  const int pad_slot = __builtin_amdgcn_readfirstlane(sm.seg_start[8]);
-  for (int bb = 0; bb < total; bb += kBatch * (int)blockDim.x) {
-    int seg[kBatch], slot[kBatch];
-#pragma unroll
-    for (int j = 0; j < kBatch; j++) {
-      const int u = bb + j * (int)blockDim.x + (int)threadIdx.x;
-      const int a = b0 + u;
-      // the last segment starting at or before a (empty ones are passed)
-      int dk = delta[0], sk = k0;
-#pragma unroll
-      for (int k = 1; k < kPlaneSeg; k++) {
-        const bool up = a >= base[k];
-        dk = up ? delta[k] : dk;
-        sk = up ? k0 + k : sk;
-      }
-      const bool in = u < total;
-      seg[j] = in ? sk : 8;
-      slot[j] = in ? a + dk : pad_slot;
-    }
-    double4 q[kBatch];
-#pragma unroll
-    for (int j = 0; j < kBatch; j++) q[j] = load(seg[j], slot[j]);
-#pragma unroll
-    for (int j = 0; j < kBatch; j++) {
-      const int u = bb + j * (int)blockDim.x + (int)threadIdx.x;
-      if (u < total) {
-        s3[3 * u] = q[j].x;
-        s3[3 * u + 1] = q[j].y;
-        s3[3 * u + 2] = q[j].z;
-      }
-    }
-  }
-  // sentinel after the staged particles: a partner that far away is past
-  // any cutoff, so a masked list entry can point here
-  if (threadIdx.x == 0) s3[3 * total] = s3[3 * total + 1] = s3[3 * total + 2] = kFar;
-  __syncthreads();
-}
-
-// A tile's partners lie in three z-planes of cell rows (dz = -1, 0, +1):
-// segments 6p .. 6p+5, list rows 3p .. 3p+2.  Every kernel that walks the
-// lists stages one plane at a time, a third of the LDS of the whole tile.
+// A real's partners lie in three z-planes of cell rows (dz = -1, 0, +1),
+// which its list walks in order.
 constexpr int kPlanes = 3;
 
-// Plane lists.  A real's list holds its partners plane by plane; an entry
-// is the partner's byte offset in its plane's LDS staging (kEntryBytes x
-// its index there), so a kernel reads the partner at staging + entry with
-// no index arithmetic.  Each plane's entries are padded to a multiple of
-// kILP with the plane's sentinel (the slot after its last particle, which
-// holds kFar), so no group of kILP entries straddles two planes and none
-// needs a mask.  A plane may stage at most kMaxPlane particles: the
+// List entries.  An entry is the partner's byte offset in its tile's LDS
+// staging (kEntryBytes x its index there), so a kernel reads the partner
+// at staging + entry with no index arithmetic.  A list is padded at its
+// end to a multiple of kILP with the sentinel's offset (the slot after the
+// last staged particle, which holds kFar), so no group of kILP entries
+// needs a mask.  A tile may stage at most kMaxStaged particles: the
 // sentinel's offset must fit 16 bits.
 constexpr int kEntryBytes = 3 * (int)sizeof(double);
-constexpr int kMaxPlane = 65535 / kEntryBytes - 1;
+constexpr int kMaxStaged = 65535 / kEntryBytes - 1;
 
-__device__ __forceinline__ unsigned short plane_entry(int index_in_plane) {
-  return (unsigned short)(kEntryBytes * index_in_plane);
+__device__ __forceinline__ unsigned short staged_entry(int index) {
+  return (unsigned short)(kEntryBytes * index);
 }
 
-// xyz of the partner at entry e of a plane staged at s3
+// xyz of the partner at entry e of a tile staged at s3
 __device__ __forceinline__ const double *staged_at(const double *s3, unsigned e) {
   return reinterpret_cast<const double *>(reinterpret_cast<const char *>(s3) + e);
-}
-
-// Where plane p's entries of a list start and end, from the list length m
-// and nb = (start of plane 1) | (start of plane 2) << 16.
-__device__ __forceinline__ int plane_begin(int p, int nb) {
-  return p == 0 ? 0 : (p == 1 ? (nb & 0xffff) : (nb >> 16));
-}
-__device__ __forceinline__ int plane_end(int p, int nb, int m) {
-  return p == 0 ? (nb & 0xffff) : (p == 1 ? (nb >> 16) : m);
 }
 
 // Grouped list layout: a tile's list entries sit in groups of kILP per
@@ -203,8 +117,10 @@ struct GroupWriter {
     ++cnt;
     ptr += (cnt % kILP) ? 1 : S * kILP - (kILP - 1);
   }
-  __device__ __forceinline__ void pad_plane(int len) {
-    while (cnt % kILP) put(plane_entry(len));
+  // the list's end: the sentinel of a staging of len particles up to a
+  // whole group
+  __device__ __forceinline__ void pad_end(int len) {
+    while (cnt % kILP) put(staged_entry(len));
   }
 };
 
@@ -223,7 +139,7 @@ struct GroupWriter {
 // in the sums.
 constexpr int kPrefetch = 3;
 template <int S, int AHEAD = 1, int PF = kPrefetch, class Term>
-__device__ __forceinline__ void plane_groups(const unsigned short *__restrict__ lst, int kb, int ke,
+__device__ __forceinline__ void list_groups(const unsigned short *__restrict__ lst, int kb, int ke,
                                              const double *s3, Term term) {
   static_assert(AHEAD == 1 || AHEAD == 2, "partners are read one or two entries ahead");
   static_assert(PF >= 2, "the group after the one in hand must be in the ring");

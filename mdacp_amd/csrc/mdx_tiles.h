@@ -14,7 +14,7 @@
 // per tile, row after row (staged row gr = gy + 5 gz), as interleaved (x, y,
 // z); sidx holds the slot of every staged particle.  A neighbour list entry
 // is 24 x the partner's staging index (its byte offset), so a tile stages
-// at most kMaxPlane particles and the sentinel after them.
+// at most kMaxStaged particles and the sentinel after them.
 //
 // Against tiles of one cell row staged a z-plane at a time, a block tile
 // stages 4.5 instead of 11 particles per real at the same LDS per
@@ -388,7 +388,7 @@ __device__ inline void blk_write_sidx(const Geom &g, const Cells &c, const BlkIn
 template <int NT, class Load>
 __device__ __forceinline__ void stage_tile(int S, const int *__restrict__ sidx, double *s3,
                                            Load load) {
-  constexpr int kB = (kMaxPlane + NT) / NT;
+  constexpr int kB = (kMaxStaged + NT) / NT;
   for (int u0 = 0; u0 < S; u0 += kB * NT) {
     int sl[kB];
 #pragma unroll
@@ -447,7 +447,7 @@ __device__ __forceinline__ void range_partners(const double *s3, const double4 &
     }
 #pragma unroll
     for (int u = 0; u < kU; u++)
-      if (ok[u]) gw.put(plane_entry(o + u));
+      if (ok[u]) gw.put(staged_entry(o + u));
   }
 }
 
@@ -537,9 +537,8 @@ __device__ __forceinline__ int block_rank(bool act, int key, int (*cw)[kRankBins
 // its 27 cells with r^2 <= SL^2, r^2 formed as the reference's AVX2 search,
 // fma(dx,dx, fma(dy,dy, dz*dz)) with dq = q_j - q_i (meshlist.cc:354-362).
 // The partners go plane by plane (dz = -1, 0, +1), row by row (dy = -1, 0,
-// +1), in x order within a row; each plane's entries are padded with the
-// sentinel to a multiple of kILP.  nb[i] holds where planes 1 and 2 start
-// (low and high 16 bits), nn[i] the padded length.
+// +1), in x order within a row; the list is padded at its end with the
+// sentinel to a multiple of kILP, and nn[i] is the padded length.
 //
 // The lanes take the tile's reals grouped by their place in their cell's
 // y-z cross-section (4 x 4 classes), which sets the x windows of the
@@ -557,7 +556,7 @@ __device__ inline int build_tile_lists(const double4 *__restrict__ pos, const Bl
                                        const BlkInfo *I, const int *__restrict__ tsidx,
                                        const Geom &g, int kcap, unsigned short *__restrict__ lnat,
                                        unsigned short *__restrict__ lst, int *__restrict__ nn,
-                                       int *__restrict__ nb, int *__restrict__ treal,
+                                       int *__restrict__ treal,
                                        double *stage_mem) {
   __shared__ int keys[kBlk];
   __shared__ int rank_cw[kBlk / 64][kRankBins], rank_base[kRankBins];
@@ -593,9 +592,8 @@ __device__ inline int build_tile_lists(const double4 *__restrict__ pos, const Bl
   // rounding allowance of the cell faces and the binning, far above it
   const double eps = 1e-9 * (g.L[0] + g.L[1] + g.L[2]);
   GroupWriter<kBlk> gw(lnat + (size_t)lane * kILP, kcap);
-  int b1 = 0, b2 = 0;
   if (act) {
-    for (int p = 0; p < kPlanes; p++) {
+    for (int p = 0; p < kPlanes; p++)
       for (int dy = -1; dy <= 1; dy++) {
         const int gr = (ry + 1 + dy) + kSRow * (rz + p);
         double h;
@@ -604,16 +602,10 @@ __device__ inline int build_tile_lists(const double4 *__restrict__ pos, const Bl
         range_partners(stage_mem, qi, g.SL2, lo, x_bound<true>(stage_mem, lo, rs[gr + 1], qi.x + h),
                        self, gw);
       }
-      gw.pad_plane(t.S);
-      if (p == 0) b1 = gw.cnt;
-      if (p == 1) b2 = gw.cnt;
-    }
+    gw.pad_end(t.S);
   }
   const int cnt = gw.cnt, m = min(cnt, kcap);
-  if (act) {
-    nn[i] = m;
-    nb[i] = min(b1, kcap) | (min(b2, kcap) << 16);
-  }
+  if (act) nn[i] = m;
   // rank by list length: thread k ranks the real at place k, and the lane
   // that built it hands its column to the rank's column
   if (act) keys[src] = m;
