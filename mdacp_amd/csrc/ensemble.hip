@@ -743,7 +743,7 @@ __global__ void k_e_blk_sidx(EnsDev d) {
 // The neighbour lists of every tile of the flagged replicas
 // (build_tile_lists, as the single system builds them), one workgroup per
 // tile; the longest list of each tile to tile_kmax.
-__global__ void __launch_bounds__(kBlk) k_e_build(EnsDev d) {
+__global__ void __launch_bounds__(kBlk, 4) k_e_build(EnsDev d) {
   extern __shared__ double stage_mem[];
   __shared__ int soff[kOffLds];
   const OffView v = load_offsets(d, OFF_TILES, soff);
@@ -833,20 +833,21 @@ __global__ void __launch_bounds__(kBlk) k_e_step(EnsDev d, int mode) {
     const unsigned short *lst =
         d.list + rp.lbase + (size_t)(slot - rp.tbase) * rp.kcap * kBlk + (size_t)lane * kILP;
     const double CL2 = rp.g.CL2, xi = qi.x, yi = qi.y, zi = qi.z;
-    // the single-system kick's sums: entry k feeds chain k % kILP
-    double ax[kILP], ay[kILP], az[kILP];
+    // the single-system kick's sums: entry k feeds chain k % kChains
+    double ax[kChains], ay[kChains], az[kChains];
 #pragma unroll
-    for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
+    for (int u = 0; u < kChains; u++) ax[u] = ay[u] = az[u] = 0.0;
     if (act) {
       list_groups<kBlk>(lst, 0, m, stage_mem, [&](int u, double xj, double yj, double zj) {
-        kick_term(xj, yj, zj, xi, yi, zi, d.pk, CL2, ax[u], ay[u], az[u]);
+        kick_term(xj, yj, zj, xi, yi, zi, d.pk, CL2, ax[u % kChains], ay[u % kChains],
+                  az[u % kChains]);
       });
     }
     double v2 = 0.0;
     if (act) {
       double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll
-      for (int u = 0; u < kILP; u++) {
+      for (int u = 0; u < kChains; u++) {
         fx += ax[u];
         fy += ay[u];
         fz += az[u];
@@ -1323,9 +1324,6 @@ EnsDev dev_view(const mdx_ensemble *e) {
 // dynamic LDS of the tile kernels: a tile's staging plus the sentinel
 size_t stage_bytes(const mdx_ensemble *e) { return (size_t)(e->stage_cap + 1) * 3 * sizeof(double); }
 
-// LDS of the list build beside its staging (ranks, row starts, columns)
-constexpr size_t kBuildLds = 12 * 1024;
-
 int launch_check(mdx_ensemble *e, const char *what) {
   hipError_t er = hipGetLastError();
   if (er != hipSuccess) return efail(e, MDX_ERR_HIP, "launch %s: %s", what, hipGetErrorString(er));
@@ -1430,7 +1428,7 @@ int layout_level(mdx_ensemble *e, int level) {
 // The largest tile staging the LDS allows: the list build's own LDS beside
 // it, the sentinel's list entry within 16 bits.
 int max_stage_cap(const mdx_ensemble *e) {
-  const long long lds = ((long long)e->lds_limit - (long long)kBuildLds) / (3 * (long long)sizeof(double)) - 1;
+  const long long lds = ((long long)e->lds_limit - (long long)kBuildStaticLds) / (3 * (long long)sizeof(double)) - 1;
   return (int)std::min<long long>(kMaxStaged, lds);
 }
 

@@ -900,12 +900,13 @@ __global__ void __launch_bounds__(kBlk) k_kick_tiles(
   }
   stage_tile<kBlk>(t.S, sidx + t.s0, stage_mem, [&](int slot) { return pos[slot]; });
   const unsigned short *lst = list + (size_t)tile * kcap * kBlk + (size_t)lane * kILP;
-  // kILP independent partner chains per lane: entry k of the list feeds
-  // chain k % kILP, and the chains are summed in a fixed order at the end.
-  // A sentinel entry adds exactly +0.0 (the accumulators are never -0.0).
-  double ax[kILP], ay[kILP], az[kILP];
+  // kChains independent partner chains per lane: entry k of the list feeds
+  // chain k % kChains, and the chains are summed in a fixed order at the
+  // end.  A sentinel entry adds exactly +0.0 (the accumulators are never
+  // -0.0).
+  double ax[kChains], ay[kChains], az[kChains];
 #pragma unroll
-  for (int u = 0; u < kILP; u++) ax[u] = ay[u] = az[u] = 0.0;
+  for (int u = 0; u < kChains; u++) ax[u] = ay[u] = az[u] = 0.0;
   const double xi = qi.x, yi = qi.y, zi = qi.z;
   if (act) {
     list_groups<kBlk, kKickAhead, kKickPrefetch>(lst, 0, m, stage_mem,
@@ -913,11 +914,12 @@ __global__ void __launch_bounds__(kBlk) k_kick_tiles(
       if (SCALED) {
         const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
         const double df = pair_df_lj(dx * dx + dy * dy + dz * dz, lj, dt);
-        ax[u] += df * dx;
-        ay[u] += df * dy;
-        az[u] += df * dz;
+        ax[u % kChains] += df * dx;
+        ay[u % kChains] += df * dy;
+        az[u % kChains] += df * dz;
       } else {
-        kick_term(xj, yj, zj, xi, yi, zi, pk, CL2, ax[u], ay[u], az[u]);
+        kick_term(xj, yj, zj, xi, yi, zi, pk, CL2, ax[u % kChains], ay[u % kChains],
+                  az[u % kChains]);
       }
     });
   }
@@ -925,7 +927,7 @@ __global__ void __launch_bounds__(kBlk) k_kick_tiles(
   if (act) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll
-    for (int u = 0; u < kILP; u++) {
+    for (int u = 0; u < kChains; u++) {
       fx += ax[u];
       fy += ay[u];
       fz += az[u];
@@ -1781,9 +1783,10 @@ size_t stage_bytes(const mdx_system *s) { return (size_t)(s->stage_max + 1) * 3 
 
 // the largest staging a tile may have: its sentinel's list entry fits 16
 // bits, and the staging fits the LDS beside the kernels' own (the list
-// build's ranks and columns, 12 KiB)
+// build's, the largest)
 int stage_cap(const mdx_system *s) {
-  const long long lds = ((long long)s->lds_limit - 12288) / (3 * (long long)sizeof(double)) - 1;
+  const long long lds =
+      ((long long)s->lds_limit - kBuildStaticLds) / (3 * (long long)sizeof(double)) - 1;
   return (int)std::min<long long>(kMaxStaged, lds);
 }
 

@@ -63,7 +63,11 @@ __device__ __forceinline__ double shifted(double x, int s, double L) {
   return __fma_rn((double)s, L, x);
 }
 
-constexpr int kILP = 8;   // independent partner chains per lane in the kick
+constexpr int kILP = 8;   // list entries per group (one 16-byte load)
+// independent accumulator chains per lane of a kick: entry k of a list
+// feeds chain k % kChains, and the chains are summed in a fixed order
+constexpr int kChains = 4;
+static_assert(kILP % kChains == 0, "a group's entries map onto whole chains");
 
 // far beyond any cutoff: the staged sentinel after a tile's partners, which
 // padding entries of a list point at

@@ -451,19 +451,6 @@ __device__ __forceinline__ void range_partners(const double *s3, const double4 &
   }
 }
 
-// First staged candidate in [lo, hi) (candidate o at s3 + 3 o) whose x is
-// >= v (ABOVE: > v); the range is sorted by x.
-template <bool ABOVE>
-__device__ __forceinline__ int x_bound(const double *s3, int lo, int hi, double v) {
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    const double x = s3[3 * mid];
-    if (ABOVE ? x <= v : x < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
 // The x window of the cell row (dy, dz) around a real at qi of cell row
 // (y, z): every candidate of that row with list_r2 <= SL2 has |x - qi.x| <=
 // h.  gy, gz are lower bounds of its distance to the row in y and z (the
@@ -532,6 +519,52 @@ __device__ __forceinline__ int block_rank(bool act, int key, int (*cw)[kRankBins
   return act ? base[key] + cw[w][key] + tie : 0;
 }
 
+// Bucket starts of a tile's staged rows: the staging window [x0, x0 + E)
+// (x0 = xa - h of the tile, E its width) cut into kBuckets equal buckets;
+// btab[gr][b] is the first staged index of row gr whose x falls in bucket b
+// or later (the row's end if none), btab[gr][kBuckets] the row's end.  x
+// order within a row makes a bucket's particles one run, so the run of a
+// window [v0, v1] is contained in btab[gr][bucket(v0)] ..
+// btab[gr][bucket(v1) + 1]: two independent LDS reads in place of two
+// binary searches.  bucket() is monotonic in x, so the range only ever
+// holds more candidates than the window, never fewer.
+constexpr int kBuckets = 64;
+// LDS the list build takes beside the staging (ranks, columns, bucket
+// starts), rounded up: what a tile's staging may use is the rest
+constexpr int kBuildStaticLds = 20 * 1024;
+struct BucketMap {
+  double x0, inv;   // bucket(x) = floor((x - x0) inv), clamped
+  __device__ __forceinline__ int operator()(double x) const {
+    const double f = (x - x0) * inv;
+    return f <= 0.0 ? 0 : (f >= (double)(kBuckets - 1) ? kBuckets - 1 : (int)f);
+  }
+};
+
+// Fill btab for a tile staged at s3 with row starts rs (every thread calls
+// it; it ends with a barrier).
+__device__ inline void fill_buckets(const double *s3, const int *rs, const BucketMap &bk,
+                                    int (*btab)[kBuckets + 1]) {
+  const int S = rs[kSRows];
+  for (int e = threadIdx.x; e < kSRows * (kBuckets + 1); e += blockDim.x)
+    btab[e / (kBuckets + 1)][e % (kBuckets + 1)] = rs[e / (kBuckets + 1) + 1];
+  __syncthreads();
+  // particle u writes the buckets from its predecessor's (exclusive) to its
+  // own: each bucket gets the first particle at or past it
+  for (int u = threadIdx.x; u < S; u += blockDim.x) {
+    int lo = 0, hi = kSRows;   // the row holding u: rs[gr] <= u < rs[gr + 1]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (rs[mid] <= u) lo = mid;
+      else hi = mid;
+    }
+    const int gr = lo;
+    const int bu = bk(s3[3 * u]);
+    const int bp = u == rs[gr] ? -1 : bk(s3[3 * (u - 1)]);
+    for (int b = bp + 1; b <= bu; b++) btab[gr][b] = u;
+  }
+  __syncthreads();
+}
+
 // Full neighbour list of every real of a tile, by a workgroup of kBlk
 // threads (every thread calls it): every partner j != i (real or image) of
 // its 27 cells with r^2 <= SL^2, r^2 formed as the reference's AVX2 search,
@@ -562,6 +595,7 @@ __device__ inline int build_tile_lists(const double4 *__restrict__ pos, const Bl
   __shared__ int rank_cw[kBlk / 64][kRankBins], rank_base[kRankBins];
   __shared__ int rs[kSRows + 1];
   __shared__ int csrc[kBlk], glen[kBlk];
+  __shared__ int btab[kSRows][kBuckets + 1];
   const int lane = threadIdx.x;
   if (lane <= kSRows) rs[lane] = I->rs[lane];
   const bool act = lane < t.nreal;
@@ -591,6 +625,14 @@ __device__ inline int build_tile_lists(const double4 *__restrict__ pos, const Bl
   stage_tile<kBlk>(t.S, tsidx, stage_mem, [&](int slot) { return pos[slot]; });
   // rounding allowance of the cell faces and the binning, far above it
   const double eps = 1e-9 * (g.L[0] + g.L[1] + g.L[2]);
+  // the tile's staging window, as blk_cut_tile cut it, in kBuckets buckets
+  BucketMap bk{0.0, 0.0};
+  if (t.nreal > 0) {
+    const double ht = sqrt(g.SL2) * (1.0 + 1e-10) + eps;
+    bk.x0 = I->xa - ht;
+    bk.inv = (double)kBuckets / (I->xb + ht - bk.x0);
+  }
+  fill_buckets(stage_mem, rs, bk, btab);
   GroupWriter<kBlk> gw(lnat + (size_t)lane * kILP, kcap);
   if (act) {
     for (int p = 0; p < kPlanes; p++)
@@ -598,9 +640,8 @@ __device__ inline int build_tile_lists(const double4 *__restrict__ pos, const Bl
         const int gr = (ry + 1 + dy) + kSRow * (rz + p);
         double h;
         if (!row_window(g, qi, y, z, dy, p - 1, eps, &h)) continue;
-        const int lo = x_bound<false>(stage_mem, rs[gr], rs[gr + 1], qi.x - h);
-        range_partners(stage_mem, qi, g.SL2, lo, x_bound<true>(stage_mem, lo, rs[gr + 1], qi.x + h),
-                       self, gw);
+        const int lo = btab[gr][bk(qi.x - h)], hi = btab[gr][bk(qi.x + h) + 1];
+        range_partners(stage_mem, qi, g.SL2, lo, hi, self, gw);
       }
     gw.pad_end(t.S);
   }
