@@ -216,7 +216,9 @@ def batched(steps: int, plain_steps: int):
     states = W.replica_set()
     ntot = sum(s.n for s in states)
     e = Ensemble(states, device=0)
-    cfg = e.autotune(10)
+    # trials long enough to hold the first rebuilds (after about 35 steps
+    # from the lattice), whose branch the grid multiple trades against
+    cfg = e.autotune(50)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e.calculate(steps)

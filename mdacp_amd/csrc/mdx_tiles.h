@@ -290,6 +290,17 @@ __device__ inline int blk_ntiles(const Geom &g, const Cells &c, const double4 *_
   __syncthreads();
   const int M = hist[g.ms[0]];
   int nt = (M + target - 1) / target;
+  if (cap < INT_MAX && M > 0) {
+    // start from the count whose tiles would just fit the cap if the
+    // block's reals were spread evenly: (ny + 2)(nz + 2) staged rows of
+    // `line` particles per unit length over a tile's x extent T / (ny nz
+    // line) plus 2 h, so that the checks below seldom raise it again
+    const int ny = min(br, g.ms[1] - br * by), nz = min(br, g.ms[2] - br * bz);
+    const double line = (double)M / ((double)(ny * nz) * g.L[0]);
+    const double ext = 0.95 * (double)cap / ((double)((ny + 2) * (nz + 2)) * line) - 2.0 * sqrt(g.SL2);
+    const double tmax = ext * (double)(ny * nz) * line;   // reals of such a tile
+    if (tmax >= 16.0) nt = max(nt, (int)ceil((double)M / tmax));
+  }
   for (int pass = 0; pass < 64 && nt > 0; pass++) {
     if (tid == 0) over = 0;
     __syncthreads();
