@@ -452,7 +452,7 @@ class Ensemble:
     def set_config(self, tile: int, grid_mult: int = 2):
         self._chk(self.lib.mdx_ensemble_set_config(self.h, tile, grid_mult))
 
-    def autotune(self, trial_steps: int = 20):
+    def autotune(self, trial_steps: int = 50):
         c = MdxEnsembleConfig()
         self._chk(self.lib.mdx_ensemble_autotune(self.h, trial_steps, ctypes.byref(c)))
         return {k: getattr(c, k) for k, _ in MdxEnsembleConfig._fields_}
