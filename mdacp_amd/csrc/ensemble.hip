@@ -757,7 +757,7 @@ __global__ void __launch_bounds__(kBlk, 4) k_e_build(EnsDev d) {
     const size_t region = rp.lbase + (size_t)k * rp.kcap * kBlk;
     const int cnt = build_tile_lists(d.qe, t, d.binfo + s, d.sidx + rp.sbase + t.s0, rp.g, rp.kcap,
                                      d.list_nat + region, d.list + region, d.nn,
-                                     d.treal + (size_t)s * kBlk, stage_mem);
+                                     d.treal + (size_t)s * kBlk, nullptr, stage_mem);
     const int km = block_max_int(cnt);
     if (threadIdx.x == 0) d.tile_kmax[s] = km;
   }

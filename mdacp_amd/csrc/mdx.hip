@@ -646,8 +646,8 @@ __global__ void __launch_bounds__(kBlk) k_build_tiles(
     const double4 *__restrict__ pos, const BlkTile *__restrict__ tiles,
     const BlkInfo *__restrict__ info, const int *__restrict__ sidx, int ntiles, Geom g, int kcap,
     unsigned short *__restrict__ list_nat, unsigned short *__restrict__ list,
-    int *__restrict__ nn, int *__restrict__ treal, double *__restrict__ partial,
-    int *__restrict__ flags) {
+    int *__restrict__ nn, int *__restrict__ treal, unsigned *__restrict__ tmeta,
+    double *__restrict__ partial, int *__restrict__ flags) {
   extern __shared__ double stage_mem[];
   const int tile = tile_of_block(blockIdx.x, ntiles);
   if (tile >= ntiles) {
@@ -657,7 +657,8 @@ __global__ void __launch_bounds__(kBlk) k_build_tiles(
   const BlkTile t = tiles[tile];
   const size_t region = (size_t)tile * kcap * kBlk;
   const int cnt = build_tile_lists(pos, t, info + tile, sidx + t.s0, g, kcap, list_nat + region,
-                                   list + region, nn, treal + (size_t)tile * kBlk, stage_mem);
+                                   list + region, nn, treal + (size_t)tile * kBlk,
+                                   tmeta + (size_t)tile * kBlk, stage_mem);
   if (cnt > kcap) atomicOr(flags, ST_LIST_OVERFLOW);
   block_max_to((double)cnt, partial);
 }
@@ -872,7 +873,7 @@ template <int MODE, bool SCALED = false>
 __global__ void __launch_bounds__(kBlk) k_kick_tiles(
     const double4 *__restrict__ pos, double4 *__restrict__ mom, const BlkTile *__restrict__ tiles,
     int ntiles, const int *__restrict__ treal, const int *__restrict__ sidx, int kcap,
-    const unsigned short *__restrict__ list, const int *__restrict__ nn, PairK pk, double dt,
+    const unsigned short *__restrict__ list, const unsigned *__restrict__ tmeta, PairK pk, double dt,
     double CL2, double4 *__restrict__ out, double *__restrict__ partial, LJ lj = LJ{},
     const int *__restrict__ halt = nullptr, double4 *__restrict__ qn = nullptr,
     HeatKick hk = HeatKick{}) {
@@ -889,16 +890,20 @@ __global__ void __launch_bounds__(kBlk) k_kick_tiles(
   const BlkTile t = tiles[tile];
   const int lane = threadIdx.x;
   const bool act = col_active(lane, t.nreal);
-  const int i = act ? treal[(size_t)tile * kBlk + lane] : 0;   // list column -> real
-  // the lane's own loads go out before the staging, which hides them
-  double4 qi = make_double4(0.0, 0.0, 0.0, 0.0), pm = qi;
-  int m = 0;
-  if (act) {
-    qi = pos[i];
-    m = nn[i];
-    if (MODE != 1) pm = mom[i];
-  }
+  // list column -> real, and the real's staging index and list length:
+  // loads of consecutive columns, where the real's own record would be a
+  // gather (the columns hold the tile's reals ranked by list length)
+  const size_t c = (size_t)tile * kBlk + lane;
+  const int i = act ? treal[c] : 0;
+  const unsigned meta = act ? tmeta[c] : 0u;
+  const int m = (int)(meta >> 16), self = (int)(meta & 0xffffu);
+  // the momentum goes out before the staging, which hides it
+  double4 pm = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (act && MODE != 1) pm = mom[i];
   stage_tile<kBlk>(t.S, sidx + t.s0, stage_mem, [&](int slot) { return pos[slot]; });
+  // the real's position from its staged copy (pos[i] bit for bit; w is 0)
+  double4 qi = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (act) qi = make_double4(stage_mem[3 * self], stage_mem[3 * self + 1], stage_mem[3 * self + 2], 0.0);
   const unsigned short *lst = list + (size_t)tile * kcap * kBlk + (size_t)lane * kILP;
   // kChains independent partner chains per lane: entry k of the list feeds
   // chain k % kChains, and the chains are summed in a fixed order at the
@@ -1621,6 +1626,7 @@ struct mdx_system {
   BlkInfo *tinfo = nullptr;
   int *treal = nullptr;                        // [tiles_cap][kBlk] real of each ranked column
   int *tile_S = nullptr, *tile_s0 = nullptr;   // [tiles_cap] staging sizes, their prefix sums
+  unsigned *tmeta = nullptr;   // [tiles_cap][kBlk] per column: staging index | list length << 16
   int *sidx = nullptr;                         // global slot of every staged particle
   size_t sidx_cap = 0;
   int *blk_nt = nullptr, *blk_toff = nullptr;  // [blk_cap] tiles of each block, their prefix
@@ -1942,7 +1948,8 @@ int build_pairlist(mdx_system *s) {
     if (ntiles > s->tiles_cap) {
       const size_t tc = (size_t)ntiles + ntiles / 8 + 64;
       if ((rc = dev_alloc(s, &s->tiles, tc)) || (rc = dev_alloc(s, &s->tinfo, tc)) ||
-          (rc = dev_alloc(s, &s->treal, tc * kBlk)) || (rc = dev_alloc(s, &s->tile_S, tc)) ||
+          (rc = dev_alloc(s, &s->treal, tc * kBlk)) || (rc = dev_alloc(s, &s->tmeta, tc * kBlk)) ||
+          (rc = dev_alloc(s, &s->tile_S, tc)) ||
           (rc = dev_alloc(s, &s->tile_s0, tc)))
         return rc;
       s->tiles_cap = (int)tc;
@@ -1988,7 +1995,7 @@ int build_pairlist(mdx_system *s) {
     HIPCHK(s, hipMemsetAsync(&s->dstat->flags, 0, sizeof(int), s->stream));
     k_build_tiles<<<tile_grid(s->ntiles), kBlk, stage_bytes(s), s->stream>>>(
         s->pos, s->tiles, s->tinfo, s->sidx, s->ntiles, s->geom, s->kcap, s->list_nat, s->list16,
-        s->nn, s->treal, s->pmax, &s->dstat->flags);
+        s->nn, s->treal, s->tmeta, s->pmax, &s->dstat->flags);
     k_max_partials<<<1, kBlock, 0, s->stream>>>(s->pmax, tile_grid(s->ntiles), &s->dstat->kmax_d);
     if ((rc = launch_check(s, "list"))) return rc;
     if ((rc = sync_status(s))) return rc;
@@ -2013,7 +2020,7 @@ int kick(mdx_system *s) {
   const int grid = tile_grid(s->ntiles);
   ev_record(s, 2);
   k_kick_tiles<0><<<grid, kBlk, stage_bytes(s), s->stream>>>(
-      s->pos, s->mom, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->nn,
+      s->pos, s->mom, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->tmeta,
       make_pairk(s->C2, s->prm.dt), s->prm.dt, s->geom.CL2, nullptr, s->pmax);
   ev_record(s, 3);
   s->vmax_valid = false;   // recomputed from p when the expiry check needs it
@@ -2102,7 +2109,7 @@ int run_batch(mdx_system *s, int want, int first_checked, int *ran) {
     if (s->timing) (void)hipEventRecord(s->evk[2 * b], s->stream);
     hk.k = s->langevin_k + (unsigned long long)b;   // Langevin kicks since mdx_set_state
     k_kick_tiles<3><<<grid, kBlk, stage_bytes(s), s->stream>>>(
-        cur, s->mom, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->nn,
+        cur, s->mom, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->tmeta,
         make_pairk(s->C2, dt), dt, s->geom.CL2, s->pos, s->pmax, LJ{}, &s->dctl->halt, nxt, hk);
     if (s->timing) (void)hipEventRecord(s->evk[2 * b + 1], s->stream);
     const int refresh_blocks = (g > 0 && b + 1 < B) ? (g + kGuard - 1) / kGuard : 0;
@@ -2285,7 +2292,7 @@ int lj_step(mdx_system *s, const LJ &lj, double4 *mid) {
                                                           s->prm.dt * 0.5, s->geom.L[0],
                                                           s->geom.L[1], s->geom.L[2], mid, nullptr);
   k_kick_tiles<2, true><<<grid, kBlk, stage_bytes(s), s->stream>>>(
-      mid, s->mom, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->nn,
+      mid, s->mom, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->tmeta,
       make_pairk(s->C2, s->prm.dt), s->prm.dt, s->geom.CL2, s->pos, s->pmax, lj);
   return launch_check(s, "lj step");
 }
@@ -2566,7 +2573,7 @@ int mdx_create(const mdx_params *prm, mdx_system **out) {
 int mdx_destroy(mdx_system *s) {
   if (!s) return MDX_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  void *bufs[] = {s->pos, s->pos_alt, s->pos_mid, s->tinfo, s->treal, s->tile_S, s->tile_s0, s->sidx,
+  void *bufs[] = {s->pos, s->pos_alt, s->pos_mid, s->tinfo, s->treal, s->tmeta, s->tile_S, s->tile_s0, s->sidx,
                   s->blk_nt, s->blk_toff, s->pmax, s->tiles, s->list16, s->list_nat,
                   s->pos_tmp, s->ref_pos, s->mom, s->mom_tmp, s->gpos_tmp, s->vec_out,
                   s->tag, s->tag_tmp, s->gsrc, s->gcode, s->gsrc_tmp, s->gcode_tmp, s->cellid,
@@ -2848,7 +2855,7 @@ int mdx_forces(mdx_system *s, double *f3) {
   if (!s || !f3) return fail(s, MDX_ERR_ARG, "null argument");
   if (!s->list_valid) return fail(s, MDX_ERR_STATE, "no pairlist");
   k_kick_tiles<1><<<tile_grid(s->ntiles), kBlk, stage_bytes(s), s->stream>>>(
-      s->pos, s->mom, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->nn,
+      s->pos, s->mom, s->tiles, s->ntiles, s->treal, s->sidx, s->kcap, s->list16, s->tmeta,
       make_pairk(s->C2, 1.0), 1.0, s->geom.CL2, s->vec_out, nullptr);
   k_scatter_by_tag<<<nblk(s->n), kBlock, 0, s->stream>>>(s->vec_out, s->tag, s->n, s->io_buf);
   HIPCHK(s, hipMemcpyAsync(f3, s->io_buf, sizeof(double) * 3 * s->n, hipMemcpyDeviceToHost, s->stream));

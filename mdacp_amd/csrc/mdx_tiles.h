@@ -592,7 +592,8 @@ __device__ inline void fill_buckets(const double *s3, const int *rs, const Bucke
 // ranked by list length, longest first, ties by the real's place in the
 // tile's rows (slot order, so that a wavefront of the kick loads and stores
 // runs of consecutive reals), and each column of the tile's region (list)
-// takes its rank's list; treal gets the real of each column.  pos holds
+// takes its rank's list; treal gets the real of each column and tmeta (if
+// given) its staging index and list length (low and high 16 bits).  pos holds
 // every staged particle at its slot; stage_mem is the workgroup's dynamic
 // LDS, at least 24 (S + 1) bytes.  Returns the thread's list length (before
 // the kcap cut; 0 for a lane without a real).
@@ -601,7 +602,7 @@ __device__ inline int build_tile_lists(const double4 *__restrict__ pos, const Bl
                                        const Geom &g, int kcap, unsigned short *__restrict__ lnat,
                                        unsigned short *__restrict__ lst, int *__restrict__ nn,
                                        int *__restrict__ treal,
-                                       double *stage_mem) {
+                                       unsigned *__restrict__ tmeta, double *stage_mem) {
   __shared__ int keys[kBlk];
   __shared__ int rank_cw[kBlk / 64][kRankBins], rank_base[kRankBins];
   __shared__ int rs[kSRows + 1];
@@ -684,6 +685,7 @@ __device__ inline int build_tile_lists(const double4 *__restrict__ pos, const Bl
   csrc[col] = lane;
   glen[col] = act ? m / kILP : 0;
   if (act) treal[col] = i;
+  if (act && tmeta) tmeta[col] = (unsigned)self | ((unsigned)m << 16);
   __syncthreads();
   {
     const uint4 *from = reinterpret_cast<const uint4 *>(lnat) + csrc[lane];
