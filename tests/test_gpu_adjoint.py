@@ -2,7 +2,7 @@
 mdx_lj_trajectory) against a dense FP64 CPU autograd reference at N=500
 and against central finite differences at N=32,000.
 
-Loss (north-star form): (mean P - 1.5)^2 + (mean PE/N + 4)^2 over the last
+Loss: (mean P - 1.5)^2 + (mean PE/N + 4)^2 over the last
 100 of 200 drift-kick-drift steps, checkpoints every 20 steps; gradients
 with respect to eps, sigma and the initial positions.
 """
